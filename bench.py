@@ -1,0 +1,149 @@
+"""Benchmark: device-resident MD with pair_style tensormd on one MI355X.
+
+Workload (the production-scale configuration): 4-element BCC random solid
+solution, a = 3.2 A, 40x40x40 cells = 128,000 atoms, Gaussian jitter
+sigma = 0.08 A, Maxwell-Boltzmann velocities at 600 K, random-init model
+(cutoff 6.5 A, 24 radial functions, max moment 3, per-element MLP
+256-128-64-1 tanh), velocity-Verlet at dt = 1 fs with device neighbor
+rebuilds (skin 1 A).  A step = velocity Verlet + neighbor check/rebuild +
+descriptor + MLP forward/backward + forces/virial + FP64 energy/virial sums.
+
+`value` is the BF16-MLP path's atom-steps/s; the plain FP32 path is timed in
+the same run (`fp32_atom_steps_per_s`).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lammps-tensormd_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+PEAK = {"hbm_gbs": 8000.0, "fp32_tflops": 157.3, "bf16_tflops": 2500.0}
+
+
+def mlp_flops_per_atom(sizes):
+    """Forward + input-gradient FLOPs: 4 * sum n_l n_{l+1}."""
+    return 4 * sum(a * b for a, b in zip(sizes[:-1], sizes[1:]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cells", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from tensormd_amd import TensorMD, workloads as W
+
+    symbols = ["Mo", "Nb", "Ta", "W"]
+    masses = [95.95, 92.906, 180.95, 183.84]
+    sizes = [384, 256, 128, 64, 1]
+    tmp = tempfile.mkdtemp()
+    model = W.write_model(os.path.join(tmp, "c2.npz"), symbols=symbols, masses=masses, rmax=6.5,
+                          num_filters=24, layer_sizes=sizes[1:], actfn="tanh", seed=2024,
+                          radial_scale=0.2)
+    n = args.cells
+    x, t, cell = W.crystal("bcc", 3.2, (n, n, n), nelt=4, sigma=0.08, seed=7)
+    natoms = len(x)
+    v = W.maxwell_boltzmann(np.array(masses)[t - 1], 600.0, seed=8)
+    print(f"bench: {natoms} atoms, steps={args.steps} warmup={args.warmup}", flush=True)
+
+    res = {}
+    prof = {}
+    for prec in ["fp32", "bf16"]:
+        m = TensorMD(model, symbols, prec, 0.001, device=0)
+        s = m.system(x, t, cell, skin=1.0)
+        s.set_velocities(v)
+        s.compute()
+        s.md_run(args.warmup, 0.001, history=False)
+        s.get()  # synchronise
+        t0 = time.perf_counter()
+        s.md_run(args.steps, 0.001, history=False)
+        out = s.get()  # synchronises the stream
+        dt = time.perf_counter() - t0
+        res[prec] = dict(time=dt, rate=natoms * args.steps / dt, energy=out["energy"],
+                         builds=s.nbuilds)
+        prof[prec] = s.profile(args.profile_steps, 0.001)
+        print(f"bench: {prec}: {res[prec]['rate'] / 1e6:.2f} M atom-steps/s, "
+              f"{1e3 * dt / args.steps:.3f} ms/step, builds={s.nbuilds}, phases(ms)={prof[prec]}",
+              flush=True)
+        s.close()
+        m.close()
+
+    # roofline of the dominant phase of the headline (bf16) path
+    p = prof["bf16"]
+    phases = {k: p[k] for k in ["neighbor", "descriptor", "mlp", "force", "reduce", "integrate"]}
+    dom = max(phases, key=phases.get)
+    if dom == "mlp":
+        ach = mlp_flops_per_atom(sizes) * natoms / (p["mlp"] * 1e-3) / 1e12
+        roof = dict(kernel="mlp (bf16 MFMA GEMMs)", bound="mfma", achieved=ach,
+                    peak=PEAK["bf16_tflops"], unit="TFLOP/s")
+    else:
+        # algorithmic bytes per atom: neighbor list within rc (~64 x 4 B) + x,type
+        # (28 B) + descriptor row (1536 B) write or dE/dG row read (1536 B) + forces
+        per_atom = {"descriptor": 64 * 4 + 28 + 1536, "force": 64 * 4 + 28 + 1536 + 24,
+                    "neighbor": 112 * 4 + 32, "reduce": 56, "integrate": 112}[dom]
+        ach = per_atom * natoms / (p[dom] * 1e-3) / 1e9
+        roof = dict(kernel=dom, bound="hbm", achieved=ach, peak=PEAK["hbm_gbs"], unit="GB/s")
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["phases_ms"] = phases
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        om = oracle.OracleModel(model, 0.001)
+        t0 = time.perf_counter()
+        nb = oracle.neighbors(x, cell, om.rc)
+        om.compute_periodic(x, t - 1, cell, nbr=nb)
+        dtc = time.perf_counter() - t0
+        cpu = dict(value=natoms / dtc, unit="atom-steps/s", cores=oracle.num_threads(),
+                   kind="port",
+                   sample=f"one force evaluation (neighbor build + descriptor + MLP + forces) "
+                          f"of the full {natoms}-atom structure by the FP64 C++/OpenMP "
+                          f"restatement, {dtc:.1f} s, OMP threads={threads}")
+
+    bf, fp = res["bf16"], res["fp32"]
+    line = {
+        "metric": "atom-steps/s at 128k atoms BF16 vs FP32 GPU; p99 latency of batched E/F requests",
+        "value": bf["rate"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * bf["time"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded jittered BCC alloy, random-init model)",
+        "config": {"workload": f"{natoms}-atom 4-element BCC NVE, 256-128-64-1 tanh MLP, "
+                               f"rc 6.5, k 24, BF16 MLP / FP32 descriptors",
+                   "natoms": natoms, "dt_fs": 1.0, "skin": 1.0},
+        "fp32_atom_steps_per_s": fp["rate"],
+        "fp32_ms_per_step": 1e3 * fp["time"] / args.steps,
+        "bf16_over_fp32": bf["rate"] / fp["rate"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

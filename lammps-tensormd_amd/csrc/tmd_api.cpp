@@ -1,0 +1,496 @@
+// C ABI implementation: contexts (model + device parameters), the
+// Pair::compute drop-in over LAMMPS neighbor lists, and device-resident
+// periodic systems with velocity-Verlet.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/tensormd_amd.h"
+#include "tmd_device.h"
+#include "tmd_internal.h"
+#include "tmd_model.h"
+
+using namespace tmd;
+
+namespace {
+thread_local std::string g_thread_err;
+}
+
+namespace tmd {
+
+void set_err(tmd_ctx *ctx, const std::string &m)
+{
+  g_thread_err = m;
+  if (ctx) ctx->err = m;
+}
+
+int hip_fail(tmd_ctx *ctx, hipError_t e, const char *what)
+{
+  set_err(ctx, std::string(what) + ": " + hipGetErrorString(e));
+  return TMD_ERR_DEVICE;
+}
+
+DeviceBuffer::~DeviceBuffer() { release(); }
+
+void DeviceBuffer::release()
+{
+  if (p) hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+hipError_t DeviceBuffer::ensure(size_t nbytes)
+{
+  if (nbytes <= bytes && p) return hipSuccess;
+  release();
+  size_t want = std::max<size_t>(nbytes, 256);
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return e;
+  }
+  bytes = want;
+  return hipSuccess;
+}
+
+static uint16_t bf16_bits(float f)
+{
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t) (u >> 16);
+}
+
+// Upload table and MLP parameters.
+int upload_model(tmd_ctx *ctx)
+{
+  const Model &m = ctx->model;
+  ModelGeo &g = ctx->geo;
+  std::memset(&g, 0, sizeof(g));
+  g.nelt = m.nelt;
+  g.k = m.k;
+  g.tab_n = m.tab_n;
+  for (int a = 0; a < m.nelt; a++)
+    for (int b = 0; b < m.nelt; b++) g.eltij[a][b] = m.eltij[a][b];
+  for (int mm = 0; mm < kNM; mm++)
+    for (int d = 0; d < kND; d++) g.T[mm][d] = (float) m.T[mm * m.nd + d];
+  g.rc2 = m.rc * m.rc;
+  g.dx = m.dx;
+  g.xdx = m.xdx;
+  g.dxf = (float) m.dx;
+  // table as float4 [n][k]
+  std::vector<float> tabf(m.tab.size());
+  for (size_t i = 0; i < tabf.size(); i++) tabf[i] = (float) m.tab[i];
+  hipError_t e = ctx->tab.ensure(tabf.size() * 4);
+  if (e != hipSuccess) return hip_fail(ctx, e, "table alloc");
+  e = hipMemcpy(ctx->tab.p, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(ctx, e, "table upload");
+
+  // MLP: per layer, all elements packed
+  const int L = m.mlp[0].nlayers;
+  ctx->layers.assign(L, LayerParams{});
+  size_t total = 0;
+  struct Off {
+    size_t W, Wt, Wb, Wtb, b;
+  };
+  std::vector<Off> offs(L);
+  auto al = [](size_t x) { return (x + 63) & ~(size_t) 63; };
+  for (int l = 0; l < L; l++) {
+    size_t K = m.mlp[0].sizes[l], N = m.mlp[0].sizes[l + 1];
+    size_t per = K * N;
+    offs[l].W = total;
+    total = al(total + per * m.nelt * 4);
+    offs[l].Wt = total;
+    total = al(total + per * m.nelt * 4);
+    offs[l].Wb = total;
+    total = al(total + per * m.nelt * 2);
+    offs[l].Wtb = total;
+    total = al(total + per * m.nelt * 2);
+    offs[l].b = total;
+    total = al(total + N * m.nelt * 4);
+  }
+  e = ctx->params.ensure(total);
+  if (e != hipSuccess) return hip_fail(ctx, e, "parameter alloc");
+  std::vector<unsigned char> host(total, 0);
+  for (int l = 0; l < L; l++) {
+    int K = m.mlp[0].sizes[l], N = m.mlp[0].sizes[l + 1];
+    size_t per = (size_t) K * N;
+    float *W = (float *) &host[offs[l].W];
+    float *Wt = (float *) &host[offs[l].Wt];
+    uint16_t *Wb = (uint16_t *) &host[offs[l].Wb];
+    uint16_t *Wtb = (uint16_t *) &host[offs[l].Wtb];
+    float *b = (float *) &host[offs[l].b];
+    bool hasb = false;
+    for (int el = 0; el < m.nelt; el++) {
+      const DenseNet &net = m.mlp[el];
+      for (int i = 0; i < K; i++)
+        for (int j = 0; j < N; j++) {
+          float w = (float) net.W[l][(size_t) i * N + j];
+          W[el * per + (size_t) i * N + j] = w;
+          Wt[el * per + (size_t) j * K + i] = w;
+          Wb[el * per + (size_t) i * N + j] = bf16_bits(w);
+          Wtb[el * per + (size_t) j * K + i] = bf16_bits(w);
+        }
+      if (!net.B[l].empty()) {
+        hasb = true;
+        for (int j = 0; j < N; j++) b[el * N + j] = (float) net.B[l][j];
+      }
+    }
+    LayerParams &lp = ctx->layers[l];
+    lp.K = K;
+    lp.N = N;
+    lp.act = m.mlp[0].act;
+    lp.res = m.mlp[0].res_layer(l) ? 1 : 0;
+    unsigned char *base = (unsigned char *) ctx->params.p;
+    lp.W = (const float *) (base + offs[l].W);
+    lp.Wt = (const float *) (base + offs[l].Wt);
+    lp.Wb = (const uint16_t *) (base + offs[l].Wb);
+    lp.Wtb = (const uint16_t *) (base + offs[l].Wtb);
+    lp.b = hasb ? (const float *) (base + offs[l].b) : nullptr;
+    lp.w_stride = (long) per;
+    lp.b_stride = N;
+  }
+  e = hipMemcpy(ctx->params.p, host.data(), total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(ctx, e, "parameter upload");
+  return TMD_OK;
+}
+
+// Allocate the MLP/descriptor workspace for `nrows` rows.
+int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows)
+{
+  const Model &m = ctx->model;
+  const int L = m.mlp[0].nlayers;
+  const int gw = m.gwidth();
+  hipError_t e;
+  if ((e = w.G.ensure((size_t) nrows * gw * 4)) != hipSuccess) return hip_fail(ctx, e, "G alloc");
+  if ((e = w.dEdG.ensure((size_t) nrows * gw * 4)) != hipSuccess) return hip_fail(ctx, e, "dEdG alloc");
+  if ((e = w.E.ensure((size_t) nrows * 4)) != hipSuccess) return hip_fail(ctx, e, "E alloc");
+  if ((e = w.wrow.ensure((size_t) nrows * 6 * 8)) != hipSuccess) return hip_fail(ctx, e, "wrow alloc");
+  if ((e = w.row2atom.ensure((size_t) nrows * 4)) != hipSuccess) return hip_fail(ctx, e, "row alloc");
+  if ((e = w.red.ensure(4096 * 8)) != hipSuccess) return hip_fail(ctx, e, "red alloc");
+  w.H.resize(L);
+  w.D.resize(L);
+  w.gH.resize(L);
+  for (int l = 0; l < L - 1; l++) {
+    size_t N = m.mlp[0].sizes[l + 1];
+    if ((e = w.H[l].ensure((size_t) nrows * N * 4)) != hipSuccess) return hip_fail(ctx, e, "H alloc");
+    if ((e = w.D[l].ensure((size_t) nrows * N * 4)) != hipSuccess) return hip_fail(ctx, e, "D alloc");
+    if ((e = w.gH[l].ensure((size_t) nrows * N * 4)) != hipSuccess) return hip_fail(ctx, e, "gH alloc");
+  }
+  return TMD_OK;
+}
+
+// descriptor -> MLP fwd/bwd -> force kernel -> FP64 reductions
+int run_chain(tmd_ctx *ctx, Workspace &w, const Cell &cell, const double4 *pos, const uint32_t *nbr,
+              const int *nn, int cap, double scale, double *f, size_t fbytes, double *vatom,
+              size_t vbytes, hipStream_t s, PhaseEvents *pe)
+{
+  const Model &m = ctx->model;
+  const int L = m.mlp[0].nlayers;
+  const int mode = ctx->opt.mlp_precision == TMD_MLP_FP32 ? 0 : 1;
+  hipError_t e;
+  const int *row2atom = (const int *) w.row2atom.p;
+  if (pe) hipEventRecord(pe->ev[0], s);
+  e = launch_descriptor(ctx->geo, cell, pos, nbr, nn, cap, row2atom, w.rl.nrows,
+                        (const float4 *) ctx->tab.p, (float *) w.G.p, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
+  if (pe) hipEventRecord(pe->ev[1], s);
+  for (int l = 0; l < L - 1; l++) {
+    const void *A = (l == 0) ? w.G.p : w.H[l - 1].p;
+    int a32 = (l == 0 || mode == 0) ? 1 : 0;
+    e = launch_dense_fwd(mode, w.rl, ctx->layers[l], A, a32, w.H[l].p, (float *) w.D[l].p, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "dense forward launch");
+  }
+  e = launch_output(w.rl, ctx->layers[L - 1], w.H[L - 2].p, mode == 0 ? 1 : 0, (float *) w.E.p, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "output launch");
+  for (int l = L - 2; l >= 0; l--) {
+    void *out = (l == 0) ? w.dEdG.p : w.gH[l - 1].p;
+    const float *gvec = (l == L - 2) ? ctx->layers[L - 1].W : nullptr;
+    e = launch_dense_bwd(mode, w.rl, ctx->layers[l], (l == L - 2) ? nullptr : w.gH[l].p, gvec,
+                         (int) ctx->layers[L - 1].w_stride, (const float *) w.D[l].p, out, 1, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "dense backward launch");
+  }
+  if (pe) hipEventRecord(pe->ev[2], s);
+  if ((e = launch_zero(f, fbytes, s)) != hipSuccess) return hip_fail(ctx, e, "zero f");
+  if (vatom && (e = launch_zero(vatom, vbytes, s)) != hipSuccess) return hip_fail(ctx, e, "zero vatom");
+  e = launch_force(ctx->geo, cell, pos, nbr, nn, cap, row2atom, w.rl.nrows,
+                   (const float4 *) ctx->tab.p, (const float *) w.dEdG.p, scale, f,
+                   (double *) w.wrow.p, vatom, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "force launch");
+  if (pe) hipEventRecord(pe->ev[3], s);
+  e = launch_reduce_ev((const float *) w.E.p, (const double *) w.wrow.p, row2atom, w.rl.nrows, scale,
+                       (double *) w.red.p, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "reduce launch");
+  if (pe) hipEventRecord(pe->ev[4], s);
+  return TMD_OK;
+}
+
+// rows grouped by element (list order inside an element, tensormd.cpp:377-387),
+// each group padded to kRowTile
+void make_rows(int nelt, const std::vector<int> &atoms, const std::vector<int> &elem_of_atom,
+               RowLayout &rl, std::vector<int> &row2atom)
+{
+  std::vector<int> cnt(nelt, 0);
+  for (int a : atoms) cnt[elem_of_atom[a]]++;
+  std::memset(&rl, 0, sizeof(rl));
+  int off = 0;
+  for (int e = 0; e < nelt; e++) {
+    rl.row_off[e] = off;
+    off += (cnt[e] + kRowTile - 1) / kRowTile * kRowTile;
+  }
+  for (int e = nelt; e <= kMaxElts; e++) rl.row_off[e] = off;
+  rl.nrows = off;
+  row2atom.assign(off, -1);
+  std::vector<int> fill(nelt);
+  for (int e = 0; e < nelt; e++) fill[e] = rl.row_off[e];
+  for (int a : atoms) row2atom[fill[elem_of_atom[a]]++] = a;
+}
+
+}  // namespace tmd
+
+// ===========================================================================
+extern "C" {
+
+void tmd_options_default(tmd_options *opt)
+{
+  if (!opt) return;
+  opt->mlp_precision = TMD_MLP_FP32;
+  opt->interp_dx = 0.001;
+  opt->cmax_exact = 1;
+  opt->device = -1;
+}
+
+const char *tmd_last_error(const tmd_ctx *ctx)
+{
+  if (ctx) return ctx->err.c_str();
+  return g_thread_err.c_str();
+}
+
+int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes,
+               const tmd_options *opt, tmd_ctx **out)
+{
+  if (!out) {
+    set_err(nullptr, "tmd_create: out is NULL");
+    return TMD_ERR_ARG;
+  }
+  *out = nullptr;
+  if (!npz_path || ntypes < 1 || !elem_symbols) {
+    set_err(nullptr, "tmd_create: invalid arguments");
+    return TMD_ERR_ARG;
+  }
+  std::unique_ptr<tmd_ctx> ctx(new tmd_ctx());
+  if (opt)
+    ctx->opt = *opt;
+  else
+    tmd_options_default(&ctx->opt);
+  if (ctx->opt.mlp_precision < TMD_MLP_FP32 || ctx->opt.mlp_precision > TMD_MLP_FP8) {
+    set_err(nullptr, "tmd_create: unknown mlp_precision");
+    return TMD_ERR_ARG;
+  }
+  if (ctx->opt.mlp_precision == TMD_MLP_FP8) {
+    set_err(nullptr, "tmd_create: the FP8 MLP path is not available in this build");
+    return TMD_ERR_ARG;
+  }
+  std::string err;
+  if (!load_model(npz_path, ctx->opt.interp_dx, ctx->model, err)) {
+    set_err(nullptr, err);
+    return TMD_ERR_MODEL;
+  }
+  // PairTensorMD::setup_eltmap (pair_tensormd.cpp:349-370)
+  ctx->ntypes = ntypes;
+  ctx->type_map.assign(ntypes + 1, -1);
+  int mapped = 0;
+  for (int t = 1; t <= ntypes; t++) {
+    const char *s = elem_symbols[t - 1];
+    if (!s) continue;
+    for (int e = 0; e < ctx->model.nelt; e++)
+      if (ctx->model.symbols[e] == s) ctx->type_map[t] = e;
+    if (ctx->type_map[t] >= 0) mapped++;
+  }
+  if (mapped == 0) {
+    set_err(nullptr, "Incorrect args for pair coefficients: no type maps to a model element");
+    return TMD_ERR_ARG;
+  }
+  if (ctx->model.mlp[0].sizes[ctx->model.mlp[0].nlayers - 1] > 4096) {
+    set_err(nullptr, "hidden layers wider than 4096 are not supported");
+    return TMD_ERR_MODEL;
+  }
+  int dev = ctx->opt.device;
+  hipError_t e;
+  if (dev >= 0) {
+    if ((e = hipSetDevice(dev)) != hipSuccess) return hip_fail(nullptr, e, "hipSetDevice");
+  } else {
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return hip_fail(nullptr, e, "hipGetDevice");
+  }
+  ctx->device = dev;
+  if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
+    return hip_fail(nullptr, e, "hipStreamCreate");
+  int rc = upload_model(ctx.get());
+  if (rc != TMD_OK) return rc;
+  *out = ctx.release();
+  return TMD_OK;
+}
+
+void tmd_destroy(tmd_ctx *ctx)
+{
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  ctx->list_ws = Workspace();
+  ctx->tab.release();
+  ctx->params.release();
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int tmd_cutoff(const tmd_ctx *ctx, double *rc)
+{
+  if (!ctx || !rc) return TMD_ERR_ARG;
+  *rc = ctx->model.rc;
+  return TMD_OK;
+}
+
+int tmd_masses(const tmd_ctx *ctx, double *mass_by_type)
+{
+  if (!ctx || !mass_by_type) return TMD_ERR_ARG;
+  mass_by_type[0] = 0.0;
+  for (int t = 1; t <= ctx->ntypes; t++)
+    mass_by_type[t] = ctx->type_map[t] >= 0 ? ctx->model.masses[ctx->type_map[t]] : 0.0;
+  return TMD_OK;
+}
+
+int tmd_type_map(const tmd_ctx *ctx, int *map)
+{
+  if (!ctx || !map) return TMD_ERR_ARG;
+  for (int t = 0; t <= ctx->ntypes; t++) map[t] = ctx->type_map[t];
+  return TMD_OK;
+}
+
+int tmd_model_info(const tmd_ctx *ctx, int *info)
+{
+  if (!ctx || !info) return TMD_ERR_ARG;
+  info[0] = ctx->model.nelt;
+  info[1] = ctx->model.k;
+  info[2] = ctx->model.max_moment;
+  info[3] = ctx->model.gwidth();
+  info[4] = ctx->model.mlp[0].nlayers;
+  return TMD_OK;
+}
+
+int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type,
+                     int inum, const int *ilist, const int *numneigh,
+                     const int *const *firstneigh, double scale, double *f, double *eatom,
+                     double *vatom, double *energy, double *virial)
+{
+  if (!ctx) {
+    set_err(nullptr, "tmd_compute_list: ctx is NULL");
+    return TMD_ERR_ARG;
+  }
+  if (nall < nlocal || inum < 0 || inum > nlocal || !x || !type || !f || !energy ||
+      (inum > 0 && (!ilist || !numneigh || !firstneigh))) {
+    set_err(ctx, "tmd_compute_list: invalid arguments");
+    return TMD_ERR_ARG;
+  }
+  if (nall > kMaxAtoms) {
+    set_err(ctx, "tmd_compute_list: nall exceeds the 2^23-1 atom limit of the neighbor encoding");
+    return TMD_ERR_CAPACITY;
+  }
+  hipSetDevice(ctx->device);
+  Workspace &w = ctx->list_ws;
+  hipStream_t s = ctx->stream;
+  // element per position (type map), packed positions
+  std::vector<double4> pos(std::max(nall, 1));
+  std::vector<int> elem(std::max(nall, 1), 0);
+  for (int i = 0; i < nall; i++) {
+    int t = type[i];
+    int e = (t >= 1 && t <= ctx->ntypes) ? ctx->type_map[t] : -1;
+    if (e < 0) {
+      set_err(ctx, "tmd_compute_list: atom of a type not mapped to a model element");
+      return TMD_ERR_ARG;
+    }
+    elem[i] = e;
+    pos[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], (double) e);
+  }
+  // padded neighbor list indexed by local atom
+  int cap = 8;
+  for (int ii = 0; ii < inum; ii++) {
+    int i = ilist[ii];
+    if (i < 0 || i >= nlocal) {
+      set_err(ctx, "tmd_compute_list: ilist entry out of range");
+      return TMD_ERR_ARG;
+    }
+    cap = std::max(cap, numneigh[i]);
+  }
+  cap = (cap + 7) & ~7;
+  std::vector<uint32_t> list((size_t) std::max(nlocal, 1) * cap, 0u);
+  std::vector<int> nn(std::max(nlocal, 1), 0);
+  std::vector<int> atoms;
+  atoms.reserve(inum);
+  for (int ii = 0; ii < inum; ii++) {
+    int i = ilist[ii];
+    atoms.push_back(i);
+    nn[i] = numneigh[i];
+    for (int p = 0; p < numneigh[i]; p++) {
+      int j = firstneigh[i][p] & 0x1FFFFFFF;  // NEIGHMASK
+      if (j < 0 || j >= nall) {
+        set_err(ctx, "tmd_compute_list: neighbor index out of range");
+        return TMD_ERR_ARG;
+      }
+      list[(size_t) i * cap + p] = (uint32_t) j | (kShiftCenter << kNbrJBits);
+    }
+  }
+  std::vector<int> row2atom;
+  make_rows(ctx->model.nelt, atoms, elem, w.rl, row2atom);
+  if (w.rl.nrows == 0) {
+    std::memset(f, 0, sizeof(double) * 3 * (size_t) nall);
+    if (vatom) std::memset(vatom, 0, sizeof(double) * 6 * (size_t) nall);
+    *energy = 0.0;
+    if (virial) std::memset(virial, 0, 6 * sizeof(double));
+    return TMD_OK;
+  }
+  int rc = ensure_rows(ctx, w, w.rl.nrows);
+  if (rc != TMD_OK) return rc;
+  hipError_t e;
+  if ((e = w.pos.ensure(sizeof(double4) * pos.size())) != hipSuccess) return hip_fail(ctx, e, "pos alloc");
+  if ((e = w.nbr.ensure(4 * list.size())) != hipSuccess) return hip_fail(ctx, e, "nbr alloc");
+  if ((e = w.nn.ensure(4 * nn.size())) != hipSuccess) return hip_fail(ctx, e, "nn alloc");
+  if ((e = w.f.ensure(sizeof(double) * 3 * (size_t) nall)) != hipSuccess) return hip_fail(ctx, e, "f alloc");
+  if (vatom && (e = w.vatom.ensure(sizeof(double) * 6 * (size_t) nall)) != hipSuccess)
+    return hip_fail(ctx, e, "vatom alloc");
+  hipMemcpyAsync(w.pos.p, pos.data(), sizeof(double4) * pos.size(), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(w.nbr.p, list.data(), 4 * list.size(), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(w.nn.p, nn.data(), 4 * nn.size(), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(w.row2atom.p, row2atom.data(), 4 * row2atom.size(), hipMemcpyHostToDevice, s);
+  Cell cell;
+  std::memset(&cell, 0, sizeof(cell));
+  rc = run_chain(ctx, w, cell, (const double4 *) w.pos.p, (const uint32_t *) w.nbr.p,
+                 (const int *) w.nn.p, cap, scale, (double *) w.f.p, sizeof(double) * 3 * (size_t) nall,
+                 vatom ? (double *) w.vatom.p : nullptr, sizeof(double) * 6 * (size_t) nall, s);
+  if (rc != TMD_OK) return rc;
+  double red[7];
+  hipMemcpyAsync(f, w.f.p, sizeof(double) * 3 * (size_t) nall, hipMemcpyDeviceToHost, s);
+  if (vatom) hipMemcpyAsync(vatom, w.vatom.p, sizeof(double) * 6 * (size_t) nall, hipMemcpyDeviceToHost, s);
+  hipMemcpyAsync(red, w.red.p, sizeof(red), hipMemcpyDeviceToHost, s);
+  std::vector<float> E;
+  if (eatom) {
+    E.resize(w.rl.nrows);
+    hipMemcpyAsync(E.data(), w.E.p, 4 * (size_t) w.rl.nrows, hipMemcpyDeviceToHost, s);
+  }
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(ctx, e, "tmd_compute_list");
+  *energy = red[0];
+  if (virial)
+    for (int q = 0; q < 6; q++) virial[q] = red[1 + q];
+  if (eatom) {
+    for (int i = 0; i < nlocal; i++) eatom[i] = 0.0;
+    for (int r = 0; r < w.rl.nrows; r++)
+      if (row2atom[r] >= 0) eatom[row2atom[r]] = (double) E[r] * scale;
+  }
+  return TMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// Host-side TensorMD model: parameters read from the reference .npz schema,
+// the multiplicity matrix T, the element-slot map and the radial
+// interpolation table.  Everything here is built once per model on the host
+// in FP64 and uploaded by the device layer.
+#pragma once
+#include <string>
+#include <vector>
+
+namespace tmd {
+
+enum Act { ACT_RELU = 0, ACT_SOFTPLUS = 1, ACT_TANH = 2, ACT_SQUAREPLUS = 3 };
+
+struct DenseNet {
+  int nlayers = 0;
+  std::vector<int> sizes;  // sizes[0] = input width, sizes[l + 1] = output of layer l
+  int act = 0;
+  bool resnet = false;
+  bool outbias = false;
+  std::vector<std::vector<double>> W;  // [l] row-major [in][out]
+  std::vector<std::vector<double>> B;  // [l] [out]; empty when absent
+  bool res_layer(int l) const { return resnet && l > 0 && l < nlayers - 1 && sizes[l] == sizes[l + 1]; }
+};
+
+struct Model {
+  int nelt = 0;
+  double rc = 0;
+  int max_moment = 0;
+  int nm = 0;  // max_moment + 1
+  int nd = 0;  // monomials
+  int k = 0;   // num_filters
+  int fctype = 0;
+  bool T_sym = false;
+  bool fp32_model = false;
+  std::vector<double> masses;
+  std::vector<std::string> symbols;
+  std::vector<double> T;  // [nm][nd]
+  int eltij[8][8] = {};
+  DenseNet fnn;
+  std::vector<DenseNet> mlp;  // per element
+  // Kerley table, [n][k][4] = {y, s, c1, c2}
+  int tab_n = 0;
+  double dx = 0, xdx = 0;
+  std::vector<double> tab;
+  int gwidth() const { return nelt * k * nm; }
+};
+
+// Reads `path`, validates the schema and builds T, slots and the table for
+// interpolation step `dx`.  Returns false with a message on failure.
+bool load_model(const std::string &path, double dx, Model &m, std::string &err);
+
+}  // namespace tmd

@@ -1,0 +1,224 @@
+// Device neighbor build for periodic cells (any triclinic cell).
+//
+// Replaces LAMMPS NPairFullBin::build (src/npair_full_bin.cpp:68-126) with
+// binning (src/nbin_standard.cpp:82-160) and the displacement trigger of
+// Neighbor::decide/check_distance (src/neighbor.cpp:2293-2360, triggersq at
+// :344).  The full list keeps every (j, image) with |d|^2 <= (rc+skin)^2,
+// i != j or a non-zero image (npair_full_bin.cpp:99).  Bins are cells of the
+// fractional lattice; a stencil of (2*reach+1)^3 bins reaches every
+// candidate, including several periodic images when the cell is small.
+//
+// The rebuild decision is taken on the device: every kernel here reads the
+// control word and exits when no rebuild is due, so a step never waits for
+// the host.
+#include "tmd_device.h"
+
+namespace tmd {
+
+namespace {
+
+__device__ inline void displacement(const double4 &xi, const double4 &xj, int s0, int s1, int s2,
+                                    const Cell &c, double d[3])
+{
+#pragma clang fp contract(off)
+  // Same expression, same order as the oracle: (xj - xi) + ((s0 h0 + s1 h1) + s2 h2)
+  double sh0 = ((double) s0 * c.h[0] + (double) s1 * c.h[3]) + (double) s2 * c.h[6];
+  double sh1 = ((double) s0 * c.h[1] + (double) s1 * c.h[4]) + (double) s2 * c.h[7];
+  double sh2 = ((double) s0 * c.h[2] + (double) s1 * c.h[5]) + (double) s2 * c.h[8];
+  d[0] = (xj.x - xi.x) + sh0;
+  d[1] = (xj.y - xi.y) + sh1;
+  d[2] = (xj.z - xi.z) + sh2;
+}
+
+__device__ inline double dist2(const double d[3])
+{
+#pragma clang fp contract(off)
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+__device__ inline bool rebuild_due(const unsigned long long *ctl, double trig2)
+{
+  double m = __longlong_as_double((long long) ctl[0]);
+  return ctl[1] != 0ull || m > trig2;
+}
+
+// max |x - x_build|^2 over atoms -> ctl[0] (as ordered bits of a positive double)
+__global__ void k_nb_check(int n, const double4 *pos, const double *xb, unsigned long long *ctl)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < n) {
+    double4 p = pos[i];
+    double dx = p.x - xb[4 * i], dy = p.y - xb[4 * i + 1], dz = p.z - xb[4 * i + 2];
+    d2 = dx * dx + dy * dy + dz * dz;
+  }
+  for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o));
+  if ((threadIdx.x & 63) == 0 && d2 > 0.0)
+    atomicMax(&ctl[0], (unsigned long long) __double_as_longlong(d2));
+}
+
+// wrap into the cell, remember build positions, bin the atoms
+__global__ void k_nb_bin(NbrBuild nb, Cell c, double4 *pos)
+{
+  if (!rebuild_due(nb.ctl, nb.trig2)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb.n) return;
+  double4 p = pos[i];
+  double f[3];
+  f[0] = p.x * c.inv[0] + p.y * c.inv[3] + p.z * c.inv[6];
+  f[1] = p.x * c.inv[1] + p.y * c.inv[4] + p.z * c.inv[7];
+  f[2] = p.x * c.inv[2] + p.y * c.inv[5] + p.z * c.inv[8];
+  double s[3] = {floor(f[0]), floor(f[1]), floor(f[2])};
+  if (s[0] != 0.0 || s[1] != 0.0 || s[2] != 0.0) {
+    p.x -= (s[0] * c.h[0] + s[1] * c.h[3]) + s[2] * c.h[6];
+    p.y -= (s[0] * c.h[1] + s[1] * c.h[4]) + s[2] * c.h[7];
+    p.z -= (s[0] * c.h[2] + s[1] * c.h[5]) + s[2] * c.h[8];
+    pos[i] = p;
+    for (int a = 0; a < 3; a++) f[a] -= s[a];
+  }
+  nb.x_build[4 * i + 0] = p.x;
+  nb.x_build[4 * i + 1] = p.y;
+  nb.x_build[4 * i + 2] = p.z;
+  int b[3];
+  for (int a = 0; a < 3; a++) {
+    int q = (int) floor(f[a] * nb.nb[a]);
+    b[a] = min(max(q, 0), nb.nb[a] - 1);
+    nb.atom_bc[3 * i + a] = b[a];
+  }
+  int flat = (b[0] * nb.nb[1] + b[1]) * nb.nb[2] + b[2];
+  nb.atom_bin[i] = flat;
+  atomicAdd(&nb.bin_count[flat], 1);
+}
+
+// exclusive scan of bin counts (single workgroup of 1024 threads)
+__global__ void k_nb_scan(NbrBuild nb, int nbins)
+{
+  if (!rebuild_due(nb.ctl, nb.trig2)) return;
+  __shared__ int part[1024];
+  int t = threadIdx.x;
+  int per = (nbins + 1023) / 1024;
+  int lo = t * per, hi = min(lo + per, nbins);
+  int s = 0;
+  for (int b = lo; b < hi; b++) s += nb.bin_count[b];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = (t >= o) ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = (t == 0) ? 0 : part[t - 1];
+  for (int b = lo; b < hi; b++) {
+    nb.bin_start[b] = run;
+    nb.bin_cursor[b] = run;
+    run += nb.bin_count[b];
+  }
+  if (t == 1023) nb.bin_start[nbins] = part[1023];
+}
+
+__global__ void k_nb_fill(NbrBuild nb)
+{
+  if (!rebuild_due(nb.ctl, nb.trig2)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb.n) return;
+  int slot = atomicAdd(&nb.bin_cursor[nb.atom_bin[i]], 1);
+  nb.bin_atoms[slot] = i;
+}
+
+// make the order inside each bin deterministic (ascending atom index)
+__global__ void k_nb_sortbins(NbrBuild nb, int nbins)
+{
+  if (!rebuild_due(nb.ctl, nb.trig2)) return;
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int lo = nb.bin_start[b], hi = nb.bin_start[b + 1];
+  for (int p = lo + 1; p < hi; p++) {
+    int v = nb.bin_atoms[p];
+    int q = p - 1;
+    while (q >= lo && nb.bin_atoms[q] > v) {
+      nb.bin_atoms[q + 1] = nb.bin_atoms[q];
+      q--;
+    }
+    nb.bin_atoms[q + 1] = v;
+  }
+}
+
+// One thread per atom walks the stencil in a fixed order.
+__global__ void k_nb_list(NbrBuild nb, Cell c, const double4 *pos, uint32_t *nbr, int *nn)
+{
+  if (!rebuild_due(nb.ctl, nb.trig2)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb.n) return;
+  double4 xi = pos[i];
+  int bc0 = nb.atom_bc[3 * i], bc1 = nb.atom_bc[3 * i + 1], bc2 = nb.atom_bc[3 * i + 2];
+  int cnt = 0;
+  uint32_t *out = nbr + (size_t) i * nb.cap;
+  for (int o0 = -nb.reach[0]; o0 <= nb.reach[0]; o0++) {
+    int q0 = bc0 + o0;
+    int s0 = (q0 >= 0) ? q0 / nb.nb[0] : -((-q0 + nb.nb[0] - 1) / nb.nb[0]);
+    int t0 = q0 - s0 * nb.nb[0];
+    for (int o1 = -nb.reach[1]; o1 <= nb.reach[1]; o1++) {
+      int q1 = bc1 + o1;
+      int s1 = (q1 >= 0) ? q1 / nb.nb[1] : -((-q1 + nb.nb[1] - 1) / nb.nb[1]);
+      int t1 = q1 - s1 * nb.nb[1];
+      for (int o2 = -nb.reach[2]; o2 <= nb.reach[2]; o2++) {
+        int q2 = bc2 + o2;
+        int s2 = (q2 >= 0) ? q2 / nb.nb[2] : -((-q2 + nb.nb[2] - 1) / nb.nb[2]);
+        int t2 = q2 - s2 * nb.nb[2];
+        int b = (t0 * nb.nb[1] + t1) * nb.nb[2] + t2;
+        int lo = nb.bin_start[b], hi = nb.bin_start[b + 1];
+        for (int p = lo; p < hi; p++) {
+          int j = nb.bin_atoms[p];
+          if (j == i && s0 == 0 && s1 == 0 && s2 == 0) continue;
+          double d[3];
+          displacement(xi, pos[j], s0, s1, s2, c, d);
+          if (dist2(d) <= nb.cut2) {
+            if (cnt < nb.cap) {
+              uint32_t code = (uint32_t) ((s0 + 3) * 49 + (s1 + 3) * 7 + (s2 + 3));
+              out[cnt] = (uint32_t) j | (code << kNbrJBits);
+            }
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  nn[i] = min(cnt, nb.cap);
+  if (cnt > nb.cap) atomicMax(&nb.ctl[2], (unsigned long long) cnt);
+}
+
+__global__ void k_nb_done(NbrBuild nb, int nbins)
+{
+  // runs as one workgroup: clear counts for the next build, reset the control word
+  bool due = rebuild_due(nb.ctl, nb.trig2);
+  __syncthreads();
+  if (due) {
+    for (int b = threadIdx.x; b <= nbins; b += blockDim.x) nb.bin_count[b] = 0;
+    if (threadIdx.x == 0) nb.ctl[3] += 1ull;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    nb.ctl[0] = 0ull;
+    nb.ctl[1] = 0ull;
+  }
+}
+
+}  // namespace
+
+hipError_t nbr_step(const NbrBuild &nb, const Cell &cell, double4 *pos, uint32_t *nbr, int *nn,
+                    hipStream_t stream)
+{
+  int nbins = nb.nb[0] * nb.nb[1] * nb.nb[2];
+  int blocks = (nb.n + 255) / 256;
+  k_nb_check<<<blocks, 256, 0, stream>>>(nb.n, pos, nb.x_build, nb.ctl);
+  k_nb_bin<<<blocks, 256, 0, stream>>>(nb, cell, pos);
+  k_nb_scan<<<1, 1024, 0, stream>>>(nb, nbins);
+  k_nb_fill<<<blocks, 256, 0, stream>>>(nb);
+  k_nb_sortbins<<<(nbins + 255) / 256, 256, 0, stream>>>(nb, nbins);
+  k_nb_list<<<(nb.n + 127) / 128, 128, 0, stream>>>(nb, cell, pos, nbr, nn);
+  k_nb_done<<<1, 1024, 0, stream>>>(nb, nbins);
+  return hipGetLastError();
+}
+
+}  // namespace tmd

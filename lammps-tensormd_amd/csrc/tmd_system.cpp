@@ -1,0 +1,501 @@
+// Device-resident periodic systems: the LAMMPS step loop around the pair
+// style (Verlet::run, src/verlet.cpp:246-350; FixNVE, src/fix_nve.cpp:64-130;
+// Neighbor::decide, src/neighbor.cpp:2293-2360) with positions, velocities,
+// forces and the neighbor list kept on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/tensormd_amd.h"
+#include "tmd_device.h"
+#include "tmd_internal.h"
+
+using namespace tmd;
+
+struct tmd_system {
+  tmd_ctx *ctx = nullptr;
+  int n = 0;
+  double skin = 0;
+  Cell cell{};
+  NbrBuild nb{};
+  int nbins = 0;
+  Workspace w;
+  std::vector<int> row2atom;
+  DeviceBuffer vel, dtfm, mass, bins, atominfo, xbuild, ctl, hist;
+  bool have_forces = false;
+  double dt_cached = -1;
+  int nbuilds_seen = 0;
+};
+
+namespace {
+
+bool invert3(const double *h, double *inv)
+{
+  double det = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+      h[2] * (h[3] * h[7] - h[4] * h[6]);
+  if (!(std::fabs(det) > 0)) return false;
+  inv[0] = (h[4] * h[8] - h[5] * h[7]) / det;
+  inv[1] = (h[2] * h[7] - h[1] * h[8]) / det;
+  inv[2] = (h[1] * h[5] - h[2] * h[4]) / det;
+  inv[3] = (h[5] * h[6] - h[3] * h[8]) / det;
+  inv[4] = (h[0] * h[8] - h[2] * h[6]) / det;
+  inv[5] = (h[2] * h[3] - h[0] * h[5]) / det;
+  inv[6] = (h[3] * h[7] - h[4] * h[6]) / det;
+  inv[7] = (h[1] * h[6] - h[0] * h[7]) / det;
+  inv[8] = (h[0] * h[4] - h[1] * h[3]) / det;
+  return true;
+}
+
+// perpendicular widths of the cell
+void widths(const double *h, double w[3])
+{
+  const double *A = h, *B = h + 3, *C = h + 6;
+  auto cross = [](const double *u, const double *v, double *o) {
+    o[0] = u[1] * v[2] - u[2] * v[1];
+    o[1] = u[2] * v[0] - u[0] * v[2];
+    o[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  double bc[3], ca[3], ab[3];
+  cross(B, C, bc);
+  cross(C, A, ca);
+  cross(A, B, ab);
+  double V = std::fabs(A[0] * bc[0] + A[1] * bc[1] + A[2] * bc[2]);
+  w[0] = V / std::sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
+  w[1] = V / std::sqrt(ca[0] * ca[0] + ca[1] * ca[1] + ca[2] * ca[2]);
+  w[2] = V / std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+}
+
+int sys_fail(tmd_system *s, int code, const std::string &m)
+{
+  set_err(s ? s->ctx : nullptr, m);
+  return code;
+}
+
+int alloc_list(tmd_system *s, int cap)
+{
+  hipError_t e;
+  s->nb.cap = cap;
+  if ((e = s->w.nbr.ensure((size_t) s->n * cap * 4)) != hipSuccess) return hip_fail(s->ctx, e, "neighbor list alloc");
+  return TMD_OK;
+}
+
+// One force evaluation at the current positions (neighbor list maintained).
+int sys_forces(tmd_system *s)
+{
+  hipStream_t st = s->ctx->stream;
+  hipError_t e = nbr_step(s->nb, s->cell, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p,
+                          (int *) s->w.nn.p, st);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "neighbor build");
+  return run_chain(s->ctx, s->w, s->cell, (const double4 *) s->w.pos.p, (const uint32_t *) s->w.nbr.p,
+                   (const int *) s->w.nn.p, s->nb.cap, 1.0, (double *) s->w.f.p,
+                   sizeof(double) * 3 * (size_t) s->n, nullptr, 0, st);
+}
+
+// overflow check after the queued work; grows the list and asks for a rebuild
+int check_overflow(tmd_system *s, bool *overflow)
+{
+  unsigned long long ctl[4];
+  hipError_t e = hipMemcpyAsync(ctl, s->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, s->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "device step");
+  s->nbuilds_seen = (int) ctl[3];
+  *overflow = ctl[2] != 0;
+  if (*overflow) {
+    int need = (int) ctl[2];
+    int cap = ((need + need / 4 + 16) + 7) & ~7;
+    int rc = alloc_list(s, cap);
+    if (rc != TMD_OK) return rc;
+    unsigned long long reset[3] = {0ull, 1ull, 0ull};
+    hipMemcpy(s->ctl.p, reset, sizeof(reset), hipMemcpyHostToDevice);
+  }
+  return TMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type,
+                      const double *cellv, double skin, tmd_system **out)
+{
+  if (!ctx || !out || !x || !type || !cellv || natoms < 1 || !(skin >= 0)) {
+    set_err(ctx, "tmd_system_create: invalid arguments");
+    return TMD_ERR_ARG;
+  }
+  *out = nullptr;
+  if (natoms > kMaxAtoms) {
+    set_err(ctx, "tmd_system_create: more atoms than the 2^23-1 limit of the neighbor encoding");
+    return TMD_ERR_CAPACITY;
+  }
+  hipSetDevice(ctx->device);
+  std::unique_ptr<tmd_system> s(new tmd_system());
+  s->ctx = ctx;
+  s->n = natoms;
+  s->skin = skin;
+  std::memcpy(s->cell.h, cellv, 9 * sizeof(double));
+  if (!invert3(s->cell.h, s->cell.inv)) {
+    set_err(ctx, "tmd_system_create: singular cell");
+    return TMD_ERR_ARG;
+  }
+  const Model &m = ctx->model;
+  double cut = m.rc + skin;
+  double w[3];
+  widths(s->cell.h, w);
+  for (int a = 0; a < 3; a++) {
+    s->nb.nb[a] = std::max(1, (int) std::floor(w[a] / (0.5 * cut)));
+    double alpha = cut * s->nb.nb[a] / w[a];
+    s->nb.reach[a] = (int) std::ceil(alpha + 1e-6);
+    // image shifts are encoded in [-3, 3]: |s| <= ceil(cut / w) + 1
+    if (std::ceil(cut / w[a]) + 1 > 3) {
+      set_err(ctx, "tmd_system_create: cell narrower than (rc+skin)/2; replicate it first");
+      return TMD_ERR_ARG;
+    }
+  }
+  s->nbins = s->nb.nb[0] * s->nb.nb[1] * s->nb.nb[2];
+  if (s->nbins > (1 << 26)) {
+    set_err(ctx, "tmd_system_create: too many bins");
+    return TMD_ERR_CAPACITY;
+  }
+  s->nb.n = natoms;
+  s->nb.cut2 = cut * cut;
+  s->nb.trig2 = 0.25 * skin * skin;
+  // expected neighbors within rc+skin, with margin for disorder
+  double V = std::fabs(s->cell.h[0] * (s->cell.h[4] * s->cell.h[8] - s->cell.h[5] * s->cell.h[7]) -
+                       s->cell.h[1] * (s->cell.h[3] * s->cell.h[8] - s->cell.h[5] * s->cell.h[6]) +
+                       s->cell.h[2] * (s->cell.h[3] * s->cell.h[7] - s->cell.h[4] * s->cell.h[6]));
+  double expect = 4.0 / 3.0 * M_PI * cut * cut * cut * natoms / V;
+  int cap = (int) std::ceil(std::max(expect * 1.35, expect + 32.0));
+  cap = (cap + 7) & ~7;
+
+  std::vector<double4> pos(natoms);
+  std::vector<int> elem(natoms);
+  std::vector<int> atoms(natoms);
+  for (int i = 0; i < natoms; i++) {
+    int t = type[i];
+    int e = (t >= 1 && t <= ctx->ntypes) ? ctx->type_map[t] : -1;
+    if (e < 0) {
+      set_err(ctx, "tmd_system_create: atom type not mapped to a model element");
+      return TMD_ERR_ARG;
+    }
+    elem[i] = e;
+    atoms[i] = i;
+    pos[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], (double) e);
+  }
+  make_rows(m.nelt, atoms, elem, s->w.rl, s->row2atom);
+  int rc = ensure_rows(ctx, s->w, s->w.rl.nrows);
+  if (rc != TMD_OK) return rc;
+  hipError_t e;
+  auto chk = [&](hipError_t er, const char *what) -> int {
+    if (er != hipSuccess) return hip_fail(ctx, er, what);
+    return TMD_OK;
+  };
+  if ((rc = chk(s->w.pos.ensure(sizeof(double4) * natoms), "pos alloc"))) return rc;
+  if ((rc = chk(s->w.nn.ensure(4 * (size_t) natoms), "nn alloc"))) return rc;
+  if ((rc = chk(s->w.f.ensure(sizeof(double) * 3 * (size_t) natoms), "f alloc"))) return rc;
+  if ((rc = chk(s->vel.ensure(sizeof(double) * 3 * (size_t) natoms), "v alloc"))) return rc;
+  if ((rc = chk(s->dtfm.ensure(sizeof(double) * kMaxElts), "dtfm alloc"))) return rc;
+  if ((rc = chk(s->mass.ensure(sizeof(double) * kMaxElts), "mass alloc"))) return rc;
+  if ((rc = chk(s->bins.ensure(sizeof(int) * (3 * (size_t) (s->nbins + 1) + 1)), "bins alloc"))) return rc;
+  if ((rc = chk(s->atominfo.ensure(sizeof(int) * 5 * (size_t) natoms), "atom bins alloc"))) return rc;
+  if ((rc = chk(s->xbuild.ensure(sizeof(double) * 4 * (size_t) natoms), "xbuild alloc"))) return rc;
+  if ((rc = chk(s->ctl.ensure(sizeof(unsigned long long) * 8), "ctl alloc"))) return rc;
+  if ((rc = chk(s->hist.ensure(sizeof(double) * 2 * 65536), "hist alloc"))) return rc;
+  if ((rc = alloc_list(s.get(), cap))) return rc;
+  int *bins = (int *) s->bins.p;
+  s->nb.bin_count = bins;
+  s->nb.bin_start = bins + (s->nbins + 1);
+  s->nb.bin_cursor = bins + 2 * (s->nbins + 1);
+  int *ai = (int *) s->atominfo.p;
+  s->nb.bin_atoms = ai;
+  s->nb.atom_bin = ai + natoms;
+  s->nb.atom_bc = ai + 2 * (size_t) natoms;
+  s->nb.x_build = (double *) s->xbuild.p;
+  s->nb.ctl = (unsigned long long *) s->ctl.p;
+  hipStream_t st = ctx->stream;
+  std::vector<double> masses(kMaxElts, 1.0);
+  for (int el = 0; el < m.nelt; el++) masses[el] = m.masses[el];
+  unsigned long long ctl0[8] = {0ull, 1ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};  // force first build
+  if ((e = hipMemcpyAsync(s->w.pos.p, pos.data(), sizeof(double4) * natoms, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(s->w.row2atom.p, s->row2atom.data(), 4 * s->row2atom.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(s->mass.p, masses.data(), sizeof(double) * kMaxElts, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(s->ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, st)) != hipSuccess)
+    return hip_fail(ctx, e, "system upload");
+  if ((e = launch_zero(s->vel.p, sizeof(double) * 3 * (size_t) natoms, st)) != hipSuccess) return hip_fail(ctx, e, "zero v");
+  if ((e = launch_zero(s->bins.p, s->bins.bytes, st)) != hipSuccess) return hip_fail(ctx, e, "zero bins");
+  if ((e = launch_zero(s->w.nn.p, 4 * (size_t) natoms, st)) != hipSuccess) return hip_fail(ctx, e, "zero nn");
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(ctx, e, "system create");
+  *out = s.release();
+  return TMD_OK;
+}
+
+void tmd_system_destroy(tmd_system *s)
+{
+  if (!s) return;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  delete s;
+}
+
+int tmd_system_set_positions(tmd_system *s, const double *x)
+{
+  if (!s || !x) return TMD_ERR_ARG;
+  hipSetDevice(s->ctx->device);
+  std::vector<double4> pos(s->n);
+  hipStream_t st = s->ctx->stream;
+  hipError_t e = hipMemcpyAsync(pos.data(), s->w.pos.p, sizeof(double4) * s->n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "set positions");
+  for (int i = 0; i < s->n; i++) {
+    pos[i].x = x[3 * i];
+    pos[i].y = x[3 * i + 1];
+    pos[i].z = x[3 * i + 2];
+  }
+  unsigned long long force_build = 1ull;
+  e = hipMemcpyAsync(s->w.pos.p, pos.data(), sizeof(double4) * s->n, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync((unsigned long long *) s->ctl.p + 1, &force_build, 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "set positions");
+  s->have_forces = false;
+  return TMD_OK;
+}
+
+int tmd_system_set_velocities(tmd_system *s, const double *v)
+{
+  if (!s || !v) return TMD_ERR_ARG;
+  hipSetDevice(s->ctx->device);
+  hipError_t e = hipMemcpy(s->vel.p, v, sizeof(double) * 3 * (size_t) s->n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "set velocities");
+  return TMD_OK;
+}
+
+int tmd_system_compute(tmd_system *s)
+{
+  if (!s) return TMD_ERR_ARG;
+  hipSetDevice(s->ctx->device);
+  for (int attempt = 0; attempt < 4; attempt++) {
+    int rc = sys_forces(s);
+    if (rc != TMD_OK) return rc;
+    bool over = false;
+    rc = check_overflow(s, &over);
+    if (rc != TMD_OK) return rc;
+    if (!over) {
+      s->have_forces = true;
+      return TMD_OK;
+    }
+  }
+  return sys_fail(s, TMD_ERR_CAPACITY, "neighbor list capacity could not be satisfied");
+}
+
+int tmd_system_get(tmd_system *s, double *x, double *v, double *f, double *eatom, double *energy,
+                   double *virial)
+{
+  if (!s) return TMD_ERR_ARG;
+  hipSetDevice(s->ctx->device);
+  hipStream_t st = s->ctx->stream;
+  std::vector<double4> pos;
+  std::vector<float> E;
+  double red[7];
+  hipError_t e = hipSuccess;
+  if (x) {
+    pos.resize(s->n);
+    e = hipMemcpyAsync(pos.data(), s->w.pos.p, sizeof(double4) * s->n, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess && v) e = hipMemcpyAsync(v, s->vel.p, sizeof(double) * 3 * (size_t) s->n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && f) e = hipMemcpyAsync(f, s->w.f.p, sizeof(double) * 3 * (size_t) s->n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && (energy || virial)) e = hipMemcpyAsync(red, s->w.red.p, sizeof(red), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && eatom) {
+    E.resize(s->w.rl.nrows);
+    e = hipMemcpyAsync(E.data(), s->w.E.p, 4 * (size_t) s->w.rl.nrows, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "tmd_system_get");
+  if (x)
+    for (int i = 0; i < s->n; i++) {
+      x[3 * i] = pos[i].x;
+      x[3 * i + 1] = pos[i].y;
+      x[3 * i + 2] = pos[i].z;
+    }
+  if (energy) *energy = red[0];
+  if (virial)
+    for (int q = 0; q < 6; q++) virial[q] = red[1 + q];
+  if (eatom)
+    for (int r = 0; r < s->w.rl.nrows; r++)
+      if (s->row2atom[r] >= 0) eatom[s->row2atom[r]] = (double) E[r];
+  return TMD_OK;
+}
+
+int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
+{
+  if (!s || nsteps < 0 || !(dt > 0)) return TMD_ERR_ARG;
+  hipSetDevice(s->ctx->device);
+  hipStream_t st = s->ctx->stream;
+  const Model &m = s->ctx->model;
+  hipError_t e;
+  if (dt != s->dt_cached) {
+    const double ftm2v = 1.0 / 1.0364269e-4;  // metal units
+    std::vector<double> dtfm(kMaxElts, 0.0);
+    for (int el = 0; el < m.nelt; el++) dtfm[el] = 0.5 * dt * ftm2v / m.masses[el];
+    if ((e = hipMemcpy(s->dtfm.p, dtfm.data(), sizeof(double) * kMaxElts, hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(s->ctx, e, "dtfm upload");
+    s->dt_cached = dt;
+  }
+  if (!s->have_forces) {
+    int rc = tmd_system_compute(s);
+    if (rc != TMD_OK) return rc;
+  }
+  double *hist = (double *) s->hist.p;
+  const int chunk = 4096;
+  for (int done = 0; done < nsteps;) {
+    int nrun = std::min(chunk, nsteps - done);
+    for (int q = 0; q < nrun; q++) {
+      if ((e = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
+                               (const double *) s->dtfm.p, dt, st)) != hipSuccess)
+        return hip_fail(s->ctx, e, "vv first");
+      int rc = sys_forces(s);
+      if (rc != TMD_OK) return rc;
+      if ((e = launch_vv_second(s->n, (const double4 *) s->w.pos.p, (double *) s->vel.p,
+                                (const double *) s->w.f.p, (const double *) s->dtfm.p, st)) != hipSuccess)
+        return hip_fail(s->ctx, e, "vv second");
+      if (epot && (e = launch_copy_scalar((const double *) s->w.red.p, hist + 2 * q, st)) != hipSuccess)
+        return hip_fail(s->ctx, e, "history");
+      if (ekin) {
+        if ((e = launch_ekin(s->n, (const double4 *) s->w.pos.p, (const double *) s->vel.p,
+                             (const double *) s->mass.p, hist + 2 * 4096 + 8 * 0 + 2 * 4096, st)) != hipSuccess)
+          return hip_fail(s->ctx, e, "ekin");
+        if ((e = launch_copy_scalar(hist + 4 * 4096, hist + 2 * q + 1, st)) != hipSuccess)
+          return hip_fail(s->ctx, e, "history");
+      }
+    }
+    bool over = false;
+    int rc = check_overflow(s, &over);
+    if (rc != TMD_OK) return rc;
+    if (over) return sys_fail(s, TMD_ERR_CAPACITY, "neighbor list overflow during the run (list grown; rerun)");
+    if (epot || ekin) {
+      std::vector<double> h(2 * (size_t) nrun);
+      if ((e = hipMemcpy(h.data(), hist, sizeof(double) * 2 * nrun, hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(s->ctx, e, "history copy");
+      for (int q = 0; q < nrun; q++) {
+        if (epot) epot[done + q] = h[2 * q];
+        if (ekin) ekin[done + q] = h[2 * q + 1];
+      }
+    }
+    done += nrun;
+  }
+  return TMD_OK;
+}
+
+int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
+{
+  if (!s || nsteps < 1 || nsteps > 100000 || !(dt > 0) || !ms) return TMD_ERR_ARG;
+  int rc = tmd_md_run(s, 1, dt, nullptr, nullptr);  // forces, dt constants, warm
+  if (rc != TMD_OK) return rc;
+  hipStream_t st = s->ctx->stream;
+  const int NE = 9;
+  std::vector<hipEvent_t> ev((size_t) nsteps * NE);
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return sys_fail(s, TMD_ERR_DEVICE, "hipEventCreate");
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < nsteps && e == hipSuccess; q++) {
+    hipEvent_t *E = &ev[(size_t) q * NE];
+    PhaseEvents pe;
+    for (int k = 0; k < 5; k++) pe.ev[k] = E[3 + k];
+    hipEventRecord(E[0], st);
+    e = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
+                        (const double *) s->dtfm.p, dt, st);
+    hipEventRecord(E[1], st);
+    if (e == hipSuccess)
+      e = nbr_step(s->nb, s->cell, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p, (int *) s->w.nn.p, st);
+    hipEventRecord(E[2], st);
+    if (e == hipSuccess) {
+      rc = run_chain(s->ctx, s->w, s->cell, (const double4 *) s->w.pos.p, (const uint32_t *) s->w.nbr.p,
+                     (const int *) s->w.nn.p, s->nb.cap, 1.0, (double *) s->w.f.p,
+                     sizeof(double) * 3 * (size_t) s->n, nullptr, 0, st, &pe);
+      if (rc != TMD_OK) return rc;
+    }
+    if (e == hipSuccess)
+      e = launch_vv_second(s->n, (const double4 *) s->w.pos.p, (double *) s->vel.p,
+                           (const double *) s->w.f.p, (const double *) s->dtfm.p, st);
+    hipEventRecord(E[8], st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "profile");
+  // phases: integrate(0-1 + 7-8), neighbor(1-2), descriptor(3-4), mlp(4-5), force(5-6), reduce(6-7)
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  auto el = [&](hipEvent_t a, hipEvent_t b) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, a, b);
+    return (double) t;
+  };
+  for (int q = 0; q < nsteps; q++) {
+    hipEvent_t *E = &ev[(size_t) q * NE];
+    acc[0] += el(E[1], E[2]);
+    acc[1] += el(E[3], E[4]);
+    acc[2] += el(E[4], E[5]);
+    acc[3] += el(E[5], E[6]);
+    acc[4] += el(E[6], E[7]);
+    acc[5] += el(E[0], E[1]) + el(E[7], E[8]);
+    acc[6] += el(E[0], E[8]);
+  }
+  for (auto &x : ev) hipEventDestroy(x);
+  for (int k = 0; k < 7; k++) ms[k] = acc[k] / nsteps;
+  bool over = false;
+  rc = check_overflow(s, &over);
+  if (rc != TMD_OK) return rc;
+  if (over) return sys_fail(s, TMD_ERR_CAPACITY, "neighbor list overflow during profiling");
+  return TMD_OK;
+}
+
+int tmd_system_nbuilds(const tmd_system *s) { return s ? s->nbuilds_seen : 0; }
+
+int tmd_system_neighbors(tmd_system *s, long cap, long *off, int *j, int *shift, long *total)
+{
+  if (!s || !off || !total) return TMD_ERR_ARG;
+  hipSetDevice(s->ctx->device);
+  std::vector<int> nn(s->n);
+  std::vector<uint32_t> list((size_t) s->n * s->nb.cap);
+  hipError_t e = hipMemcpy(nn.data(), s->w.nn.p, 4 * (size_t) s->n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(list.data(), s->w.nbr.p, 4 * list.size(), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "neighbor export");
+  long t = 0;
+  for (int i = 0; i < s->n; i++) {
+    off[i] = t;
+    for (int p = 0; p < nn[i]; p++, t++) {
+      if (t < cap && j && shift) {
+        uint32_t v = list[(size_t) i * s->nb.cap + p];
+        uint32_t c = v >> kNbrJBits;
+        j[t] = (int) (v & kNbrJMask);
+        shift[3 * t] = (int) (c / 49) - 3;
+        shift[3 * t + 1] = (int) ((c / 7) % 7) - 3;
+        shift[3 * t + 2] = (int) (c % 7) - 3;
+      }
+    }
+  }
+  off[s->n] = t;
+  *total = t;
+  return TMD_OK;
+}
+
+int tmd_system_descriptors(tmd_system *s, float *G, float *dEdG)
+{
+  if (!s) return TMD_ERR_ARG;
+  hipSetDevice(s->ctx->device);
+  int gw = s->ctx->model.gwidth();
+  int nr = s->w.rl.nrows;
+  std::vector<float> g((size_t) nr * gw), dg((size_t) nr * gw);
+  hipError_t e = hipMemcpy(g.data(), s->w.G.p, 4 * g.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(dg.data(), s->w.dEdG.p, 4 * dg.size(), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "descriptor export");
+  for (int r = 0; r < nr; r++) {
+    int a = s->row2atom[r];
+    if (a < 0) continue;
+    if (G) std::memcpy(G + (size_t) a * gw, g.data() + (size_t) r * gw, 4 * gw);
+    if (dEdG) std::memcpy(dEdG + (size_t) a * gw, dg.data() + (size_t) r * gw, 4 * gw);
+  }
+  return TMD_OK;
+}
+
+}  // extern "C"

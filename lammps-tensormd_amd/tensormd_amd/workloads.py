@@ -1,0 +1,208 @@
+"""Synthetic workloads: TensorMD model files and crystals, all seeded.
+
+The model writer produces .npz files in the reference schema (keys read by
+TensorMD::read_npz, src/ML-TENSORMD/tensormd.cpp:108-177; NNP
+setup_global, src/ML-TENSORMD/nnp.cpp:110-145; FeaturePotential,
+src/ML-TENSORMD/fnn.cpp:32-55): int32 scalars, float64 arrays, weights
+row-major [in][out], ZIP_STORED like the shipped models.
+"""
+import numpy as np
+
+# LAMMPS metal units
+KB = 8.617343e-5          # eV/K
+MVV2E = 1.0364269e-4      # amu (A/ps)^2 -> eV
+FTM2V = 1.0 / MVV2E
+NKTV2P = 1.6021765e6      # eV/A^3 -> bar
+
+ACT = {"relu": 0, "softplus": 1, "tanh": 2, "squareplus": 3}
+
+
+def _symbol_codes(symbols):
+    codes = []
+    for s in symbols:
+        c = [ord(ch) for ch in s[:2]]
+        codes += c + [0] * (2 - len(c))
+    return np.array(codes, dtype=np.int32)
+
+
+def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn="tanh",
+                fnn_hidden=None, fnn_actfn="softplus", max_moment=3, use_resnet_dt=False,
+                apply_output_bias=True, fctype=0, seed=0, radial_scale=0.1, bias_sigma=0.05):
+    """Write a random-init TensorMD model (Xavier-normal FP64 weights).
+
+    layer_sizes: main MLP widths, last must be 1 (e.g. [256, 128, 64, 1]).
+    fnn_hidden: hidden widths of the radial network (default [num_filters]);
+    its output width is num_filters.  radial_scale multiplies the radial
+    network's output layer so that the descriptors of a dense crystal are of
+    order one (trained models are normalised the same way by training).
+    """
+    rng = np.random.default_rng(seed)
+    nelt = len(symbols)
+    k = int(num_filters)
+    fnn_hidden = list(fnn_hidden) if fnn_hidden is not None else [k]
+    fnn_sizes = fnn_hidden + [k]
+    gw = nelt * k * (max_moment + 1)
+    z = {}
+    i32 = lambda v: np.array(v, dtype=np.int32)
+    z["rmax"] = np.array(float(rmax))
+    z["nelt"] = i32(nelt)
+    z["masses"] = np.array(masses, dtype=np.float64)
+    z["numbers"] = _symbol_codes(symbols)
+    z["tdnp"] = i32(0)
+    z["precision"] = i32(64)
+    z["use_fnn"] = i32(1)
+    z["fnn::nlayers"] = i32(len(fnn_sizes))
+    z["fnn::layer_sizes"] = i32(fnn_sizes)
+    z["fnn::num_filters"] = i32(k)
+    z["fnn::actfn"] = i32(ACT[fnn_actfn])
+    z["fnn::use_resnet_dt"] = i32(0)
+    z["fnn::apply_output_bias"] = i32(0)
+
+    def xavier(n_in, n_out):
+        return rng.normal(0.0, np.sqrt(2.0 / (n_in + n_out)), size=(n_in, n_out))
+
+    prev = 1
+    for l, w in enumerate(fnn_sizes):
+        W = xavier(prev, w)
+        if l == len(fnn_sizes) - 1:
+            W *= radial_scale
+        z[f"fnn::weights_0_{l}"] = W.reshape(w) if prev == 1 else W
+        if l < len(fnn_sizes) - 1:
+            z[f"fnn::biases_0_{l}"] = rng.normal(0.0, 0.5, size=w)
+        prev = w
+    sizes = list(layer_sizes)
+    assert sizes[-1] == 1
+    z["nlayers"] = i32(len(sizes))
+    z["max_moment"] = i32(max_moment)
+    z["actfn"] = i32(ACT[actfn])
+    z["fctype"] = i32(fctype)
+    z["layer_sizes"] = i32(sizes)
+    z["use_resnet_dt"] = i32(1 if use_resnet_dt else 0)
+    z["apply_output_bias"] = i32(1 if apply_output_bias else 0)
+    z["is_T_symmetric"] = i32(0)
+    for e in range(nelt):
+        prev = gw
+        for l, w in enumerate(sizes):
+            W = xavier(prev, w)
+            z[f"weights_{e}_{l}"] = W.reshape(prev) if w == 1 else W
+            if l < len(sizes) - 1:
+                z[f"biases_{e}_{l}"] = rng.normal(0.0, bias_sigma, size=w)
+            elif apply_output_bias:
+                z[f"biases_{e}_{l}"] = np.array(rng.normal(0.0, 0.1))
+            prev = w
+    np.savez(path, **z)
+    return path
+
+
+# ---------------------------------------------------------------------------
+def lattice(kind, a, n):
+    """Perfect fcc/bcc supercell: (positions [N,3], cell [3,3] rows)."""
+    if kind == "fcc":
+        base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]], dtype=np.float64)
+    elif kind == "bcc":
+        base = np.array([[0, 0, 0], [0.5, 0.5, 0.5]], dtype=np.float64)
+    else:
+        raise ValueError(kind)
+    nx, ny, nz = n
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1)
+    g = g.reshape(-1, 1, 3).astype(np.float64)
+    x = ((g + base[None]) * a).reshape(-1, 3)
+    cell = np.diag([nx * a, ny * a, nz * a]).astype(np.float64)
+    return x, cell
+
+
+def wrap(x, cell):
+    f = x @ np.linalg.inv(cell)
+    f -= np.floor(f)
+    f[f >= 1.0] = 0.0
+    return f @ cell
+
+
+def crystal(kind, a, n, *, nelt=1, sigma=0.0, seed=0, equiatomic=True):
+    """Jittered crystal with (exact) equiatomic random site occupancy.
+
+    Returns (x [N,3], types [N] 1-based, cell [3,3])."""
+    rng = np.random.default_rng(seed)
+    x, cell = lattice(kind, a, n)
+    N = x.shape[0]
+    if nelt == 1:
+        types = np.ones(N, dtype=np.int32)
+    elif equiatomic:
+        types = (np.arange(N) % nelt + 1).astype(np.int32)
+        rng.shuffle(types)
+    else:
+        types = rng.integers(1, nelt + 1, size=N).astype(np.int32)
+    if sigma > 0:
+        x = x + rng.normal(0.0, sigma, size=x.shape)
+    return wrap(x, cell), types, cell
+
+
+def maxwell_boltzmann(masses, temperature, seed=0):
+    """Velocities (A/ps) at exactly `temperature` (dof 3N-3), zero momentum."""
+    rng = np.random.default_rng(seed)
+    m = np.asarray(masses, dtype=np.float64)
+    N = m.size
+    v = rng.normal(size=(N, 3)) * np.sqrt(KB * temperature / (m * MVV2E))[:, None]
+    p = (m[:, None] * v).sum(0) / m.sum()
+    v -= p
+    ke = 0.5 * MVV2E * (m[:, None] * v * v).sum()
+    t_now = 2.0 * ke / ((3 * N - 3) * KB)
+    if t_now > 0:
+        v *= np.sqrt(temperature / t_now)
+    return v
+
+
+def kinetic_energy(masses, v):
+    return 0.5 * MVV2E * (np.asarray(masses)[:, None] * v * v).sum()
+
+
+# ---------------------------------------------------------------------------
+def lammps_ghost_list(x, cell, cut):
+    """Build a LAMMPS-style full neighbor list with explicit ghost atoms.
+
+    Returns (xall [nall,3], owner [nall], ilist, numneigh, neigh_flat) where
+    atoms 0..N-1 are local and ghosts are periodic images (j, shift) that
+    fall within `cut` of some local atom, like LAMMPS' ghost shell after
+    Comm::borders.  Brute force over images: for test-sized systems.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    cell = np.asarray(cell, dtype=np.float64)
+    N = x.shape[0]
+    inv = np.linalg.inv(cell)
+    widths = 1.0 / np.linalg.norm(inv, axis=0)
+    nimg = [int(np.ceil(cut / w)) + 1 for w in widths]
+    shifts = np.array([(a, b, c) for a in range(-nimg[0], nimg[0] + 1)
+                       for b in range(-nimg[1], nimg[1] + 1)
+                       for c in range(-nimg[2], nimg[2] + 1)], dtype=np.int64)
+    ghost_index = {}
+    xall = [x]
+    owner = list(range(N))
+    lists = []
+    cut2 = cut * cut
+    for i in range(N):
+        nb = []
+        for s in shifts:
+            sh = s @ cell
+            d = (x - x[i]) + sh
+            r2 = (d * d).sum(1)
+            ok = np.nonzero(r2 <= cut2)[0]
+            for j in ok:
+                if j == i and not s.any():
+                    continue
+                if not s.any():
+                    nb.append(int(j))
+                    continue
+                key = (int(j), int(s[0]), int(s[1]), int(s[2]))
+                g = ghost_index.get(key)
+                if g is None:
+                    g = N + len(ghost_index)
+                    ghost_index[key] = g
+                    xall.append((x[j] + sh)[None])
+                    owner.append(int(j))
+                nb.append(g)
+        lists.append(nb)
+    xall = np.concatenate(xall, 0)
+    numneigh = np.array([len(l) for l in lists], dtype=np.int32)
+    neigh_flat = np.array([j for l in lists for j in l], dtype=np.int32)
+    ilist = np.arange(N, dtype=np.int32)
+    return xall, np.array(owner, dtype=np.int64), ilist, numneigh, neigh_flat

@@ -1,0 +1,94 @@
+"""C ABI: the library loads, exports every declared symbol, and rejects bad
+inputs with status codes (no GPU needed: model parsing and argument checks
+happen before the device is touched)."""
+import ctypes
+import os
+import re
+import zipfile
+
+import numpy as np
+import pytest
+
+import tensormd_amd as T
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "tensormd_amd.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tmd_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(T.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_options_default():
+    o = T.Options()
+    T.lib().tmd_options_default(ctypes.byref(o))
+    assert o.mlp_precision == 0 and o.interp_dx == 0.001 and o.device == -1
+
+
+def _create_rc(path, symbols, precision="fp32", dx=0.001):
+    try:
+        T.TensorMD(path, symbols, precision, dx)
+        return 0, ""
+    except T.TmdError as e:
+        return e.code, str(e)
+
+
+def test_missing_file_is_model_error(tmp_path):
+    rc, msg = _create_rc(str(tmp_path / "nope.npz"), ["W"])
+    assert rc == -2 and "cannot open" in msg
+
+
+def test_corrupt_file_is_model_error(tmp_path):
+    p = tmp_path / "bad.npz"
+    p.write_bytes(b"PK\x03\x04 this is not a zip")
+    rc, _ = _create_rc(str(p), ["W"])
+    assert rc == -2
+
+
+def test_missing_key_is_model_error(tmp_path, synth_models):
+    src = np.load(synth_models["c1"])
+    d = {k: src[k] for k in src.files if k != "fnn::num_filters"}
+    p = tmp_path / "nokey.npz"
+    np.savez(p, **d)
+    rc, msg = _create_rc(str(p), ["Cu"])
+    assert rc == -2 and "fnn::num_filters" in msg
+
+
+def test_unknown_element_is_arg_error(synth_models):
+    rc, msg = _create_rc(synth_models["c1"], ["Xx"])
+    assert rc == -1 and "pair coefficients" in msg
+
+
+def test_bad_interp_is_model_error(synth_models):
+    rc, _ = _create_rc(synth_models["c1"], ["Cu"], dx=-1.0)
+    assert rc == -2
+
+
+def test_fp8_not_yet_accepted_loudly(synth_models):
+    rc, msg = _create_rc(synth_models["c1"], ["Cu"], precision="fp8")
+    assert rc == -1 and "FP8" in msg
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device path")
+def test_valid_model_reaches_device_init(synth_models, tmp_path):
+    """Valid models (stored and deflated npz) parse; without a GPU the call
+    fails at device initialisation, not in the model reader."""
+    rc, msg = _create_rc(synth_models["c1"], ["Cu"])
+    assert rc == -3, msg
+    src = np.load(synth_models["c1"])
+    p = tmp_path / "deflated.npz"
+    np.savez_compressed(p, **{k: src[k] for k in src.files})
+    with zipfile.ZipFile(p) as z:
+        assert any(i.compress_type == zipfile.ZIP_DEFLATED for i in z.infolist())
+    rc, msg = _create_rc(str(p), ["Cu"])
+    assert rc == -3, msg

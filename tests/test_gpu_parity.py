@@ -1,0 +1,154 @@
+"""HIP path vs the FP64 oracle, through the C ABI (needs an MI355X).
+
+Tolerances (FP32 path vs FP64 oracle): max-abs force error <= 1e-5 eV/A,
+energy error <= 1e-7 eV/atom.  BF16 MLP path: max-abs force error
+<= 5e-3 eV/A.  Integer work (the neighbor set) is compared bit-exactly.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import model_path
+from tensormd_amd import TensorMD, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+F_TOL = 1e-5
+E_TOL = 1e-7
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _errs(gpu, ref, n):
+    return dict(dF=float(abs(gpu["f"] - ref["f"]).max()),
+                dE=abs(gpu["energy"] - ref["energy"]) / n,
+                dV=float(abs(gpu["virial"] - ref["virial"]).max()) / n)
+
+
+def test_list_mode_fcc_c1(synth_models):
+    say("case: list mode, C1 model, fcc 4x4x4 jittered")
+    om = oracle.OracleModel(synth_models["c1"], 0.001)
+    x, t, cell = W.crystal("fcc", 3.615, (4, 4, 4), sigma=0.05, seed=21)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, om.rc + 1.0)
+    tall = t[owner]
+    ref = om.compute_list(len(x), xall, tall - 1, ilist, numneigh, flat)
+    m = TensorMD(synth_models["c1"], ["Cu"], "fp32", 0.001)
+    got = m.compute_list(len(x), xall, tall, ilist, numneigh, flat)
+    e = _errs(got, ref, len(x))
+    say("errors", e)
+    assert e["dF"] <= F_TOL
+    assert e["dE"] <= E_TOL
+    assert abs(got["eatom"] - ref["eatom"]).max() <= 1e-6
+    assert abs(got["vatom"] - ref["vatom"]).max() <= 1e-5
+    # fdotr virial of the reference equals the pair-sum virial returned here
+    assert e["dV"] <= 1e-6
+
+
+def test_system_c2_shape(synth_models):
+    say("case: periodic system, C2 model, bcc 8x8x8 4 elements")
+    om = oracle.OracleModel(synth_models["c2"], 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (8, 8, 8), nelt=4, sigma=0.08, seed=22)
+    ref = om.compute_periodic(x, t - 1, cell)
+    m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)
+    s = m.system(x, t, cell, skin=1.0)
+    got = s.compute()
+    e = _errs(got, ref, len(x))
+    say("errors", e)
+    assert e["dF"] <= F_TOL
+    assert e["dE"] <= E_TOL
+    assert abs(got["eatom"] - ref["eatom"]).max() <= 1e-6
+    assert e["dV"] <= 1e-6
+
+
+def test_neighbor_set_bit_exact(synth_models):
+    say("case: neighbor set, bcc 10x9x8 tilted cell")
+    m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (10, 9, 8), nelt=4, sigma=0.1, seed=23)
+    cell = cell.copy()
+    cell[1, 0] += 1.3
+    cell[2, 1] -= 0.7
+    x = W.wrap(x, cell)
+    s = m.system(x, t, cell, skin=1.0)
+    s.compute()
+    off, j, sh = s.neighbors()
+    xd = s.get()["x"]  # positions as wrapped on the device
+    ro, rj, rs = oracle.neighbors(xd, cell, m.cutoff + 1.0, method="brute")
+    assert np.array_equal(np.diff(off), np.diff(ro))
+    for i in range(len(x)):
+        a = sorted(zip(j[off[i]:off[i + 1]], map(tuple, sh[off[i]:off[i + 1]])))
+        b = sorted(zip(rj[ro[i]:ro[i + 1]], map(tuple, rs[ro[i]:ro[i + 1]])))
+        assert a == b, i
+
+
+@pytest.mark.parametrize("case", ["W_k32", "MoNi_m3"])
+def test_golden_models(goldens, case):
+    say("case: golden model", case)
+    c = goldens["cases"][case]
+    if case == "W_k32":
+        x, cell = W.lattice("bcc", c["a"], (3, 3, 3))
+        types = np.ones(len(x), dtype=np.int32)
+        symbols = ["W"]
+    else:
+        s = goldens["structures"]["Mo8_Ni8"]
+        x1 = np.array(s["x"])
+        c1 = np.array(s["cell"])
+        # replicate 2x2x2 so that the cell is wider than the list cutoff
+        x = np.concatenate([W.wrap(x1, c1) + np.array([a, b, cc]) @ c1
+                            for a in range(2) for b in range(2) for cc in range(2)])
+        cell = 2.0 * c1
+        x = W.wrap(x, cell)
+        types = np.tile(np.array(s["types"]), 8)
+        symbols = c["symbols"]
+    m = TensorMD(model_path(c["model"]), symbols, "fp32", c["interp"])
+    got = m.system(x, types, cell, skin=1.0).compute()
+    om = oracle.OracleModel(model_path(c["model"]), c["interp"])
+    ref = om.compute_periodic(x, types - 1, cell)
+    e = _errs(got, ref, len(x))
+    say("errors", e, "E/atom", got["energy"] / len(x), "golden", c["pe"] / c["natoms"])
+    assert e["dF"] <= F_TOL
+    assert e["dE"] <= 1e-6  # |E| ~ 10 eV/atom here; 1e-7 relative
+    assert abs(got["energy"] / len(x) - c["pe"] / c["natoms"]) <= 1e-6
+    if case == "MoNi_m3":
+        fy = np.array([r[2:] for r in goldens["forces_MoNi"]["rows"]])
+        assert abs(got["f"][:16] - fy).max() <= F_TOL
+
+
+def test_bf16_mlp_error(synth_models):
+    say("case: bf16 MLP, C2 model, bcc 8x8x8")
+    om = oracle.OracleModel(synth_models["c2"], 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (8, 8, 8), nelt=4, sigma=0.08, seed=24)
+    ref = om.compute_periodic(x, t - 1, cell)
+    m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "bf16", 0.001)
+    got = m.system(x, t, cell, skin=1.0).compute()
+    e = _errs(got, ref, len(x))
+    rms = float(np.sqrt(((got["f"] - ref["f"]) ** 2).mean()))
+    say("bf16 errors", e, "rmsF", rms)
+    assert e["dF"] <= 5e-3
+
+
+def test_nve_c1_trajectory(synth_models):
+    say("case: NVE 100 steps, C1 2048 atoms")
+    om = oracle.OracleModel(synth_models["c1"], 0.001)
+    x, t, cell = W.crystal("fcc", 3.615, (8, 8, 8), sigma=0.05, seed=25)
+    v = W.maxwell_boltzmann(np.full(len(x), om.masses[0]), 300.0, seed=26)
+    nsteps = 100
+    ref = om.nve(x, v, t - 1, cell, 0.001, nsteps)
+    m = TensorMD(synth_models["c1"], ["Cu"], "fp32", 0.001)
+    s = m.system(x, t, cell, skin=1.0)
+    s.set_velocities(v)
+    s.compute()
+    ep, ek = s.md_run(nsteps, 0.001)
+    n = len(x)
+    de = np.abs(ep - ref["epot"][1:]) / n
+    dk = np.abs(ek - ref["ekin"][1:]) / n
+    say("max |dEpot|/atom", de.max(), "max |dEkin|/atom", dk.max(), "builds", s.nbuilds)
+    assert de.max() <= 1e-6
+    assert dk.max() <= 1e-6
+    # energy conservation of the GPU trajectory
+    etot = ep + ek
+    assert abs(etot - etot[0]).max() / n < 1e-5

@@ -92,17 +92,17 @@ int upload_model(tmd_ctx *ctx)
   if (e != hipSuccess) return hip_fail(ctx, e, "table upload");
 
   // MLP: per layer, all elements packed
-  const int L = m.mlp[0].nlayers;
-  ctx->layers.assign(L, LayerParams{});
+  const DenseNet &n0 = m.mlp[0];
+  const int L = n0.nlayers;
+  if (L > kMaxLayers) return (set_err(ctx, "more than 8 MLP layers"), TMD_ERR_MODEL);
   size_t total = 0;
   struct Off {
     size_t W, Wt, Wb, Wtb, b;
   };
   std::vector<Off> offs(L);
-  auto al = [](size_t x) { return (x + 63) & ~(size_t) 63; };
+  auto al = [](size_t x) { return (x + 255) & ~(size_t) 255; };
   for (int l = 0; l < L; l++) {
-    size_t K = m.mlp[0].sizes[l], N = m.mlp[0].sizes[l + 1];
-    size_t per = K * N;
+    size_t per = (size_t) n0.sizes[l] * n0.sizes[l + 1];
     offs[l].W = total;
     total = al(total + per * m.nelt * 4);
     offs[l].Wt = total;
@@ -112,13 +112,19 @@ int upload_model(tmd_ctx *ctx)
     offs[l].Wtb = total;
     total = al(total + per * m.nelt * 2);
     offs[l].b = total;
-    total = al(total + N * m.nelt * 4);
+    total = al(total + (size_t) n0.sizes[l + 1] * m.nelt * 4);
   }
   e = ctx->params.ensure(total);
   if (e != hipSuccess) return hip_fail(ctx, e, "parameter alloc");
   std::vector<unsigned char> host(total, 0);
+  MlpParams &mp = ctx->mlp;
+  std::memset(&mp, 0, sizeof(mp));
+  mp.nl = L;
+  for (int l = 0; l <= L; l++) mp.sizes[l] = n0.sizes[l];
+  mp.act = n0.act;
+  unsigned char *base = (unsigned char *) ctx->params.p;
   for (int l = 0; l < L; l++) {
-    int K = m.mlp[0].sizes[l], N = m.mlp[0].sizes[l + 1];
+    int K = n0.sizes[l], N = n0.sizes[l + 1];
     size_t per = (size_t) K * N;
     float *W = (float *) &host[offs[l].W];
     float *Wt = (float *) &host[offs[l].Wt];
@@ -141,20 +147,21 @@ int upload_model(tmd_ctx *ctx)
         for (int j = 0; j < N; j++) b[el * N + j] = (float) net.B[l][j];
       }
     }
-    LayerParams &lp = ctx->layers[l];
-    lp.K = K;
-    lp.N = N;
-    lp.act = m.mlp[0].act;
-    lp.res = m.mlp[0].res_layer(l) ? 1 : 0;
-    unsigned char *base = (unsigned char *) ctx->params.p;
-    lp.W = (const float *) (base + offs[l].W);
-    lp.Wt = (const float *) (base + offs[l].Wt);
-    lp.Wb = (const uint16_t *) (base + offs[l].Wb);
-    lp.Wtb = (const uint16_t *) (base + offs[l].Wtb);
-    lp.b = hasb ? (const float *) (base + offs[l].b) : nullptr;
-    lp.w_stride = (long) per;
-    lp.b_stride = N;
+    mp.Wf32[l] = (const float *) (base + offs[l].W);
+    mp.Wt32[l] = (const float *) (base + offs[l].Wt);
+    mp.Wb16[l] = (const uint16_t *) (base + offs[l].Wb);
+    mp.Wtb[l] = (const uint16_t *) (base + offs[l].Wtb);
+    mp.b[l] = hasb ? (const float *) (base + offs[l].b) : nullptr;
+    mp.wstride[l] = (long) per;
+    mp.bstride[l] = N;
+    mp.res[l] = n0.res_layer(l) ? 1 : 0;
   }
+  mp.wout = mp.Wf32[L - 1];
+  mp.bout = mp.b[L - 1];
+  mp.b[L - 1] = nullptr;
+  // the fused kernel keeps one row tile's activations in LDS
+  if (mlp_lds_bytes(mp, ctx->opt.mlp_precision == TMD_MLP_FP32 ? 0 : 1, 16, nullptr) > 160 * 1024)
+    return (set_err(ctx, "MLP too wide for the fused kernel's LDS budget"), TMD_ERR_MODEL);
   e = hipMemcpy(ctx->params.p, host.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(ctx, e, "parameter upload");
   return TMD_OK;
@@ -164,7 +171,6 @@ int upload_model(tmd_ctx *ctx)
 int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows)
 {
   const Model &m = ctx->model;
-  const int L = m.mlp[0].nlayers;
   const int gw = m.gwidth();
   hipError_t e;
   if ((e = w.G.ensure((size_t) nrows * gw * 4)) != hipSuccess) return hip_fail(ctx, e, "G alloc");
@@ -173,15 +179,6 @@ int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows)
   if ((e = w.wrow.ensure((size_t) nrows * 6 * 8)) != hipSuccess) return hip_fail(ctx, e, "wrow alloc");
   if ((e = w.row2atom.ensure((size_t) nrows * 4)) != hipSuccess) return hip_fail(ctx, e, "row alloc");
   if ((e = w.red.ensure(4096 * 8)) != hipSuccess) return hip_fail(ctx, e, "red alloc");
-  w.H.resize(L);
-  w.D.resize(L);
-  w.gH.resize(L);
-  for (int l = 0; l < L - 1; l++) {
-    size_t N = m.mlp[0].sizes[l + 1];
-    if ((e = w.H[l].ensure((size_t) nrows * N * 4)) != hipSuccess) return hip_fail(ctx, e, "H alloc");
-    if ((e = w.D[l].ensure((size_t) nrows * N * 4)) != hipSuccess) return hip_fail(ctx, e, "D alloc");
-    if ((e = w.gH[l].ensure((size_t) nrows * N * 4)) != hipSuccess) return hip_fail(ctx, e, "gH alloc");
-  }
   return TMD_OK;
 }
 
@@ -190,8 +187,6 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Cell &cell, const double4 *pos, 
               const int *nn, int cap, double scale, double *f, size_t fbytes, double *vatom,
               size_t vbytes, hipStream_t s, PhaseEvents *pe)
 {
-  const Model &m = ctx->model;
-  const int L = m.mlp[0].nlayers;
   const int mode = ctx->opt.mlp_precision == TMD_MLP_FP32 ? 0 : 1;
   hipError_t e;
   const int *row2atom = (const int *) w.row2atom.p;
@@ -200,21 +195,8 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Cell &cell, const double4 *pos, 
                         (const float4 *) ctx->tab.p, (float *) w.G.p, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
   if (pe) hipEventRecord(pe->ev[1], s);
-  for (int l = 0; l < L - 1; l++) {
-    const void *A = (l == 0) ? w.G.p : w.H[l - 1].p;
-    int a32 = (l == 0 || mode == 0) ? 1 : 0;
-    e = launch_dense_fwd(mode, w.rl, ctx->layers[l], A, a32, w.H[l].p, (float *) w.D[l].p, s);
-    if (e != hipSuccess) return hip_fail(ctx, e, "dense forward launch");
-  }
-  e = launch_output(w.rl, ctx->layers[L - 1], w.H[L - 2].p, mode == 0 ? 1 : 0, (float *) w.E.p, s);
-  if (e != hipSuccess) return hip_fail(ctx, e, "output launch");
-  for (int l = L - 2; l >= 0; l--) {
-    void *out = (l == 0) ? w.dEdG.p : w.gH[l - 1].p;
-    const float *gvec = (l == L - 2) ? ctx->layers[L - 1].W : nullptr;
-    e = launch_dense_bwd(mode, w.rl, ctx->layers[l], (l == L - 2) ? nullptr : w.gH[l].p, gvec,
-                         (int) ctx->layers[L - 1].w_stride, (const float *) w.D[l].p, out, 1, s);
-    if (e != hipSuccess) return hip_fail(ctx, e, "dense backward launch");
-  }
+  e = launch_mlp(mode, w.rl, ctx->mlp, (const float *) w.G.p, (float *) w.E.p, (float *) w.dEdG.p, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "mlp launch");
   if (pe) hipEventRecord(pe->ev[2], s);
   if ((e = launch_zero(f, fbytes, s)) != hipSuccess) return hip_fail(ctx, e, "zero f");
   if (vatom && (e = launch_zero(vatom, vbytes, s)) != hipSuccess) return hip_fail(ctx, e, "zero vatom");
@@ -315,10 +297,6 @@ int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes
   if (mapped == 0) {
     set_err(nullptr, "Incorrect args for pair coefficients: no type maps to a model element");
     return TMD_ERR_ARG;
-  }
-  if (ctx->model.mlp[0].sizes[ctx->model.mlp[0].nlayers - 1] > 4096) {
-    set_err(nullptr, "hidden layers wider than 4096 are not supported");
-    return TMD_ERR_MODEL;
   }
   int dev = ctx->opt.device;
   hipError_t e;

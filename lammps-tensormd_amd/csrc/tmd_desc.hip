@@ -1,21 +1,25 @@
 // Fused descriptor and force/virial kernels (gfx950, wave64).
 //
-// One wavefront owns one atom.  Its neighbor list is walked in chunks of 64
-// entries: each lane takes one entry and forms the FP64 displacement; the
-// kept neighbors (1e-10 < r^2 < rc^2) are sorted by element slot with
-// ballots and staged in LDS with their monomials M_d(u) and radial values
-// H_k(r) (Kerley interpolation of the table).  The lanes then accumulate
-// P[b][k][d] = sum_c H_k(r_c) M_d(u_c) as register tiles of KT x 4 entries.
+// One wavefront owns one atom.
+//  1. Sweep the atom's neighbor list (cutoff rc+skin) in chunks of 64: each
+//     lane forms one FP64 displacement, keeps 1e-10 < r^2 < rc^2
+//     (gpu/cuda/kernel.cu:1163) and finds the element slot b
+//     (tensormd.cpp:165-176).  Kept entries are compacted into LDS sorted
+//     by slot (ballot prefix counts), so later passes run on full lanes.
+//  2. For each chunk of 64 kept neighbors: geometry u = d/r (FP32 from FP64),
+//     monomials M_d(u), the Kerley-interpolated radial values H_k(r)
+//     (gpu/cuda/ration.cu:13-71) with lanes over (neighbor, filter), then
+//     P[b][k][d] += H_k M_d accumulated in registers: each lane owns a
+//     KT x DT block of (k, d) for every slot.
+//  3. Descriptor: G[b][k][m] = sum_d T[m][d] P^2 (m >= 1), G[b][k][0] = P0
+//     (= sign(P0) sqrt(P0^2), kernel.cu:1028-1061).
+//     Force: S = P * (2 sum_m T[m][d] g'_m), g'_0 = g_0/(2 G_0) guarded at
+//     1e-8 (kernel.cu:730-817); per neighbor U_k = S M, V_d = S H,
+//     F = scale (sum_k U_k H'_k) u + scale V . dM/dd (kernel.cu:559-646,
+//     :869-965); f_i += F, f_j -= F, virial -d (x) F (kernel.cu:1714-1758).
 //
-// This replaces, for one atom at a time and without materialising anything
-// per neighbor in HBM, the reference chain
-//   setup_tensors_device (gpu/cuda/kernel.cu:1122-1416),
-//   ration_compute_device (gpu/cuda/ration.cu:13-71),
-//   the P GEMM + kernel_1_fused_device (kernel.cu:1098-1120, :1028-1061),
-// and, in the force kernel,
-//   kernel2_device (kernel.cu:730-817), the V GEMM (:989-1006),
-//   kernel_fused_F2_device (:583-646), kernel4_device (:869-965) and
-//   sum_forces_device (:1714-1758).
+// Nothing per neighbor is stored in HBM: the reference materialises R, M,
+// H, dH/dr, P, dE/dS, V, F1, F2 per padded slot (kernel.cu:1545-1582).
 #include "tmd_device.h"
 
 namespace tmd {
@@ -26,10 +30,12 @@ constexpr int kMS = 24;  // LDS stride of a monomial row (16-byte aligned)
 
 template <int K>
 struct Tiling {
-  static constexpr int KT = (K <= 32) ? 4 : ((K <= 64) ? 8 : 16);  // k per lane
-  static constexpr int LPS = (K / KT) * 5;                           // lanes per slot
-  static constexpr int NG = (64 / LPS) > 0 ? (64 / LPS) : 1;         // slot groups per wave
-  static constexpr int HS = K + 1;                                   // odd stride for per-lane rows
+  // each lane owns KT filters x DT monomials of every slot
+  static constexpr int KT = K == 16 ? 1 : (K <= 32 ? 2 : (K == 64 ? 4 : 8));
+  static constexpr int DT = (K == 24) ? 4 : 5;
+  static constexpr int NDG = kND / DT;
+  static constexpr int LANES = (K / KT) * NDG;  // 60 or 64
+  static constexpr int HS = K + 1;              // odd row stride: per-lane rows conflict-free
 };
 
 __device__ inline void displacement(const double4 &xi, const double4 &xj, uint32_t code,
@@ -77,7 +83,7 @@ __device__ inline void monomials20(float x, float y, float z, float *M)
 }
 
 // Kerley rational interpolation of one filter (gpu/cuda/ration.cu:59-68);
-// the caller handles the zeroed end cells.
+// the caller zeroes the two end cells.
 __device__ inline void kerley(float4 t, float q, float p, float dx, float &h, float &dh)
 {
   float c1 = t.z, c2 = t.w, s = t.y;
@@ -89,223 +95,252 @@ __device__ inline void kerley(float4 t, float q, float p, float dx, float &h, fl
   dh = s + (q - p) * c4 + dx * (c4 - c2) * (1.0f - c3);
 }
 
-// Per-chunk staging shared by both kernels.
-struct ChunkLDS {
-  float *M;       // [64][kMS]
-  float *H;       // [64][HS]
-  float *dH;      // [64][HS] (force kernel only)
-  float4 *rec;    // [64]: ux, uy, uz, r
-  float4 *drec;   // [64]: dx, dy, dz (fp32 of the fp64 displacement), j as bits
-  int *tcell;     // [64]
-  float2 *qp;     // [64]
-  int *slot;      // [64]
-  int *sstart;    // [kMaxElts + 1]
+__device__ inline unsigned lane_rank(unsigned long long m)
+{
+  return __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
+}
+
+struct LDSMap {
+  uint32_t *kept;  // [cap] compacted, slot-sorted neighbor entries
+  float4 *rec;     // [64] ux, uy, uz, r of the current chunk
+  int *tcell;      // [64]
+  float2 *qp;      // [64]
+  float4 *drec;    // [64] fp32 displacement, j as bits (force)
+  float *M;        // [64][kMS]   (aliased by P/S after the accumulation)
+  float *H;        // [64][HS]
+  float *dH;       // [64][HS]    (force)
+  float *PS;       // [nslot][K][20] = M region
 };
 
-// Geometry + slot sort of one raw chunk.  Returns the number of kept
-// neighbors staged; sstart[b] .. sstart[b+1] is slot b's range.
-template <bool FORCE>
-__device__ inline int stage_chunk(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
-                                  const uint32_t *list, int base, int nraw, const double4 *pos,
-                                  ChunkLDS &L)
+template <int K>
+__host__ __device__ inline size_t lds_layout(int nelt, int cap, bool force, size_t *offs)
 {
-  const int lane = threadIdx.x & 63;
-  int idx = base + lane;
-  bool keep = false;
-  int slot = -1, j = 0, tcell = 0;
-  double d[3] = {0, 0, 0}, r = 0.0, rinv = 0.0;
-  if (idx < nraw) {
-    uint32_t e = list[idx];
-    j = (int) (e & kNbrJMask);
-    double4 xj = pos[j];
-    displacement(xi, xj, e >> kNbrJBits, cell, d);
-    double r2 = dist2(d);
-    if (r2 < g.rc2 && r2 > 1e-10) {  // kernel.cu:1163
-      keep = true;
-      slot = g.eltij[ei][(int) xj.w];
-      r = sqrt(r2);
-      rinv = 1.0 / r;
-    }
-  }
-  // stable partition by slot: position = start(slot) + rank among equal slots
-  int start = 0, mypos = -1;
-  for (int b = 0; b < g.nelt; b++) {
-    unsigned long long m = __ballot(slot == b);
-    if (lane == 0) L.sstart[b] = start;
-    if (slot == b) mypos = start + (int) __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
-    start += __popcll(m);
-  }
-  if (lane == 0) L.sstart[g.nelt] = start;
-  if (keep) {
-    float ux = (float) (d[0] * rinv), uy = (float) (d[1] * rinv), uz = (float) (d[2] * rinv);
-    float M[kND];
-    monomials20(ux, uy, uz, M);
-    float *Mr = L.M + mypos * kMS;
-#pragma unroll
-    for (int q = 0; q < kND; q++) Mr[q] = M[q];
-    L.rec[mypos] = make_float4(ux, uy, uz, (float) r);
-    // table cell and offsets in FP64 (ration.cu:29-32)
-    int n = (int) (r * g.xdx);
-    n = min(g.tab_n - 2, n);
-    double q = r - (double) n * g.dx;
-    double p = g.dx - q;
-    L.tcell[mypos] = n;
-    L.qp[mypos] = make_float2((float) q, (float) p);
-    L.slot[mypos] = slot;
-    if (FORCE) L.drec[mypos] = make_float4((float) d[0], (float) d[1], (float) d[2], __int_as_float(j));
-  }
-  return start;
+  constexpr int HS = Tiling<K>::HS;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    size_t r = o;
+    o += (bytes + 15) & ~(size_t) 15;
+    return r;
+  };
+  size_t mreg = (size_t) 64 * kMS * 4;
+  size_t preg = (size_t) nelt * K * kND * 4;
+  offs[0] = take((size_t) cap * 4);
+  offs[1] = take(64 * 16);
+  offs[2] = take(64 * 4);
+  offs[3] = take(64 * 8);
+  offs[4] = take(force ? 64 * 16 : 16);
+  offs[5] = take(mreg > preg ? mreg : preg);
+  offs[6] = take((size_t) 64 * HS * 4);
+  offs[7] = take(force ? (size_t) 64 * HS * 4 : 16);
+  return o;
 }
 
-// H (and dH) for the staged neighbors: lanes over (neighbor, filter)
-template <int K, bool FORCE>
-__device__ inline void stage_radial(const ModelGeo &g, const float4 *tab, int nk, ChunkLDS &L)
+template <int K>
+__device__ inline LDSMap carve(unsigned char *base, int nelt, int cap, bool force)
+{
+  size_t offs[8];
+  lds_layout<K>(nelt, cap, force, offs);
+  LDSMap L;
+  L.kept = (uint32_t *) (base + offs[0]);
+  L.rec = (float4 *) (base + offs[1]);
+  L.tcell = (int *) (base + offs[2]);
+  L.qp = (float2 *) (base + offs[3]);
+  L.drec = (float4 *) (base + offs[4]);
+  L.M = (float *) (base + offs[5]);
+  L.PS = L.M;
+  L.H = (float *) (base + offs[6]);
+  L.dH = (float *) (base + offs[7]);
+  return L;
+}
+
+// Phase 1: compact the kept entries (slot-sorted).  start[b] .. start[b+1].
+__device__ inline int compact(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
+                              const uint32_t *list, int nraw, const double4 *pos, uint32_t *kept,
+                              int *start)
 {
   const int lane = threadIdx.x & 63;
-  constexpr int HS = FORCE ? Tiling<K>::HS : K;
-  for (int e = lane; e < nk * K; e += 64) {
-    int c = e / K;
-    int kk = e - c * K;
-    int n = L.tcell[c];
+  int cnt[kMaxElts];
+  for (int b = 0; b < kMaxElts; b++) cnt[b] = 0;
+  for (int base = 0; base < nraw; base += 64) {
+    int idx = base + lane, slot = -1;
+    if (idx < nraw) {
+      uint32_t e = list[idx];
+      double4 xj = pos[e & kNbrJMask];
+      double d[3];
+      displacement(xi, xj, e >> kNbrJBits, cell, d);
+      double r2 = dist2(d);
+      if (r2 < g.rc2 && r2 > 1e-10) slot = g.eltij[ei][(int) xj.w];
+    }
+    for (int b = 0; b < g.nelt; b++) cnt[b] += __popcll(__ballot(slot == b));
+  }
+  int s = 0;
+  for (int b = 0; b < g.nelt; b++) {
+    start[b] = s;
+    s += cnt[b];
+  }
+  start[g.nelt] = s;
+  int run[kMaxElts];
+  for (int b = 0; b < kMaxElts; b++) run[b] = start[b];
+  for (int base = 0; base < nraw; base += 64) {
+    int idx = base + lane, slot = -1;
+    uint32_t e = 0;
+    if (idx < nraw) {
+      e = list[idx];
+      double4 xj = pos[e & kNbrJMask];
+      double d[3];
+      displacement(xi, xj, e >> kNbrJBits, cell, d);
+      double r2 = dist2(d);
+      if (r2 < g.rc2 && r2 > 1e-10) slot = g.eltij[ei][(int) xj.w];
+    }
+    for (int b = 0; b < g.nelt; b++) {
+      unsigned long long m = __ballot(slot == b);
+      if (slot == b) kept[run[b] + (int) lane_rank(m)] = e;
+      run[b] += __popcll(m);
+    }
+  }
+  return s;
+}
+
+// Phase 2 staging of kept neighbors [base, base+64): geometry, monomials, H (and dH)
+template <int K, bool FORCE>
+__device__ inline void stage(const ModelGeo &g, const Cell &cell, const double4 &xi,
+                             const double4 *pos, const float4 *tab, int base, int nk, LDSMap &L,
+                             bool want_M)
+{
+  const int lane = threadIdx.x & 63;
+  constexpr int HS = Tiling<K>::HS;
+  int c = base + lane;
+  if (c < nk) {
+    uint32_t e = L.kept[c];
+    int j = (int) (e & kNbrJMask);
+    double d[3];
+    displacement(xi, pos[j], e >> kNbrJBits, cell, d);
+    double r = sqrt(dist2(d));
+    double rinv = 1.0 / r;
+    float ux = (float) (d[0] * rinv), uy = (float) (d[1] * rinv), uz = (float) (d[2] * rinv);
+    L.rec[lane] = make_float4(ux, uy, uz, (float) r);
+    int n = min(g.tab_n - 2, (int) (r * g.xdx));  // ration.cu:29-32, in FP64
+    double q = r - (double) n * g.dx;
+    L.tcell[lane] = n;
+    L.qp[lane] = make_float2((float) q, (float) (g.dx - q));
+    if (FORCE) L.drec[lane] = make_float4((float) d[0], (float) d[1], (float) d[2], __int_as_float(j));
+    if (want_M) {
+      float M[kND];
+      monomials20(ux, uy, uz, M);
+      float4 *Mr = reinterpret_cast<float4 *>(L.M + lane * kMS);
+#pragma unroll
+      for (int q4 = 0; q4 < kND / 4; q4++)
+        Mr[q4] = make_float4(M[4 * q4], M[4 * q4 + 1], M[4 * q4 + 2], M[4 * q4 + 3]);
+    }
+  }
+  __syncthreads();
+  const int cnt = min(64, nk - base);
+  for (int e = lane; e < cnt * K; e += 64) {
+    int cc = e / K;
+    int kk = e - cc * K;
+    int n = L.tcell[cc];
     float h = 0.0f, dh = 0.0f;
-    if (n > 0 && n < g.tab_n - 2) {  // endpoint cells are zero (ration.hpp:29)
-      float2 qp = L.qp[c];
+    if (n > 0 && n < g.tab_n - 2) {  // end cells are zero (ration.hpp:29)
+      float2 qp = L.qp[cc];
       kerley(tab[(size_t) n * K + kk], qp.x, qp.y, g.dxf, h, dh);
     }
-    L.H[c * HS + kk] = h;
-    if (FORCE) L.dH[c * HS + kk] = dh;
-  }
-}
-
-// P[b][k][d] += sum over the chunk's slot-b neighbors of H_k M_d.
-template <int K, bool FORCE>
-__device__ inline void accumulate_P(const ModelGeo &g, ChunkLDS &L, float *sP)
-{
-  using T = Tiling<K>;
-  constexpr int HS = FORCE ? T::HS : K;
-  const int lane = threadIdx.x & 63;
-  const int grp = lane / T::LPS;
-  const int w = lane - grp * T::LPS;
-  const int kq = w / 5, dq = w - (w / 5) * 5;
-  if (grp >= T::NG) return;
-  for (int b = grp; b < g.nelt; b += T::NG) {
-    int c0 = L.sstart[b], c1 = L.sstart[b + 1];
-    if (c0 == c1) continue;
-    float acc[T::KT][4];
-#pragma unroll
-    for (int a = 0; a < T::KT; a++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) acc[a][q] = 0.0f;
-    for (int c = c0; c < c1; c++) {
-      float4 m = *reinterpret_cast<const float4 *>(L.M + c * kMS + dq * 4);
-      const float *hr = L.H + c * HS + kq * T::KT;
-#pragma unroll
-      for (int a = 0; a < T::KT; a++) {
-        float h = hr[a];
-        acc[a][0] += h * m.x;
-        acc[a][1] += h * m.y;
-        acc[a][2] += h * m.z;
-        acc[a][3] += h * m.w;
-      }
-    }
-    float *P = sP + ((size_t) b * K + kq * T::KT) * kND + dq * 4;
-#pragma unroll
-    for (int a = 0; a < T::KT; a++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) P[a * kND + q] += acc[a][q];
-  }
-}
-
-template <int K, bool FORCE>
-__device__ inline void build_P(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
-                               const uint32_t *list, int nraw, const double4 *pos,
-                               const float4 *tab, ChunkLDS &L, float *sP)
-{
-  const int lane = threadIdx.x & 63;
-  for (int e = lane; e < g.nelt * K * kND; e += 64) sP[e] = 0.0f;
-  for (int base = 0; base < nraw; base += 64) {
-    __syncthreads();
-    int nk = stage_chunk<false>(g, cell, xi, ei, list, base, nraw, pos, L);
-    __syncthreads();
-    stage_radial<K, FORCE>(g, tab, nk, L);
-    __syncthreads();
-    accumulate_P<K, FORCE>(g, L, sP);
+    L.H[cc * HS + kk] = h;
+    if (FORCE) L.dH[cc * HS + kk] = dh;
   }
   __syncthreads();
 }
 
-template <int K>
-size_t lds_bytes(int nelt, bool force)
+// Phase 2 accumulation over the staged chunk into the lane's register block
+template <int K, int NS>
+__device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int *start, int base,
+                                  float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
 {
-  constexpr int HS = Tiling<K>::HS;
-  size_t b = 0;
-  b += (size_t) nelt * K * kND * 4;  // P / S
-  b += 64 * kMS * 4;                 // M
-  b += (size_t) 64 * (force ? HS : K) * 4 * (force ? 2 : 1);
-  b += 64 * 16 * (force ? 2 : 1);    // rec (+ drec)
-  b += 64 * 4 + 64 * 8 + 64 * 4 + (kMaxElts + 1) * 4;
-  return b + 64;
+  using T = Tiling<K>;
+  const int lane = threadIdx.x & 63;
+  if (lane >= T::LANES) return;
+  const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
+  const float *Hb = L.H + kq * T::KT;
+  const float *Mb = L.M + dq * T::DT;
+#pragma unroll
+  for (int b = 0; b < NS; b++) {
+    if (b >= g.nelt) break;
+    int c0 = max(start[b] - base, 0), c1 = min(start[b + 1] - base, 64);
+    for (int c = c0; c < c1; c++) {
+      float h[T::KT], m[T::DT];
+#pragma unroll
+      for (int a = 0; a < T::KT; a++) h[a] = Hb[c * T::HS + a];
+#pragma unroll
+      for (int q = 0; q < T::DT; q++) m[q] = Mb[c * kMS + q];
+#pragma unroll
+      for (int a = 0; a < T::KT; a++)
+#pragma unroll
+        for (int q = 0; q < T::DT; q++) acc[b][a][q] += h[a] * m[q];
+    }
+  }
 }
 
-__device__ inline ChunkLDS carve(char *base, int nelt, int K, int HS, bool force, float **sP)
+// Phases 1-2 and the write of P to LDS (L.PS, [b][k][d]).  Returns nk.
+template <int K, int NS, bool FORCE>
+__device__ inline int build_P(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
+                              const uint32_t *list, int nraw, const double4 *pos, const float4 *tab,
+                              LDSMap &L, int *start)
 {
-  ChunkLDS L;
-  char *p = base;
-  *sP = (float *) p;
-  p += (size_t) nelt * K * kND * 4;
-  L.M = (float *) p;
-  p += 64 * kMS * 4;
-  L.H = (float *) p;
-  p += (size_t) 64 * HS * 4;
-  if (force) {
-    L.dH = (float *) p;
-    p += (size_t) 64 * HS * 4;
-  } else {
-    L.dH = nullptr;
+  using T = Tiling<K>;
+  const int lane = threadIdx.x & 63;
+  int nk = compact(g, cell, xi, ei, list, nraw, pos, L.kept, start);
+  __syncthreads();
+  float acc[NS][T::KT][T::DT];
+#pragma unroll
+  for (int b = 0; b < NS; b++)
+#pragma unroll
+    for (int a = 0; a < T::KT; a++)
+#pragma unroll
+      for (int q = 0; q < T::DT; q++) acc[b][a][q] = 0.0f;
+  for (int base = 0; base < nk; base += 64) {
+    stage<K, FORCE>(g, cell, xi, pos, tab, base, nk, L, true);
+    accumulate<K, NS>(g, L, start, base, acc);
+    __syncthreads();
   }
-  L.rec = (float4 *) p;
-  p += 64 * 16;
-  if (force) {
-    L.drec = (float4 *) p;
-    p += 64 * 16;
-  } else {
-    L.drec = nullptr;
+  // registers -> LDS P[b][k][d] (overwrites the monomial rows)
+  if (lane < T::LANES) {
+    const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
+#pragma unroll
+    for (int b = 0; b < NS; b++) {
+      if (b >= g.nelt) break;
+#pragma unroll
+      for (int a = 0; a < T::KT; a++)
+#pragma unroll
+        for (int q = 0; q < T::DT; q++)
+          L.PS[((size_t) b * K + kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
+    }
   }
-  L.tcell = (int *) p;
-  p += 64 * 4;
-  L.qp = (float2 *) p;
-  p += 64 * 8;
-  L.slot = (int *) p;
-  p += 64 * 4;
-  L.sstart = (int *) p;
-  return L;
+  __syncthreads();
+  return nk;
 }
 
-template <int K>
+template <int K, int NS>
 __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Cell cell, const double4 *pos,
                                                    const uint32_t *nbr, const int *nn, int cap,
                                                    const int *row2atom, int nrows,
                                                    const float4 *tab, float *G)
 {
-  extern __shared__ __align__(16) char smem[];
+  extern __shared__ __align__(16) unsigned char smem[];
   const int row = blockIdx.x;
   if (row >= nrows) return;
   const int lane = threadIdx.x & 63;
   const int gw = g.nelt * K * kNM;
   float *Grow = G + (size_t) row * gw;
   const int i = row2atom[row];
-  if (i < 0) {  // padding row: zero descriptor
+  if (i < 0) {  // padding row
     for (int e = lane; e < gw; e += 64) Grow[e] = 0.0f;
     return;
   }
-  float *sP;
-  ChunkLDS L = carve(smem, g.nelt, K, K, false, &sP);
+  LDSMap L = carve<K>(smem, g.nelt, cap, false);
+  int start[kMaxElts + 1];
   const double4 xi = pos[i];
-  const int ei = (int) xi.w;
-  build_P<K, false>(g, cell, xi, ei, nbr + (size_t) i * cap, nn[i], pos, tab, L, sP);
-  // G[b][k][m] = sum_d T[m][d] P^2 (m >= 1); G[b][k][0] = P0 (= sign(P0) sqrt(P0^2))
+  build_P<K, NS, false>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start);
   for (int e = lane; e < g.nelt * K; e += 64) {
-    const float *P = sP + (size_t) e * kND;
+    const float *P = L.PS + (size_t) e * kND;
     float Gv[kNM] = {0, 0, 0, 0};
 #pragma unroll
     for (int d = 0; d < kND; d++) {
@@ -318,7 +353,7 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Cell cell, const 
   }
 }
 
-template <int K>
+template <int K, int NS>
 __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const double4 *pos,
                                               const uint32_t *nbr, const int *nn, int cap,
                                               const int *row2atom, int nrows, const float4 *tab,
@@ -327,7 +362,7 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
 {
   using T = Tiling<K>;
   constexpr int HS = T::HS;
-  extern __shared__ __align__(16) char smem[];
+  extern __shared__ __align__(16) unsigned char smem[];
   const int row = blockIdx.x;
   if (row >= nrows) return;
   const int lane = threadIdx.x & 63;
@@ -336,18 +371,16 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
     if (lane < 6) wrow[(size_t) row * 6 + lane] = 0.0;
     return;
   }
-  float *sP;
-  ChunkLDS L = carve(smem, g.nelt, K, HS, true, &sP);
+  LDSMap L = carve<K>(smem, g.nelt, cap, true);
+  int start[kMaxElts + 1];
   const double4 xi = pos[i];
-  const int ei = (int) xi.w;
-  const uint32_t *list = nbr + (size_t) i * cap;
-  const int nraw = nn[i];
-  build_P<K, true>(g, cell, xi, ei, list, nraw, pos, tab, L, sP);
+  const int nk = build_P<K, NS, true>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab,
+                                  L, start);
 
-  // S[b][k][d] = P[b][k][d] * sum_m 2 T[m][d] g'_m  (kernel.cu:752-767), in place
+  // S[b][k][d] = P * sum_m 2 T[m][d] g'_m, in place
   const float *grow = dEdG + (size_t) row * g.nelt * K * kNM;
   for (int e = lane; e < g.nelt * K; e += 64) {
-    float *P = sP + (size_t) e * kND;
+    float *P = L.PS + (size_t) e * kND;
     float4 gv = *reinterpret_cast<const float4 *>(grow + (size_t) e * kNM);
     float G0 = P[0];
     float g0 = (fabsf(G0) < 1e-8f) ? 0.0f : gv.x / G0 * 0.5f;
@@ -364,25 +397,23 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
 
   double fi[3] = {0, 0, 0};
   double wi[6] = {0, 0, 0, 0, 0, 0};
-  const float fscale = 1.0f;
-  for (int base = 0; base < nraw; base += 64) {
-    __syncthreads();
-    int nk = stage_chunk<true>(g, cell, xi, ei, list, base, nraw, pos, L);
-    __syncthreads();
-    stage_radial<K, true>(g, tab, nk, L);
-    __syncthreads();
-    if (lane < nk) {
-      const int c = lane;
-      const int b = L.slot[c];
+  // chunks in reverse: the last staged chunk is still in LDS
+  const int last = nk > 0 ? ((nk - 1) / 64) * 64 : 0;
+  for (int base = last; base >= 0 && nk > 0; base -= 64) {
+    if (base != last) stage<K, true>(g, cell, xi, pos, tab, base, nk, L, false);
+    const int c = lane;
+    if (base + c < nk) {
+      int b = 0;
+      while (b + 1 < g.nelt && base + c >= start[b + 1]) b++;
+      float4 rc = L.rec[c];
+      const float x = rc.x, y = rc.y, z = rc.z;
       float M[kND];
-      const float *Mr = L.M + c * kMS;
-#pragma unroll
-      for (int q = 0; q < kND; q++) M[q] = Mr[q];
+      monomials20(x, y, z, M);
       float V[kND];
 #pragma unroll
       for (int q = 0; q < kND; q++) V[q] = 0.0f;
       float f1 = 0.0f;
-      const float *S = sP + (size_t) b * K * kND;
+      const float *S = L.PS + (size_t) b * K * kND;
       const float *Hr = L.H + c * HS;
       const float *dHr = L.dH + c * HS;
       for (int kk = 0; kk < K; kk++) {
@@ -400,9 +431,7 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
         }
         f1 += U * dh;
       }
-      float4 rc = L.rec[c];
-      float x = rc.x, y = rc.y, z = rc.z;
-      // g_beta = sum_d V_d dM_d/du_beta for the 20 monomials
+      // g_beta = sum_d V_d dM_d/du_beta
       float gx = V[1] + 2.f * x * V[4] + y * V[5] + z * V[6] + 3.f * x * x * V[10] +
           2.f * x * y * V[11] + 2.f * x * z * V[12] + y * y * V[13] + y * z * V[14] + z * z * V[15];
       float gy = V[2] + x * V[5] + 2.f * y * V[7] + z * V[8] + x * x * V[11] + 2.f * x * y * V[13] +
@@ -411,14 +440,12 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
           2.f * x * z * V[15] + y * y * V[17] + 2.f * y * z * V[18] + 3.f * z * z * V[19];
       float gu = gx * x + gy * y + gz * z;
       float rinv = 1.0f / rc.w;
-      // dE_i/dDelta = F1 + F2 (kernel.cu:559-571 and :583-646)
+      // dE_i/dd = F1 + F2; dM/dd_a = sum_b dM/du_b (delta_ab - u_a u_b)/r
       double F0 = (double) (f1 * x + (gx - gu * x) * rinv) * scale;
       double F1v = (double) (f1 * y + (gy - gu * y) * rinv) * scale;
       double F2v = (double) (f1 * z + (gz - gu * z) * rinv) * scale;
-      (void) fscale;
       float4 dr = L.drec[c];
       int j = __float_as_int(dr.w);
-      // f_i += F, f_j -= F (kernel.cu:1726-1730)
       fi[0] += F0;
       fi[1] += F1v;
       fi[2] += F2v;
@@ -434,8 +461,8 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
         for (int q = 0; q < 6; q++) atomicAdd(&vatom[6 * (size_t) j + q], 0.5 * v[q]);
       }
     }
+    __syncthreads();
   }
-  // wave reductions
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
@@ -455,32 +482,58 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
   }
 }
 
-template <int K>
+template <int K, int NS>
 hipError_t desc_impl(const ModelGeo &g, const Cell &cell, const double4 *pos, const uint32_t *nbr,
                      const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
                      float *G, hipStream_t s)
 {
-  size_t lds = lds_bytes<K>(g.nelt, false);
+  size_t offs[8];
+  size_t lds = lds_layout<K>(g.nelt, cap, false, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
-  if (lds > 64 * 1024)
-    hipFuncSetAttribute((const void *) k_descriptor<K>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-  k_descriptor<K><<<nrows, 64, lds, s>>>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G);
+  hipFuncSetAttribute((const void *) k_descriptor<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+  k_descriptor<K, NS><<<nrows, 64, lds, s>>>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G);
   return hipGetLastError();
 }
 
-template <int K>
+template <int K, int NS>
 hipError_t force_impl(const ModelGeo &g, const Cell &cell, const double4 *pos, const uint32_t *nbr,
                       const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
                       const float *dEdG, double scale, double *f, double *wrow, double *vatom,
                       hipStream_t s)
 {
-  size_t lds = lds_bytes<K>(g.nelt, true);
+  size_t offs[8];
+  size_t lds = lds_layout<K>(g.nelt, cap, true, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
-  if (lds > 64 * 1024)
-    hipFuncSetAttribute((const void *) k_force<K>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-  k_force<K><<<nrows, 64, lds, s>>>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, scale,
-                                    f, wrow, vatom);
+  hipFuncSetAttribute((const void *) k_force<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+  k_force<K, NS><<<nrows, 64, lds, s>>>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, scale,
+                                        f, wrow, vatom);
   return hipGetLastError();
+}
+
+// slot capacity: the smallest of 1, 2, 4, 8 that holds nelt, within a
+// register budget of 128 accumulators per lane
+template <int K, bool FORCE>
+hipError_t dispatch(const ModelGeo &g, const Cell &cell, const double4 *pos, const uint32_t *nbr,
+                    const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
+                    float *G, const float *dEdG, double scale, double *f, double *wrow,
+                    double *vatom, hipStream_t s)
+{
+  constexpr int per = Tiling<K>::KT * Tiling<K>::DT;
+#define TMD_NS_CASE(NS)                                                                            \
+  if (g.nelt <= NS) {                                                                              \
+    if (NS * per > 128) return hipErrorInvalidConfiguration;                                       \
+    if (FORCE)                                                                                     \
+      return force_impl<K, (NS * per > 128 ? 1 : NS)>(g, cell, pos, nbr, nn, cap, row2atom, nrows, \
+                                                       tab, dEdG, scale, f, wrow, vatom, s);        \
+    return desc_impl<K, (NS * per > 128 ? 1 : NS)>(g, cell, pos, nbr, nn, cap, row2atom, nrows,    \
+                                                   tab, G, s);                                      \
+  }
+  TMD_NS_CASE(1)
+  TMD_NS_CASE(2)
+  TMD_NS_CASE(4)
+  TMD_NS_CASE(8)
+#undef TMD_NS_CASE
+  return hipErrorInvalidConfiguration;
 }
 
 }  // namespace
@@ -490,14 +543,19 @@ hipError_t launch_descriptor(const ModelGeo &g, const Cell &cell, const double4 
                              int nrows, const float4 *tab, float *G, hipStream_t stream)
 {
   if (nrows <= 0) return hipSuccess;
+#define TMD_K_CASE(K)                                                                          \
+  case K:                                                                                      \
+    return dispatch<K, false>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G, nullptr, 0, \
+                              nullptr, nullptr, nullptr, stream);
   switch (g.k) {
-    case 16: return desc_impl<16>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G, stream);
-    case 24: return desc_impl<24>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G, stream);
-    case 32: return desc_impl<32>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G, stream);
-    case 64: return desc_impl<64>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G, stream);
-    case 128: return desc_impl<128>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G, stream);
+    TMD_K_CASE(16)
+    TMD_K_CASE(24)
+    TMD_K_CASE(32)
+    TMD_K_CASE(64)
+    TMD_K_CASE(128)
     default: return hipErrorInvalidValue;
   }
+#undef TMD_K_CASE
 }
 
 hipError_t launch_force(const ModelGeo &g, const Cell &cell, const double4 *pos,
@@ -506,14 +564,19 @@ hipError_t launch_force(const ModelGeo &g, const Cell &cell, const double4 *pos,
                         double *wrow, double *vatom, hipStream_t stream)
 {
   if (nrows <= 0) return hipSuccess;
+#define TMD_K_CASE(K)                                                                           \
+  case K:                                                                                       \
+    return dispatch<K, true>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, nullptr, dEdG,   \
+                             scale, f, wrow, vatom, stream);
   switch (g.k) {
-    case 16: return force_impl<16>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, scale, f, wrow, vatom, stream);
-    case 24: return force_impl<24>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, scale, f, wrow, vatom, stream);
-    case 32: return force_impl<32>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, scale, f, wrow, vatom, stream);
-    case 64: return force_impl<64>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, scale, f, wrow, vatom, stream);
-    case 128: return force_impl<128>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, scale, f, wrow, vatom, stream);
+    TMD_K_CASE(16)
+    TMD_K_CASE(24)
+    TMD_K_CASE(32)
+    TMD_K_CASE(64)
+    TMD_K_CASE(128)
     default: return hipErrorInvalidValue;
   }
+#undef TMD_K_CASE
 }
 
 }  // namespace tmd

@@ -43,22 +43,28 @@ struct RowLayout {
   int row_off[kMaxElts + 1];     // element e occupies [row_off[e], row_off[e+1])
 };
 
-// One dense layer of every element, packed: W[e] at w + e * w_stride.
-struct LayerParams {
-  int K, N;            // in, out widths
-  int act;             // activation of this layer (hidden layers)
-  int res;             // resnet skip
-  const float *W;      // [K][N] fp32
-  const float *Wt;     // [N][K] fp32
-  const uint16_t *Wb;  // [K][N] bf16
-  const uint16_t *Wtb; // [N][K] bf16
-  const uint8_t *W8;   // [K][N] fp8 e4m3 (scaled per output column)
-  const uint8_t *Wt8;  // [N][K] fp8 e4m3 (scaled per output column of the backward GEMM)
-  const float *s8;     // [N] dequant scale of W8 columns
-  const float *st8;    // [K] dequant scale of Wt8 columns
-  const float *b;      // [N] or null
-  long w_stride;       // elements between elements' W (same for all arrays)
-  int b_stride;
+// The per-element MLP, all elements packed: layer l of element e starts at
+// W*[l] + e * wstride[l].  Forward B operands: Wf32 = W [K][N] (fp32),
+// Wtb = W^T [N][K] (bf16); backward: Wt32 = W^T [N][K] (fp32), Wb16 = W
+// [K][N] (bf16).  The LDS fields are filled by the launcher.
+constexpr int kMaxLayers = 8;
+struct MlpParams {
+  int nl;                        // layers including the width-1 output layer
+  int sizes[kMaxLayers + 1];     // sizes[0] = descriptor width
+  int act;
+  int res[kMaxLayers];           // resnet skip of hidden layer l
+  const float *Wf32[kMaxLayers];
+  const float *Wt32[kMaxLayers];
+  const uint16_t *Wtb[kMaxLayers];
+  const uint16_t *Wb16[kMaxLayers];
+  const float *b[kMaxLayers];    // hidden biases (or null)
+  const float *wout;             // output layer weights [K_last]
+  const float *bout;             // output bias (or null)
+  long wstride[kMaxLayers];
+  int bstride[kMaxLayers];
+  // LDS layout (elements of the mode's type)
+  int offH[kMaxLayers], offD[kMaxLayers], ld[kMaxLayers + 1];
+  int offGH, ldGH;
 };
 
 // ---- neighbor build ---------------------------------------------------------
@@ -96,18 +102,10 @@ hipError_t launch_force(const ModelGeo &g, const Cell &cell, const double4 *pos,
                         double *f, double *wrow, double *vatom, hipStream_t stream);
 
 // ---- MLP ----------------------------------------------------------------------
-// mode: 0 fp32, 1 bf16, 2 fp8
-hipError_t launch_dense_fwd(int mode, const RowLayout &rl, const LayerParams &lp,
-                            const void *A, int a_is_fp32, void *H, float *D, hipStream_t stream);
-// Output layer of width 1: E[row] = H.w + b
-hipError_t launch_output(const RowLayout &rl, const LayerParams &lp, const void *H, int h_is_fp32,
-                         float *E, hipStream_t stream);
-// Backward of a hidden layer: Gin = (Gout .* D) W^T (+ Gout if res).  If
-// gout_vec is non-null it replaces Gout by a per-element row vector
-// (the output-layer weights).  Gin is fp32 if gin_fp32 else the mode's type.
-hipError_t launch_dense_bwd(int mode, const RowLayout &rl, const LayerParams &lp,
-                            const void *Gout, const float *gout_vec, int vec_stride,
-                            const float *D, void *Gin, int gin_fp32, hipStream_t stream);
+// mode: 0 fp32, 1 bf16.  E[row] (fp32) and dEdG[row][width] (fp32).
+hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, float *E,
+                      float *dEdG, hipStream_t stream);
+size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out);
 
 // ---- reductions / integration ---------------------------------------------------
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t stream);

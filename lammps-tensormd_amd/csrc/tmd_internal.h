@@ -44,7 +44,6 @@ struct Workspace {
   RowLayout rl{};
   DeviceBuffer pos, nbr, nn, f, vatom;
   DeviceBuffer G, dEdG, E, wrow, row2atom, red;
-  std::vector<DeviceBuffer> H, D, gH;
 };
 
 }  // namespace tmd
@@ -59,7 +58,7 @@ struct tmd_ctx {
   std::vector<int> type_map;
   tmd::ModelGeo geo{};
   tmd::DeviceBuffer tab, params;
-  std::vector<tmd::LayerParams> layers;
+  tmd::MlpParams mlp{};
   tmd::Workspace list_ws;
 };
 

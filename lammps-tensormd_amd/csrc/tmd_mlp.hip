@@ -1,18 +1,25 @@
-// Per-element MLP forward and backward (dE/dG) as MFMA GEMMs over
-// element-sorted row tiles, plus the small reduction/integration kernels.
+// Per-element MLP forward + backward (dE/dG) in ONE kernel (gfx950 MFMA).
 //
-// Replaces forward_resnet_GPU / backward_resnet_GPU (gpu/cuda/nnp.cu:149-281,
-// cuBLAS GEMMs + axpy_* bias/activation kernels) and the CPU dense path
-// (src/ML-TENSORMD/nnp.cpp:246-366) with one launch per layer for all
-// elements: a 64-row tile never straddles two elements, and the tile's
-// element selects the weights.
+// Replaces forward_resnet_GPU / backward_resnet_GPU (gpu/cuda/nnp.cu:149-281:
+// one cuBLAS GEMM + one bias/activation kernel per layer, activations in
+// HBM) and the CPU dense path (src/ML-TENSORMD/nnp.cpp:246-366).
 //
-// FP32 mode uses v_mfma_f32_16x16x4_f32, BF16 mode v_mfma_f32_16x16x32_bf16
-// (gfx950), both with FP32 accumulation; the epilogue adds the bias, applies
-// the activation, adds the resnet input and stores act'(z) for the backward.
+// A workgroup (4 waves) owns R rows of one element (rows are element-sorted
+// and each element group starts at a multiple of 64, so R | 64 keeps a tile
+// inside one element).  Every hidden activation H_l and derivative
+// D_l = act'(z_l) of the tile stays in LDS; the waves split each layer's
+// output columns 16 at a time, reading A from LDS (layer 0: the FP32
+// descriptor rows from HBM) and streaming the element's weights from L2
+// straight into MFMA B operands.  The backward pass reuses the D_l slots for
+// G_l = dE/dH_l * D_l and writes dE/dG (FP32) to HBM.  HBM traffic per atom
+// is the descriptor read plus the gradient write.
+//
+// FP32 mode: v_mfma_f32_16x16x4_f32; BF16 mode: v_mfma_f32_16x16x32_bf16
+// (operands rounded to BF16, FP32 accumulation; H/D kept as BF16 in LDS).
+// Operand layouts (16x16 tiles): A lane l holds row l%16, k-group l/16;
+// B lane l holds column l%16, k-group l/16; C lane l holds rows
+// 4*(l/16)+i, column l%16.
 #include "tmd_device.h"
-
-#include <algorithm>
 
 namespace tmd {
 
@@ -20,26 +27,24 @@ namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef long fp8x8;
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
-__device__ inline float bf16_to_f32(uint16_t v) { return __uint_as_float(((uint32_t) v) << 16); }
-__device__ inline uint16_t f32_to_bf16(float f)
+__device__ inline uint16_t f2bf(float f)
 {
-  // round to nearest even (no NaN inputs on this path)
   uint32_t u = __float_as_uint(f);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t) (u >> 16);
 }
+__device__ inline float bf2f(uint16_t v) { return __uint_as_float(((uint32_t) v) << 16); }
 
-// activation and derivative (nnp.cu:9-49, activation.hpp:158-261)
+// activation value and derivative (nnp.cu:9-49, activation.hpp:158-261)
 template <int ACT>
 __device__ inline float act_f(float z, float &dz)
 {
-  if (ACT == 1) {  // softplus, linear above -(log(eps)+2) (nnp.cu:44-45)
-    float e = __expf(z);
-    const float thr = 13.942385f;  // -(log(1.19209e-07) + 2)
-    float h = (z < thr) ? log1pf(e) : z;
-    dz = 1.0f / (1.0f + 1.0f / e);
+  if (ACT == 1) {  // softplus; linear above -(log(eps_f32)+2) (nnp.cu:44-45)
+    float ez = __expf(z);
+    float h = (z < 13.942385f) ? log1pf(ez) : z;
+    dz = 1.0f / (1.0f + 1.0f / ez);
     return h;
   } else if (ACT == 2) {
     float h = tanhf(z);
@@ -50,11 +55,38 @@ __device__ inline float act_f(float z, float &dz)
     float h = 0.5f * (z + s);
     dz = h / s;
     return h;
-  } else {
-    dz = z >= 0.0f ? 1.0f : 0.0f;
-    return z >= 0.0f ? z : 0.0f;
   }
+  dz = z >= 0.0f ? 1.0f : 0.0f;
+  return z >= 0.0f ? z : 0.0f;
 }
+
+template <int MODE>
+struct Traits;
+template <>
+struct Traits<0> {
+  using T = float;
+  static constexpr int KS = 4;  // k per MFMA
+  static constexpr int CT = 4;  // column tiles per pass per wave
+};
+template <>
+struct Traits<1> {
+  using T = uint16_t;
+  static constexpr int KS = 32;
+  static constexpr int CT = 4;
+};
+
+template <typename T>
+__device__ inline float to_f(T v);
+template <>
+__device__ inline float to_f<float>(float v) { return v; }
+template <>
+__device__ inline float to_f<uint16_t>(uint16_t v) { return bf2f(v); }
+template <typename T>
+__device__ inline T from_f(float v);
+template <>
+__device__ inline float from_f<float>(float v) { return v; }
+template <>
+__device__ inline uint16_t from_f<uint16_t>(float v) { return f2bf(v); }
 
 __device__ inline int tile_elem(const RowLayout &rl, int row0)
 {
@@ -63,436 +95,308 @@ __device__ inline int tile_elem(const RowLayout &rl, int row0)
   return e;
 }
 
-template <typename T>
-__device__ inline float ld_f(const T *p, size_t i);
-template <>
-__device__ inline float ld_f<float>(const float *p, size_t i) { return p[i]; }
-template <>
-__device__ inline float ld_f<uint16_t>(const uint16_t *p, size_t i) { return bf16_to_f32(p[i]); }
-
-template <typename T>
-__device__ inline void st_f(T *p, size_t i, float v);
-template <>
-__device__ inline void st_f<float>(float *p, size_t i, float v) { p[i] = v; }
-template <>
-__device__ inline void st_f<uint16_t>(uint16_t *p, size_t i, float v) { p[i] = f32_to_bf16(v); }
-
-// ---------------------------------------------------------------------------
-// C[64 x 64 tile] = Aeff[rows x K] * B[K x N]
-//   PRO = 0: Aeff = A                (forward)
-//   PRO = 1: Aeff = Gout .* D         (backward; Gout row-major [M][Kd] where Kd = K)
-//   PRO = 2: Aeff = vec[k] * D        (backward of the last hidden layer)
-// B is [K][N] row-major fp32 (FP32 mode) or bf16 (BF16 mode).
-// 256 threads: 4 waves in 2x2, each wave 32x32 = 2x2 MFMA tiles of 16x16.
-constexpr int BM = 64, BN = 64;
-
-template <int PRO, typename TA>
-__device__ inline float load_a(const TA *A, const float *D, const float *vec, size_t row, int k,
-                               int ldk)
+// ---- one MFMA step ------------------------------------------------------------
+// A operand: from LDS tile (row stride ld) or from the FP32 descriptor rows.
+template <int MODE, bool AGLOBAL>
+__device__ inline void load_a(const typename Traits<MODE>::T *lds, int ld, const float *G, int gw,
+                              int row, int k, int K, float &af, bf16x8 &ab)
 {
-  if (PRO == 0) return ld_f<TA>(A, row * ldk + k);
-  if (PRO == 1) return ld_f<TA>(A, row * ldk + k) * D[row * ldk + k];
-  return vec[k] * D[row * ldk + k];
-}
-
-// ---- FP32 -------------------------------------------------------------------
-template <int PRO, typename TA>
-__device__ inline void gemm_f32(const TA *A, const float *D, const float *vec, int K, int N,
-                                const float *B, int row0, int col0, f32x4 acc[2][2])
-{
-  constexpr int BK = 16;
-  __shared__ float sA[BM][BK + 1];
-  __shared__ float sB[BK][BN + 4];
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0, 0, 0, 0};
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    // A tile: 64 x 16, 4 per thread
-    {
-      int r = t >> 2, c = (t & 3) * 4;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        int k = k0 + c + q;
-        sA[r][c + q] = (k < K) ? load_a<PRO, TA>(A, D, vec, (size_t) (row0 + r), k, K) : 0.0f;
+  const int lane = threadIdx.x & 63;
+  if (MODE == 0) {
+    int kk = k + (lane >> 4);
+    if (AGLOBAL)
+      af = (kk < K) ? G[(size_t) row * gw + kk] : 0.0f;
+    else
+      af = (kk < K) ? ((const float *) lds)[row * ld + kk] : 0.0f;
+  } else {
+    int kk = k + (lane >> 4) * 8;
+    if (AGLOBAL) {
+      u16x8 v;
+      if (kk + 8 <= K) {
+        const float4 *p = reinterpret_cast<const float4 *>(G + (size_t) row * gw + kk);
+        float4 x0 = p[0], x1 = p[1];
+        v[0] = f2bf(x0.x); v[1] = f2bf(x0.y); v[2] = f2bf(x0.z); v[3] = f2bf(x0.w);
+        v[4] = f2bf(x1.x); v[5] = f2bf(x1.y); v[6] = f2bf(x1.z); v[7] = f2bf(x1.w);
+      } else {
+        for (int q = 0; q < 8; q++) v[q] = (kk + q < K) ? f2bf(G[(size_t) row * gw + kk + q]) : 0;
+      }
+      ab = __builtin_bit_cast(bf16x8, v);
+    } else {
+      const uint16_t *p = (const uint16_t *) lds + row * ld + kk;
+      if (kk + 8 <= K) {
+        ab = *reinterpret_cast<const bf16x8 *>(p);
+      } else {
+        u16x8 v;
+        for (int q = 0; q < 8; q++) v[q] = (kk + q < K) ? p[q] : 0;
+        ab = __builtin_bit_cast(bf16x8, v);
       }
     }
-    // B tile: 16 x 64, 4 per thread
-    {
-      int r = t >> 4, c = (t & 15) * 4;
-      int k = k0 + r;
+  }
+}
+
+// B operand for output column `col`, inner index k:
+//   FP32: B row-major [K][N] (bf[k*N + col]); BF16: B stored [N][K] (k contiguous)
+template <int MODE>
+__device__ inline void load_b(const void *B, int K, int N, int col, int k, float &bf, bf16x8 &bb)
+{
+  const int lane = threadIdx.x & 63;
+  if (MODE == 0) {
+    int kk = k + (lane >> 4);
+    bf = (kk < K && col < N) ? ((const float *) B)[(size_t) kk * N + col] : 0.0f;
+  } else {
+    int kk = k + (lane >> 4) * 8;
+    const uint16_t *p = (const uint16_t *) B + (size_t) col * K + kk;
+    if (col < N && kk + 8 <= K) {
+      bb = *reinterpret_cast<const bf16x8 *>(p);
+    } else {
+      u16x8 v;
+      for (int q = 0; q < 8; q++) v[q] = (col < N && kk + q < K) ? p[q] : 0;
+      bb = __builtin_bit_cast(bf16x8, v);
+    }
+  }
+}
+
+// Wave-level GEMM over all R rows for up to CT column tiles (tiles[j] < 0 =
+// unused): acc[rt][j] = sum_k A[rt*16.., k] B[k, tiles[j]*16..]
+template <int MODE, int R, bool AGLOBAL>
+__device__ inline void wave_gemm(const typename Traits<MODE>::T *lds, int ld, const float *G, int gw,
+                                 int grow0, const void *B, int K, int N, const int *tiles,
+                                 f32x4 acc[R / 16][Traits<MODE>::CT])
+{
+  constexpr int CT = Traits<MODE>::CT, KS = Traits<MODE>::KS, RT = R / 16;
+  const int lane = threadIdx.x & 63;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        int n = col0 + c + q;
-        sB[r][c + q] = (k < K && n < N) ? B[(size_t) k * N + n] : 0.0f;
+  for (int r = 0; r < RT; r++)
+#pragma unroll
+    for (int j = 0; j < CT; j++) acc[r][j] = f32x4{0, 0, 0, 0};
+  for (int k = 0; k < K; k += KS) {
+    float bf[CT];
+    bf16x8 bb[CT];
+#pragma unroll
+    for (int j = 0; j < CT; j++)
+      if (tiles[j] >= 0) load_b<MODE>(B, K, N, tiles[j] * 16 + (lane & 15), k, bf[j], bb[j]);
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+      float af;
+      bf16x8 ab;
+      int row = r * 16 + (lane & 15);
+      load_a<MODE, AGLOBAL>(lds, ld, G, gw, AGLOBAL ? grow0 + row : row, k, K, af, ab);
+#pragma unroll
+      for (int j = 0; j < CT; j++) {
+        if (tiles[j] < 0) continue;
+        if (MODE == 0)
+          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af, bf[j], acc[r][j], 0, 0, 0);
+        else
+          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab, bb[j], acc[r][j], 0, 0, 0);
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      float av[2], bv[2];
-#pragma unroll
-      for (int a = 0; a < 2; a++) av[a] = sA[wm * 32 + a * 16 + (lane & 15)][kk + (lane >> 4)];
-#pragma unroll
-      for (int b = 0; b < 2; b++) bv[b] = sB[kk + (lane >> 4)][wn * 32 + b * 16 + (lane & 15)];
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
-    }
-    __syncthreads();
   }
 }
-
-// ---- BF16 -------------------------------------------------------------------
-template <int PRO, typename TA>
-__device__ inline void gemm_bf16(const TA *A, const float *D, const float *vec, int K, int N,
-                                 const uint16_t *B, int row0, int col0, f32x4 acc[2][2])
-{
-  constexpr int BK = 32;
-  // k-contiguous rows so one lane's 8 consecutive k are one 16-byte read
-  __shared__ __align__(16) uint16_t sA[BM][BK + 8];
-  __shared__ __align__(16) uint16_t sBt[BN][BK + 8];
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0, 0, 0, 0};
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    {
-      // A: 64 x 32, 8 per thread
-      int r = t >> 2, c = (t & 3) * 8;
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        int k = k0 + c + q;
-        float v = (k < K) ? load_a<PRO, TA>(A, D, vec, (size_t) (row0 + r), k, K) : 0.0f;
-        sA[r][c + q] = f32_to_bf16(v);
-      }
-    }
-    {
-      // B: 32 x 64 -> transposed into sBt[n][k], 8 per thread
-      int r = t >> 3, c = (t & 7) * 8;
-      int k = k0 + r;
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        int n = col0 + c + q;
-        sBt[c + q][r] = (k < K && n < N) ? B[(size_t) k * N + n] : (uint16_t) 0;
-      }
-    }
-    __syncthreads();
-    bf16x8 av[2], bv[2];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-      av[a] = *reinterpret_cast<const bf16x8 *>(&sA[wm * 32 + a * 16 + (lane & 15)][(lane >> 4) * 8]);
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-      bv[b] = *reinterpret_cast<const bf16x8 *>(&sBt[wn * 32 + b * 16 + (lane & 15)][(lane >> 4) * 8]);
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
-    __syncthreads();
-  }
-}
-
-// Forward hidden layer: H = act(A W + b) (+ A if res), D = act'(.)
-template <int MODE, int ACT, typename TA, typename TH>
-__global__ __launch_bounds__(256) void k_dense_fwd(RowLayout rl, LayerParams lp, const TA *A, TH *H,
-                                                   float *D)
-{
-  const int row0 = blockIdx.x * BM, col0 = blockIdx.y * BN;
-  if (row0 >= rl.nrows) return;
-  const int e = tile_elem(rl, row0);
-  f32x4 acc[2][2];
-  if (MODE == 0)
-    gemm_f32<0, TA>(A, nullptr, nullptr, lp.K, lp.N, lp.W + e * lp.w_stride, row0, col0, acc);
-  else
-    gemm_bf16<0, TA>(A, nullptr, nullptr, lp.K, lp.N, lp.Wb + e * lp.w_stride, row0, col0, acc);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const float *bias = lp.b ? lp.b + e * lp.b_stride : nullptr;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        int r = row0 + wm * 32 + a * 16 + (lane >> 4) * 4 + q;
-        int c = col0 + wn * 32 + b * 16 + (lane & 15);
-        if (c >= lp.N) continue;
-        float z = acc[a][b][q] + (bias ? bias[c] : 0.0f);
-        float dz;
-        float h = act_f<ACT>(z, dz);
-        if (lp.res) h += ld_f<TA>(A, (size_t) r * lp.K + c);
-        st_f<TH>(H, (size_t) r * lp.N + c, h);
-        D[(size_t) r * lp.N + c] = dz;
-      }
-}
-
-// Backward: Gin = Aeff W^T (+ Gout if res).  B = W^T stored [N][K] -> here
-// the GEMM inner dimension is the layer's N and the output width is K.
-template <int MODE, int PRO, typename TG, typename TO>
-__global__ __launch_bounds__(256) void k_dense_bwd(RowLayout rl, LayerParams lp, const TG *Gout,
-                                                   const float *gvec, int vec_stride,
-                                                   const float *D, TO *Gin)
-{
-  const int row0 = blockIdx.x * BM, col0 = blockIdx.y * BN;
-  if (row0 >= rl.nrows) return;
-  const int e = tile_elem(rl, row0);
-  const float *vec = gvec ? gvec + (size_t) e * vec_stride : nullptr;
-  f32x4 acc[2][2];
-  // GEMM: [M x N] * [N x K]
-  if (MODE == 0)
-    gemm_f32<PRO, TG>(Gout, D, vec, lp.N, lp.K, lp.Wt + e * lp.w_stride, row0, col0, acc);
-  else
-    gemm_bf16<PRO, TG>(Gout, D, vec, lp.N, lp.K, lp.Wtb + e * lp.w_stride, row0, col0, acc);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        int r = row0 + wm * 32 + a * 16 + (lane >> 4) * 4 + q;
-        int c = col0 + wn * 32 + b * 16 + (lane & 15);
-        if (c >= lp.K) continue;
-        float v = acc[a][b][q];
-        if (lp.res) v += (PRO == 2) ? vec[c] : ld_f<TG>(Gout, (size_t) r * lp.N + c);
-        st_f<TO>(Gin, (size_t) r * lp.K + c, v);
-      }
-}
-
-// Output layer (width 1): one wave per row group, E = H . w + b
-template <typename TH>
-__global__ void k_output(RowLayout rl, LayerParams lp, const TH *H, float *E)
-{
-  int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (row >= rl.nrows) return;
-  int e = tile_elem(rl, row & ~(kRowTile - 1));
-  const float *w = lp.W + e * lp.w_stride;
-  float s = 0.0f;
-  for (int k = lane; k < lp.K; k += 64) s += ld_f<TH>(H, (size_t) row * lp.K + k) * w[k];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) E[row] = s + (lp.b ? lp.b[e * lp.b_stride] : 0.0f);
-}
-
-// ---------------------------------------------------------------------------
-template <int MODE, typename TA, typename TH>
-hipError_t fwd_act(int act, const RowLayout &rl, const LayerParams &lp, const TA *A, TH *H, float *D,
-                   hipStream_t s)
-{
-  dim3 grid(rl.nrows / BM, (lp.N + BN - 1) / BN);
-  switch (act) {
-    case 0: k_dense_fwd<MODE, 0, TA, TH><<<grid, 256, 0, s>>>(rl, lp, A, H, D); break;
-    case 1: k_dense_fwd<MODE, 1, TA, TH><<<grid, 256, 0, s>>>(rl, lp, A, H, D); break;
-    case 2: k_dense_fwd<MODE, 2, TA, TH><<<grid, 256, 0, s>>>(rl, lp, A, H, D); break;
-    default: k_dense_fwd<MODE, 3, TA, TH><<<grid, 256, 0, s>>>(rl, lp, A, H, D); break;
-  }
-  return hipGetLastError();
-}
-
-// reductions --------------------------------------------------------------------
-__global__ void k_zero(uint4 *p, size_t n16, unsigned char *tail, size_t ntail)
-{
-  size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t) gridDim.x * blockDim.x;
-  for (size_t q = i; q < n16; q += stride) p[q] = make_uint4(0, 0, 0, 0);
-  if (i < ntail) tail[i] = 0;
-}
-
-// fixed-order two-level reduction: each block writes a partial, the last
-// kernel sums partials in order (deterministic)
-__global__ void k_reduce_ev_part(const float *E, const double *wrow, const int *row2atom, int nrows,
-                                 double *part)
-{
-  __shared__ double sh[7][256];
-  int t = threadIdx.x;
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int r = blockIdx.x * 256 + t; r < nrows; r += gridDim.x * 256) {
-    if (row2atom[r] < 0) continue;
-    v[0] += (double) E[r];
-    for (int q = 0; q < 6; q++) v[1 + q] += wrow[(size_t) r * 6 + q];
-  }
-  for (int q = 0; q < 7; q++) sh[q][t] = v[q];
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (t < o)
-      for (int q = 0; q < 7; q++) sh[q][t] += sh[q][t + o];
-    __syncthreads();
-  }
-  if (t < 7) part[(size_t) blockIdx.x * 7 + t] = sh[t][0];
-}
-
-__global__ void k_reduce_ev_final(const double *part, int nparts, double scale, double *out)
-{
-  int t = threadIdx.x;
-  if (t < 7) {
-    double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[(size_t) p * 7 + t];
-    out[t] = (t == 0) ? s * scale : s;
-  }
-}
-
-__global__ void k_vv_first(int n, double4 *pos, double *v, const double *f, const double *dtfm,
-                           double dt)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double4 p = pos[i];
-  double c = dtfm[(int) p.w];
-  double vx = v[3 * i] + c * f[3 * i];
-  double vy = v[3 * i + 1] + c * f[3 * i + 1];
-  double vz = v[3 * i + 2] + c * f[3 * i + 2];
-  v[3 * i] = vx;
-  v[3 * i + 1] = vy;
-  v[3 * i + 2] = vz;
-  p.x += dt * vx;
-  p.y += dt * vy;
-  p.z += dt * vz;
-  pos[i] = p;
-}
-
-__global__ void k_vv_second(int n, const double4 *pos, double *v, const double *f, const double *dtfm)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double c = dtfm[(int) pos[i].w];
-  v[3 * i] += c * f[3 * i];
-  v[3 * i + 1] += c * f[3 * i + 1];
-  v[3 * i + 2] += c * f[3 * i + 2];
-}
-
-__global__ void k_ekin_part(int n, const double4 *pos, const double *v, const double *mass,
-                            double *part)
-{
-  __shared__ double sh[256];
-  int t = threadIdx.x;
-  double s = 0.0;
-  for (int i = blockIdx.x * 256 + t; i < n; i += gridDim.x * 256) {
-    double m = mass[(int) pos[i].w];
-    s += m * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
-  }
-  sh[t] = s;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (t < o) sh[t] += sh[t + o];
-    __syncthreads();
-  }
-  if (t == 0) part[blockIdx.x] = sh[0];
-}
-
-__global__ void k_ekin_final(const double *part, int nparts, double *out)
-{
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[p];
-    out[0] = 0.5 * 1.0364269e-4 * s;  // mvv2e (metal units)
-  }
-}
-
-__global__ void k_copy_scalar(const double *src, double *dst)
-{
-  if (threadIdx.x == 0) dst[0] = src[0];
-}
-
-constexpr int kRedParts = 240;
 
 }  // namespace
 
-hipError_t launch_dense_fwd(int mode, const RowLayout &rl, const LayerParams &lp, const void *A,
-                            int a_is_fp32, void *H, float *D, hipStream_t s)
+template <int MODE, int ACT, int R>
+__global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const float *Gd, float *E,
+                                             float *dEdG)
 {
-  if (mode == 0)
-    return fwd_act<0, float, float>(lp.act, rl, lp, (const float *) A, (float *) H, D, s);
-  if (a_is_fp32)
-    return fwd_act<1, float, uint16_t>(lp.act, rl, lp, (const float *) A, (uint16_t *) H, D, s);
-  return fwd_act<1, uint16_t, uint16_t>(lp.act, rl, lp, (const uint16_t *) A, (uint16_t *) H, D, s);
-}
+  using T = typename Traits<MODE>::T;
+  constexpr int CT = Traits<MODE>::CT, RT = R / 16;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *sm = reinterpret_cast<T *>(smem_raw);
+  const int row0 = blockIdx.x * R;
+  if (row0 >= rl.nrows) return;
+  const int e = tile_elem(rl, row0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gw = mp.sizes[0];
+  const int L = mp.nl;
 
-hipError_t launch_output(const RowLayout &rl, const LayerParams &lp, const void *H, int h_is_fp32,
-                         float *E, hipStream_t s)
-{
-  int blocks = (rl.nrows + 3) / 4;
-  if (h_is_fp32)
-    k_output<float><<<blocks, 256, 0, s>>>(rl, lp, (const float *) H, E);
-  else
-    k_output<uint16_t><<<blocks, 256, 0, s>>>(rl, lp, (const uint16_t *) H, E);
-  return hipGetLastError();
-}
-
-hipError_t launch_dense_bwd(int mode, const RowLayout &rl, const LayerParams &lp, const void *Gout,
-                            const float *gout_vec, int vec_stride, const float *D, void *Gin,
-                            int gin_fp32, hipStream_t s)
-{
-  dim3 grid(rl.nrows / BM, (lp.K + BN - 1) / BN);
-  if (mode == 0) {
-    if (gout_vec)
-      k_dense_bwd<0, 2, float, float><<<grid, 256, 0, s>>>(rl, lp, nullptr, gout_vec, vec_stride, D, (float *) Gin);
-    else
-      k_dense_bwd<0, 1, float, float><<<grid, 256, 0, s>>>(rl, lp, (const float *) Gout, nullptr, 0, D, (float *) Gin);
-  } else {
-    // BF16 mode: gradients between layers are kept in FP32 (they are
-    // rounded to BF16 only as MFMA operands)
-    if (gout_vec)
-      k_dense_bwd<1, 2, float, float><<<grid, 256, 0, s>>>(rl, lp, nullptr, gout_vec, vec_stride, D, (float *) Gin);
-    else
-      k_dense_bwd<1, 1, float, float><<<grid, 256, 0, s>>>(rl, lp, (const float *) Gout, nullptr, 0, D, (float *) Gin);
+  // ---------------- forward ----------------
+  for (int l = 0; l < L - 1; l++) {
+    const int K = mp.sizes[l], N = mp.sizes[l + 1];
+    const void *B = MODE == 0 ? (const void *) (mp.Wf32[l] + (size_t) e * mp.wstride[l])
+                              : (const void *) (mp.Wtb[l] + (size_t) e * mp.wstride[l]);
+    const float *bias = mp.b[l] ? mp.b[l] + (size_t) e * mp.bstride[l] : nullptr;
+    T *H = sm + mp.offH[l];
+    T *D = sm + mp.offD[l];
+    const int ldo = mp.ld[l + 1];
+    const T *Hin = l > 0 ? sm + mp.offH[l - 1] : nullptr;
+    const int ldi = mp.ld[l];
+    const int NT = (N + 15) / 16;
+    for (int t0 = wave; t0 < NT; t0 += 4 * CT) {
+      int tiles[CT];
+#pragma unroll
+      for (int j = 0; j < CT; j++) tiles[j] = (t0 + 4 * j < NT) ? t0 + 4 * j : -1;
+      f32x4 acc[RT][CT];
+      if (l == 0)
+        wave_gemm<MODE, R, true>(nullptr, 0, Gd, gw, row0, B, K, N, tiles, acc);
+      else
+        wave_gemm<MODE, R, false>(Hin, ldi, nullptr, gw, row0, B, K, N, tiles, acc);
+#pragma unroll
+      for (int j = 0; j < CT; j++) {
+        if (tiles[j] < 0) continue;
+        int col = tiles[j] * 16 + (lane & 15);
+        if (col >= N) continue;
+        float bv = bias ? bias[col] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            int row = r * 16 + (lane >> 4) * 4 + i;
+            float dz;
+            float h = act_f<ACT>(acc[r][j][i] + bv, dz);
+            if (mp.res[l]) h += to_f<T>(Hin[row * ldi + col]);
+            H[row * ldo + col] = from_f<T>(h);
+            D[row * ldo + col] = from_f<T>(dz);
+          }
+      }
+    }
+    __syncthreads();
   }
-  (void) gin_fp32;
-  return hipGetLastError();
+
+  // ---------------- output layer: E = H w + b (FP64 sum) ----------------
+  {
+    const int K = mp.sizes[L - 1];
+    const T *H = sm + mp.offH[L - 2];
+    const int ld = mp.ld[L - 1];
+    const float *w = mp.wout + (size_t) e * mp.wstride[L - 1];
+    const float bo = mp.bout ? mp.bout[(size_t) e * mp.bstride[L - 1]] : 0.0f;
+    for (int r = wave; r < R; r += 4) {
+      double s = 0.0;
+      for (int k = lane; k < K; k += 64) s += (double) (to_f<T>(H[r * ld + k]) * w[k]);
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      if (lane == 0) E[row0 + r] = (float) (s + (double) bo);
+    }
+  }
+  // seed: G_{L-2} = w_out * D_{L-2} (in the D slot); gradH = w_out for a resnet layer
+  {
+    const int N = mp.sizes[L - 1];
+    const float *w = mp.wout + (size_t) e * mp.wstride[L - 1];
+    T *D = sm + mp.offD[L - 2];
+    T *gH = sm + mp.offGH;
+    const int ld = mp.ld[L - 1];
+    for (int q = threadIdx.x; q < R * N; q += 256) {
+      int r = q / N, c = q - r * N;
+      D[r * ld + c] = from_f<T>(w[c] * to_f<T>(D[r * ld + c]));
+      if (mp.res[L - 2]) gH[r * mp.ldGH + c] = from_f<T>(w[c]);
+    }
+  }
+  __syncthreads();
+
+  // ---------------- backward ----------------
+  for (int l = L - 2; l >= 0; l--) {
+    const int K = mp.sizes[l], N = mp.sizes[l + 1];
+    // Gin[R x K] = G_l[R x N] W_l^T : inner N, output columns K
+    const void *B = MODE == 0 ? (const void *) (mp.Wt32[l] + (size_t) e * mp.wstride[l])
+                              : (const void *) (mp.Wb16[l] + (size_t) e * mp.wstride[l]);
+    const T *Gl = sm + mp.offD[l];
+    const int ldg = mp.ld[l + 1];
+    T *gH = sm + mp.offGH;
+    const int NT = (K + 15) / 16;
+    for (int t0 = wave; t0 < NT; t0 += 4 * CT) {
+      int tiles[CT];
+#pragma unroll
+      for (int j = 0; j < CT; j++) tiles[j] = (t0 + 4 * j < NT) ? t0 + 4 * j : -1;
+      f32x4 acc[RT][CT];
+      wave_gemm<MODE, R, false>(Gl, ldg, nullptr, gw, row0, B, N, K, tiles, acc);
+#pragma unroll
+      for (int j = 0; j < CT; j++) {
+        if (tiles[j] < 0) continue;
+        int col = tiles[j] * 16 + (lane & 15);
+        if (col >= K) continue;
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            int row = r * 16 + (lane >> 4) * 4 + i;
+            float v = acc[r][j][i];
+            if (mp.res[l]) v += to_f<T>(gH[row * mp.ldGH + col]);  // dE/dH_l skip path
+            if (l == 0) {
+              dEdG[(size_t) (row0 + row) * gw + col] = v;
+            } else {
+              T *D = sm + mp.offD[l - 1];
+              const int ld = mp.ld[l];
+              D[row * ld + col] = from_f<T>(v * to_f<T>(D[row * ld + col]));
+              if (mp.res[l - 1]) gH[row * mp.ldGH + col] = from_f<T>(v);
+            }
+          }
+      }
+    }
+    __syncthreads();
+  }
 }
 
-hipError_t launch_zero(void *p, size_t bytes, hipStream_t s)
+namespace {
+template <int MODE, int R>
+hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, float *E, float *dEdG,
+                    size_t lds, hipStream_t s)
 {
-  if (bytes == 0) return hipSuccess;
-  size_t n16 = bytes / 16, tail = bytes - n16 * 16;
-  int blocks = (int) std::min<size_t>(4096, (n16 + 255) / 256 + 1);
-  k_zero<<<blocks, 256, 0, s>>>((uint4 *) p, n16, (unsigned char *) p + n16 * 16, tail);
+  dim3 grid(rl.nrows / R);
+#define TMD_MLP_CASE(A)                                                                          \
+  case A:                                                                                        \
+    hipFuncSetAttribute((const void *) k_mlp<MODE, A, R>,                                        \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                  \
+    k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG);                                \
+    break;
+  switch (mp.act) {
+    TMD_MLP_CASE(0)
+    TMD_MLP_CASE(1)
+    TMD_MLP_CASE(2)
+    default:
+      TMD_MLP_CASE(3)
+  }
+#undef TMD_MLP_CASE
   return hipGetLastError();
 }
+}  // namespace
 
-hipError_t launch_reduce_ev(const float *E, const double *wrow, const int *row2atom, int nrows,
-                            double scale, double *out, hipStream_t s)
+// LDS layout for R rows: H_l and D_l per hidden layer (+ dE/dH for resnet).
+size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
 {
-  // out must have room for 7 + kRedParts*7 doubles: partials live after out[7]
-  double *part = out + 8;
-  k_reduce_ev_part<<<kRedParts, 256, 0, s>>>(E, wrow, row2atom, nrows, part);
-  k_reduce_ev_final<<<1, 64, 0, s>>>(part, kRedParts, scale, out);
-  return hipGetLastError();
+  MlpParams m = mp;
+  size_t esz = mode == 0 ? 4 : 2;
+  size_t off = 0;
+  int maxw = 0;
+  bool anyres = false;
+  for (int l = 0; l <= mp.nl - 1; l++) {
+    int w = mp.sizes[l];
+    m.ld[l] = (mode == 0) ? w + 4 : w + 8;
+  }
+  for (int l = 0; l < mp.nl - 1; l++) {
+    int N = mp.sizes[l + 1];
+    maxw = std::max(maxw, N);
+    if (mp.res[l]) anyres = true;
+    m.offH[l] = (int) off;
+    off += (size_t) R * m.ld[l + 1];
+    m.offD[l] = (int) off;
+    off += (size_t) R * m.ld[l + 1];
+  }
+  m.ldGH = (mode == 0) ? maxw + 4 : maxw + 8;
+  m.offGH = (int) off;
+  if (anyres) off += (size_t) R * m.ldGH;
+  if (out) *out = m;
+  return off * esz;
 }
 
-hipError_t launch_vv_first(int n, double4 *pos, double *v, const double *f, const double *dtfm,
-                           double dt, hipStream_t s)
+hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, float *E,
+                      float *dEdG, hipStream_t s)
 {
-  k_vv_first<<<(n + 255) / 256, 256, 0, s>>>(n, pos, v, f, dtfm, dt);
-  return hipGetLastError();
-}
-
-hipError_t launch_vv_second(int n, const double4 *pos, double *v, const double *f,
-                            const double *dtfm, hipStream_t s)
-{
-  k_vv_second<<<(n + 255) / 256, 256, 0, s>>>(n, pos, v, f, dtfm);
-  return hipGetLastError();
-}
-
-hipError_t launch_ekin(int n, const double4 *pos, const double *v, const double *mass, double *out,
-                       hipStream_t s)
-{
-  double *part = out + 8;
-  k_ekin_part<<<kRedParts, 256, 0, s>>>(n, pos, v, mass, part);
-  k_ekin_final<<<1, 64, 0, s>>>(part, kRedParts, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_copy_scalar(const double *src, double *dst, hipStream_t s)
-{
-  k_copy_scalar<<<1, 64, 0, s>>>(src, dst);
-  return hipGetLastError();
+  if (rl.nrows <= 0) return hipSuccess;
+  const size_t limit = 160 * 1024;
+  static const int Rs[3] = {64, 32, 16};
+  for (int q = (mode == 0 ? 1 : 0); q < 3; q++) {
+    int R = Rs[q];
+    MlpParams m;
+    size_t lds = mlp_lds_bytes(mp, mode, R, &m);
+    if (lds > limit) continue;
+    if (mode == 0) {
+      if (R == 32) return launch_r<0, 32>(rl, m, G, E, dEdG, lds, s);
+      return launch_r<0, 16>(rl, m, G, E, dEdG, lds, s);
+    }
+    if (R == 64) return launch_r<1, 64>(rl, m, G, E, dEdG, lds, s);
+    if (R == 32) return launch_r<1, 32>(rl, m, G, E, dEdG, lds, s);
+    return launch_r<1, 16>(rl, m, G, E, dEdG, lds, s);
+  }
+  return hipErrorInvalidConfiguration;
 }
 
 }  // namespace tmd

@@ -1,0 +1,166 @@
+// Small device kernels: buffer clearing (a kernel, so it is safe inside
+// captured graphs), fixed-order FP64 reductions of energy/virial and kinetic
+// energy, and the two velocity-Verlet half steps (src/fix_nve.cpp:64-130).
+#include "tmd_device.h"
+
+#include <algorithm>
+
+namespace tmd {
+
+namespace {
+
+constexpr int kRedParts = 240;
+
+__global__ void k_zero(uint4 *p, size_t n16, unsigned char *tail, size_t ntail)
+{
+  size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for (size_t q = i; q < n16; q += stride) p[q] = make_uint4(0, 0, 0, 0);
+  if (i < ntail) tail[i] = 0;
+}
+
+// Each block writes one partial; the final kernel adds partials in order,
+// so the result does not depend on scheduling.
+__global__ void k_reduce_ev_part(const float *E, const double *wrow, const int *row2atom, int nrows,
+                                 double *part)
+{
+  __shared__ double sh[7][256];
+  int t = threadIdx.x;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int r = blockIdx.x * 256 + t; r < nrows; r += gridDim.x * 256) {
+    if (row2atom[r] < 0) continue;
+    v[0] += (double) E[r];
+    for (int q = 0; q < 6; q++) v[1 + q] += wrow[(size_t) r * 6 + q];
+  }
+  for (int q = 0; q < 7; q++) sh[q][t] = v[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int q = 0; q < 7; q++) sh[q][t] += sh[q][t + o];
+    __syncthreads();
+  }
+  if (t < 7) part[(size_t) blockIdx.x * 7 + t] = sh[t][0];
+}
+
+__global__ void k_reduce_ev_final(const double *part, int nparts, double scale, double *out)
+{
+  int t = threadIdx.x;
+  if (t < 7) {
+    double s = 0.0;
+    for (int p = 0; p < nparts; p++) s += part[(size_t) p * 7 + t];
+    out[t] = (t == 0) ? s * scale : s;
+  }
+}
+
+__global__ void k_vv_first(int n, double4 *pos, double *v, const double *f, const double *dtfm,
+                           double dt)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 p = pos[i];
+  double c = dtfm[(int) p.w];
+  double vx = v[3 * i] + c * f[3 * i];
+  double vy = v[3 * i + 1] + c * f[3 * i + 1];
+  double vz = v[3 * i + 2] + c * f[3 * i + 2];
+  v[3 * i] = vx;
+  v[3 * i + 1] = vy;
+  v[3 * i + 2] = vz;
+  p.x += dt * vx;
+  p.y += dt * vy;
+  p.z += dt * vz;
+  pos[i] = p;
+}
+
+__global__ void k_vv_second(int n, const double4 *pos, double *v, const double *f, const double *dtfm)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double c = dtfm[(int) pos[i].w];
+  v[3 * i] += c * f[3 * i];
+  v[3 * i + 1] += c * f[3 * i + 1];
+  v[3 * i + 2] += c * f[3 * i + 2];
+}
+
+__global__ void k_ekin_part(int n, const double4 *pos, const double *v, const double *mass,
+                            double *part)
+{
+  __shared__ double sh[256];
+  int t = threadIdx.x;
+  double s = 0.0;
+  for (int i = blockIdx.x * 256 + t; i < n; i += gridDim.x * 256) {
+    double m = mass[(int) pos[i].w];
+    s += m * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+  }
+  sh[t] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) sh[t] += sh[t + o];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_ekin_final(const double *part, int nparts, double *out)
+{
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int p = 0; p < nparts; p++) s += part[p];
+    out[0] = 0.5 * 1.0364269e-4 * s;  // mvv2e, metal units
+  }
+}
+
+__global__ void k_copy_scalar(const double *src, double *dst)
+{
+  if (threadIdx.x == 0) dst[0] = src[0];
+}
+
+}  // namespace
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s)
+{
+  if (bytes == 0) return hipSuccess;
+  size_t n16 = bytes / 16, tail = bytes - n16 * 16;
+  int blocks = (int) std::min<size_t>(4096, (n16 + 255) / 256 + 1);
+  k_zero<<<blocks, 256, 0, s>>>((uint4 *) p, n16, (unsigned char *) p + n16 * 16, tail);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_ev(const float *E, const double *wrow, const int *row2atom, int nrows,
+                            double scale, double *out, hipStream_t s)
+{
+  double *part = out + 8;  // out holds 8 + kRedParts*7 doubles
+  k_reduce_ev_part<<<kRedParts, 256, 0, s>>>(E, wrow, row2atom, nrows, part);
+  k_reduce_ev_final<<<1, 64, 0, s>>>(part, kRedParts, scale, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_vv_first(int n, double4 *pos, double *v, const double *f, const double *dtfm,
+                           double dt, hipStream_t s)
+{
+  k_vv_first<<<(n + 255) / 256, 256, 0, s>>>(n, pos, v, f, dtfm, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_vv_second(int n, const double4 *pos, double *v, const double *f,
+                            const double *dtfm, hipStream_t s)
+{
+  k_vv_second<<<(n + 255) / 256, 256, 0, s>>>(n, pos, v, f, dtfm);
+  return hipGetLastError();
+}
+
+hipError_t launch_ekin(int n, const double4 *pos, const double *v, const double *mass, double *out,
+                       hipStream_t s)
+{
+  double *part = out + 8;
+  k_ekin_part<<<kRedParts, 256, 0, s>>>(n, pos, v, mass, part);
+  k_ekin_final<<<1, 64, 0, s>>>(part, kRedParts, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_scalar(const double *src, double *dst, hipStream_t s)
+{
+  k_copy_scalar<<<1, 64, 0, s>>>(src, dst);
+  return hipGetLastError();
+}
+
+}  // namespace tmd

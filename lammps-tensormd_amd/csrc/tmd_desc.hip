@@ -154,22 +154,63 @@ __device__ inline LDSMap carve(unsigned char *base, int nelt, int cap, bool forc
 }
 
 // Phase 1: compact the kept entries (slot-sorted).  start[b] .. start[b+1].
+__device__ inline int kept_slot(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
+                                uint32_t e, const double4 &xj)
+{
+  double d[3];
+  displacement(xi, xj, e >> kNbrJBits, cell, d);
+  double r2 = dist2(d);
+  return (r2 < g.rc2 && r2 > 1e-10) ? g.eltij[ei][(int) xj.w] : -1;  // kernel.cu:1163
+}
+
 __device__ inline int compact(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
                               const uint32_t *list, int nraw, const double4 *pos, uint32_t *kept,
                               int *start)
 {
   const int lane = threadIdx.x & 63;
+  constexpr int NCH = 4;
+  if (nraw <= 64 * NCH) {
+    // all loads first, then the slot of every entry stays in registers
+    uint32_t ent[NCH];
+    double4 xj[NCH];
+    int slot[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++) {
+      int idx = ch * 64 + lane;
+      ent[ch] = idx < nraw ? list[idx] : 0u;
+    }
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++)
+      if (ch * 64 + lane < nraw) xj[ch] = pos[ent[ch] & kNbrJMask];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++)
+      slot[ch] = (ch * 64 + lane < nraw) ? kept_slot(g, cell, xi, ei, ent[ch], xj[ch]) : -1;
+    int s = 0;
+    for (int b = 0; b < g.nelt; b++) {
+      start[b] = s;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ch++) s += __popcll(__ballot(slot[ch] == b));
+    }
+    start[g.nelt] = s;
+    for (int b = 0; b < g.nelt; b++) {
+      int run = start[b];
+#pragma unroll
+      for (int ch = 0; ch < NCH; ch++) {
+        unsigned long long m = __ballot(slot[ch] == b);
+        if (slot[ch] == b) kept[run + (int) lane_rank(m)] = ent[ch];
+        run += __popcll(m);
+      }
+    }
+    return s;
+  }
+  // long lists: two sweeps
   int cnt[kMaxElts];
   for (int b = 0; b < kMaxElts; b++) cnt[b] = 0;
   for (int base = 0; base < nraw; base += 64) {
     int idx = base + lane, slot = -1;
     if (idx < nraw) {
       uint32_t e = list[idx];
-      double4 xj = pos[e & kNbrJMask];
-      double d[3];
-      displacement(xi, xj, e >> kNbrJBits, cell, d);
-      double r2 = dist2(d);
-      if (r2 < g.rc2 && r2 > 1e-10) slot = g.eltij[ei][(int) xj.w];
+      slot = kept_slot(g, cell, xi, ei, e, pos[e & kNbrJMask]);
     }
     for (int b = 0; b < g.nelt; b++) cnt[b] += __popcll(__ballot(slot == b));
   }
@@ -186,11 +227,7 @@ __device__ inline int compact(const ModelGeo &g, const Cell &cell, const double4
     uint32_t e = 0;
     if (idx < nraw) {
       e = list[idx];
-      double4 xj = pos[e & kNbrJMask];
-      double d[3];
-      displacement(xi, xj, e >> kNbrJBits, cell, d);
-      double r2 = dist2(d);
-      if (r2 < g.rc2 && r2 > 1e-10) slot = g.eltij[ei][(int) xj.w];
+      slot = kept_slot(g, cell, xi, ei, e, pos[e & kNbrJMask]);
     }
     for (int b = 0; b < g.nelt; b++) {
       unsigned long long m = __ballot(slot == b);
@@ -235,17 +272,32 @@ __device__ inline void stage(const ModelGeo &g, const Cell &cell, const double4 
   }
   __syncthreads();
   const int cnt = min(64, nk - base);
-  for (int e = lane; e < cnt * K; e += 64) {
-    int cc = e / K;
-    int kk = e - cc * K;
-    int n = L.tcell[cc];
-    float h = 0.0f, dh = 0.0f;
-    if (n > 0 && n < g.tab_n - 2) {  // end cells are zero (ration.hpp:29)
-      float2 qp = L.qp[cc];
-      kerley(tab[(size_t) n * K + kk], qp.x, qp.y, g.dxf, h, dh);
+  const int total = cnt * K;
+  constexpr int UNR = 8;  // table loads in flight per lane
+  for (int e0 = 0; e0 < total; e0 += 64 * UNR) {
+    float4 t[UNR];
+    float2 qp[UNR];
+    int cc[UNR], kk[UNR];
+    bool ok[UNR], valid[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      int e = e0 + lane + 64 * u;
+      valid[u] = e < total;
+      cc[u] = valid[u] ? e / K : 0;
+      kk[u] = e - cc[u] * K;
+      int n = L.tcell[cc[u]];
+      ok[u] = valid[u] && n > 0 && n < g.tab_n - 2;  // end cells are zero (ration.hpp:29)
+      t[u] = ok[u] ? tab[(size_t) n * K + kk[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+      qp[u] = L.qp[cc[u]];
     }
-    L.H[cc * HS + kk] = h;
-    if (FORCE) L.dH[cc * HS + kk] = dh;
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      if (!valid[u]) continue;
+      float h = 0.0f, dh = 0.0f;
+      if (ok[u]) kerley(t[u], qp[u].x, qp[u].y, g.dxf, h, dh);
+      L.H[cc[u] * HS + kk[u]] = h;
+      if (FORCE) L.dH[cc[u] * HS + kk[u]] = dh;
+    }
   }
   __syncthreads();
 }
@@ -416,6 +468,7 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
       const float *S = L.PS + (size_t) b * K * kND;
       const float *Hr = L.H + c * HS;
       const float *dHr = L.dH + c * HS;
+#pragma unroll 4
       for (int kk = 0; kk < K; kk++) {
         float h = Hr[kk], dh = dHr[kk];
         const float4 *Sk = reinterpret_cast<const float4 *>(S + kk * kND);

@@ -65,6 +65,7 @@ struct MlpParams {
   // LDS layout (elements of the mode's type)
   int offH[kMaxLayers], offD[kMaxLayers], ld[kMaxLayers + 1];
   int offGH, ldGH;
+  int storeD;  // D_l kept in LDS (otherwise recomputed from H_l: tanh)
 };
 
 // ---- neighbor build ---------------------------------------------------------

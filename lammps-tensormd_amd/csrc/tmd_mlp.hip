@@ -169,12 +169,18 @@ __device__ inline void wave_gemm(const typename Traits<MODE>::T *lds, int ld, co
   for (int r = 0; r < RT; r++)
 #pragma unroll
     for (int j = 0; j < CT; j++) acc[r][j] = f32x4{0, 0, 0, 0};
-  for (int k = 0; k < K; k += KS) {
-    float bf[CT];
-    bf16x8 bb[CT];
+  // B operands of step k+KS are loaded while step k's MFMAs run
+  float bfc[CT], bfn[CT];
+  bf16x8 bbc[CT], bbn[CT];
 #pragma unroll
-    for (int j = 0; j < CT; j++)
-      if (tiles[j] >= 0) load_b<MODE>(B, K, N, tiles[j] * 16 + (lane & 15), k, bf[j], bb[j]);
+  for (int j = 0; j < CT; j++)
+    if (tiles[j] >= 0) load_b<MODE>(B, K, N, tiles[j] * 16 + (lane & 15), 0, bfc[j], bbc[j]);
+  for (int k = 0; k < K; k += KS) {
+    if (k + KS < K) {
+#pragma unroll
+      for (int j = 0; j < CT; j++)
+        if (tiles[j] >= 0) load_b<MODE>(B, K, N, tiles[j] * 16 + (lane & 15), k + KS, bfn[j], bbn[j]);
+    }
 #pragma unroll
     for (int r = 0; r < RT; r++) {
       float af;
@@ -185,10 +191,15 @@ __device__ inline void wave_gemm(const typename Traits<MODE>::T *lds, int ld, co
       for (int j = 0; j < CT; j++) {
         if (tiles[j] < 0) continue;
         if (MODE == 0)
-          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af, bf[j], acc[r][j], 0, 0, 0);
+          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af, bfc[j], acc[r][j], 0, 0, 0);
         else
-          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab, bb[j], acc[r][j], 0, 0, 0);
+          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab, bbc[j], acc[r][j], 0, 0, 0);
       }
+    }
+#pragma unroll
+    for (int j = 0; j < CT; j++) {
+      bfc[j] = bfn[j];
+      bbc[j] = bbn[j];
     }
   }
 }
@@ -246,7 +257,7 @@ __global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const f
             float h = act_f<ACT>(acc[r][j][i] + bv, dz);
             if (mp.res[l]) h += to_f<T>(Hin[row * ldi + col]);
             H[row * ldo + col] = from_f<T>(h);
-            D[row * ldo + col] = from_f<T>(dz);
+            if (mp.storeD) D[row * ldo + col] = from_f<T>(dz);
           }
       }
     }
@@ -267,6 +278,8 @@ __global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const f
       if (lane == 0) E[row0 + r] = (float) (s + (double) bo);
     }
   }
+  // the seed below may overwrite H_{L-2} (when D aliases H): all output dots first
+  __syncthreads();
   // seed: G_{L-2} = w_out * D_{L-2} (in the D slot); gradH = w_out for a resnet layer
   {
     const int N = mp.sizes[L - 1];
@@ -276,7 +289,8 @@ __global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const f
     const int ld = mp.ld[L - 1];
     for (int q = threadIdx.x; q < R * N; q += 256) {
       int r = q / N, c = q - r * N;
-      D[r * ld + c] = from_f<T>(w[c] * to_f<T>(D[r * ld + c]));
+      float v = to_f<T>(D[r * ld + c]);  // D, or H when D is not stored (tanh: D = 1 - h^2)
+      D[r * ld + c] = from_f<T>(w[c] * (mp.storeD ? v : 1.0f - v * v));
       if (mp.res[L - 2]) gH[r * mp.ldGH + c] = from_f<T>(w[c]);
     }
   }
@@ -315,7 +329,8 @@ __global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const f
             } else {
               T *D = sm + mp.offD[l - 1];
               const int ld = mp.ld[l];
-              D[row * ld + col] = from_f<T>(v * to_f<T>(D[row * ld + col]));
+              float dv = to_f<T>(D[row * ld + col]);
+              D[row * ld + col] = from_f<T>(v * (mp.storeD ? dv : 1.0f - dv * dv));
               if (mp.res[l - 1]) gH[row * mp.ldGH + col] = from_f<T>(v);
             }
           }
@@ -361,14 +376,20 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
     int w = mp.sizes[l];
     m.ld[l] = (mode == 0) ? w + 4 : w + 8;
   }
+  for (int l = 0; l < mp.nl - 1; l++)
+    if (mp.res[l]) anyres = true;
+  // tanh without skips: D = 1 - h^2 is recomputed from H, so G_l overwrites H_l
+  m.storeD = !(mp.act == 2 && !anyres);
   for (int l = 0; l < mp.nl - 1; l++) {
     int N = mp.sizes[l + 1];
     maxw = std::max(maxw, N);
-    if (mp.res[l]) anyres = true;
     m.offH[l] = (int) off;
     off += (size_t) R * m.ld[l + 1];
-    m.offD[l] = (int) off;
-    off += (size_t) R * m.ld[l + 1];
+    m.offD[l] = m.offH[l];
+    if (m.storeD) {
+      m.offD[l] = (int) off;
+      off += (size_t) R * m.ld[l + 1];
+    }
   }
   m.ldGH = (mode == 0) ? maxw + 4 : maxw + 8;
   m.offGH = (int) off;
@@ -383,12 +404,13 @@ hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const 
   if (rl.nrows <= 0) return hipSuccess;
   const size_t limit = 160 * 1024;
   static const int Rs[3] = {64, 32, 16};
-  for (int q = (mode == 0 ? 1 : 0); q < 3; q++) {
+  for (int q = 0; q < 3; q++) {
     int R = Rs[q];
     MlpParams m;
     size_t lds = mlp_lds_bytes(mp, mode, R, &m);
     if (lds > limit) continue;
     if (mode == 0) {
+      if (R == 64) return launch_r<0, 64>(rl, m, G, E, dEdG, lds, s);
       if (R == 32) return launch_r<0, 32>(rl, m, G, E, dEdG, lds, s);
       return launch_r<0, 16>(rl, m, G, E, dEdG, lds, s);
     }

@@ -110,9 +110,13 @@ def test_golden_models(goldens, case):
     ref = om.compute_periodic(x, types - 1, cell)
     e = _errs(got, ref, len(x))
     say("errors", e, "E/atom", got["energy"] / len(x), "golden", c["pe"] / c["natoms"])
+    # |E| ~ 10 eV/atom for these trained models: one FP32 ulp of the atomic
+    # energy is ~1e-6 eV and the resnet output sums terms that cancel, so the
+    # energy bar is relative (4e-7 |E|, a few FP32 ulps)
+    e_tol = max(E_TOL, 4e-7 * abs(ref["energy"]) / len(x))
     assert e["dF"] <= F_TOL
-    assert e["dE"] <= 1e-6  # |E| ~ 10 eV/atom here; 1e-7 relative
-    assert abs(got["energy"] / len(x) - c["pe"] / c["natoms"]) <= 1e-6
+    assert e["dE"] <= e_tol
+    assert abs(got["energy"] / len(x) - c["pe"] / c["natoms"]) <= e_tol + 5e-7
     if case == "MoNi_m3":
         fy = np.array([r[2:] for r in goldens["forces_MoNi"]["rows"]])
         assert abs(got["f"][:16] - fy).max() <= F_TOL

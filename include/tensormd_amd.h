@@ -123,6 +123,9 @@ int tmd_md_run(tmd_system *sys, int nsteps, double dt, double *epot, double *eki
  * maintenance, descriptor kernel, MLP GEMMs, force kernel, FP64
  * reductions, integration, and the whole step. */
 int tmd_md_profile(tmd_system *sys, int nsteps, double dt, double *ms);
+/* Replay each MD step as one captured HIP graph (default 1) or launch its
+ * kernels one by one (0); results are the same either way. */
+int tmd_system_set_graph(tmd_system *sys, int enable);
 /* Number of neighbor-list builds done so far. */
 int tmd_system_nbuilds(const tmd_system *sys);
 /* Export the current full neighbor list (cutoff rc + skin): off[natoms+1],
@@ -132,6 +135,12 @@ int tmd_system_neighbors(tmd_system *sys, long cap, long *off, int *j, int *shif
 /* Descriptor rows G[natoms][width] and dE/dG of the last compute (FP32,
  * atom order), for kernel-level parity tests. */
 int tmd_system_descriptors(tmd_system *sys, float *G, float *dEdG);
+
+/* ---- measurement ----------------------------------------------------------
+ * Device-to-device copy of `bytes` (read + write) repeated `iters` times on
+ * the current device; *gbps = achieved read+write GB/s.  Re-measures the
+ * HBM roofline and calibrates memory counters.  No reference counterpart. */
+int tmd_bench_copy(size_t bytes, int iters, double *gbps);
 
 #ifdef __cplusplus
 }

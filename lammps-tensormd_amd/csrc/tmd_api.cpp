@@ -471,4 +471,35 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   return TMD_OK;
 }
 
+// STREAM-like copy: bytes read = bytes written = `bytes`.
+int tmd_bench_copy(size_t bytes, int iters, double *gbps)
+{
+  if (bytes < 16 || iters < 1 || !gbps) return TMD_ERR_ARG;
+  bytes &= ~(size_t) 15;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_zero(a, bytes, s);
+  if (e == hipSuccess) e = launch_copy(a, b, bytes, s);  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int q = 0; q < iters && e == hipSuccess; q++) e = launch_copy(a, b, bytes, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) *gbps = 2.0 * (double) bytes * iters / (ms * 1e-3) / 1e9;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (s) hipStreamDestroy(s);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "tmd_bench_copy");
+  return TMD_OK;
+}
+
 }  // extern "C"

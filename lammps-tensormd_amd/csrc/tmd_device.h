@@ -110,6 +110,8 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out);
 
 // ---- reductions / integration ---------------------------------------------------
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t stream);
+// plain device-to-device copy (bandwidth calibration)
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
 // out[0] = scale * sum_r E[r] over real rows; out[1..6] = sum_r wrow[r][*]
 hipError_t launch_reduce_ev(const float *E, const double *wrow, const int *row2atom, int nrows,
                             double scale, double *out, hipStream_t stream);
@@ -120,6 +122,9 @@ hipError_t launch_vv_second(int n, const double4 *pos, double *v, const double *
 // kinetic energy sum(m v^2)/2 * mvv2e into out[0] (out zeroed inside)
 hipError_t launch_ekin(int n, const double4 *pos, const double *v, const double *mass_by_elem,
                        double *out, hipStream_t stream);
+// per-step energy history through a device counter (safe to replay in a graph)
+hipError_t launch_record_step(const double *epot, const double *ekin, double *hist,
+                              unsigned long long *counter, int cap, hipStream_t stream);
 // copy e (per-step scalars) into history slots
 hipError_t launch_copy_scalar(const double *src, double *dst, hipStream_t stream);
 

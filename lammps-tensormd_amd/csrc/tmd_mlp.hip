@@ -29,12 +29,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
-__device__ inline uint16_t f2bf(float f)
-{
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t) (u >> 16);
-}
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16) f); }
+typedef float f32x8 __attribute__((ext_vector_type(8)));
 __device__ inline float bf2f(uint16_t v) { return __uint_as_float(((uint32_t) v) << 16); }
 
 // activation value and derivative (nnp.cu:9-49, activation.hpp:158-261)
@@ -115,8 +111,9 @@ __device__ inline void load_a(const typename Traits<MODE>::T *lds, int ld, const
       if (kk + 8 <= K) {
         const float4 *p = reinterpret_cast<const float4 *>(G + (size_t) row * gw + kk);
         float4 x0 = p[0], x1 = p[1];
-        v[0] = f2bf(x0.x); v[1] = f2bf(x0.y); v[2] = f2bf(x0.z); v[3] = f2bf(x0.w);
-        v[4] = f2bf(x1.x); v[5] = f2bf(x1.y); v[6] = f2bf(x1.z); v[7] = f2bf(x1.w);
+        f32x8 xv = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+        ab = __builtin_convertvector(xv, bf16x8);
+        return;
       } else {
         for (int q = 0; q < 8; q++) v[q] = (kk + q < K) ? f2bf(G[(size_t) row * gw + kk + q]) : 0;
       }
@@ -169,37 +166,51 @@ __device__ inline void wave_gemm(const typename Traits<MODE>::T *lds, int ld, co
   for (int r = 0; r < RT; r++)
 #pragma unroll
     for (int j = 0; j < CT; j++) acc[r][j] = f32x4{0, 0, 0, 0};
-  // B operands of step k+KS are loaded while step k's MFMAs run
-  float bfc[CT], bfn[CT];
-  bf16x8 bbc[CT], bbn[CT];
+  // A and B operands of step k+KS are loaded while step k's MFMAs run
+  float bfc[CT], bfn[CT], afc[RT], afn[RT];
+  bf16x8 bbc[CT], bbn[CT], abc[RT], abn[RT];
 #pragma unroll
   for (int j = 0; j < CT; j++)
     if (tiles[j] >= 0) load_b<MODE>(B, K, N, tiles[j] * 16 + (lane & 15), 0, bfc[j], bbc[j]);
+#pragma unroll
+  for (int r = 0; r < RT; r++) {
+    int row = r * 16 + (lane & 15);
+    load_a<MODE, AGLOBAL>(lds, ld, G, gw, AGLOBAL ? grow0 + row : row, 0, K, afc[r], abc[r]);
+  }
   for (int k = 0; k < K; k += KS) {
-    if (k + KS < K) {
+    const bool more = k + KS < K;
+    if (more) {
 #pragma unroll
       for (int j = 0; j < CT; j++)
         if (tiles[j] >= 0) load_b<MODE>(B, K, N, tiles[j] * 16 + (lane & 15), k + KS, bfn[j], bbn[j]);
+#pragma unroll
+      for (int r = 0; r < RT; r++) {
+        int row = r * 16 + (lane & 15);
+        load_a<MODE, AGLOBAL>(lds, ld, G, gw, AGLOBAL ? grow0 + row : row, k + KS, K, afn[r], abn[r]);
+      }
     }
 #pragma unroll
     for (int r = 0; r < RT; r++) {
-      float af;
-      bf16x8 ab;
-      int row = r * 16 + (lane & 15);
-      load_a<MODE, AGLOBAL>(lds, ld, G, gw, AGLOBAL ? grow0 + row : row, k, K, af, ab);
 #pragma unroll
       for (int j = 0; j < CT; j++) {
         if (tiles[j] < 0) continue;
         if (MODE == 0)
-          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af, bfc[j], acc[r][j], 0, 0, 0);
+          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(afc[r], bfc[j], acc[r][j], 0, 0, 0);
         else
-          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab, bbc[j], acc[r][j], 0, 0, 0);
+          acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abc[r], bbc[j], acc[r][j], 0, 0, 0);
       }
     }
+    if (more) {
 #pragma unroll
-    for (int j = 0; j < CT; j++) {
-      bfc[j] = bfn[j];
-      bbc[j] = bbn[j];
+      for (int j = 0; j < CT; j++) {
+        bfc[j] = bfn[j];
+        bbc[j] = bbn[j];
+      }
+#pragma unroll
+      for (int r = 0; r < RT; r++) {
+        afc[r] = afn[r];
+        abc[r] = abn[r];
+      }
     }
   }
 }

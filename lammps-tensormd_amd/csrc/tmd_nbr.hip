@@ -53,8 +53,14 @@ __global__ void k_nb_check(int n, const double4 *pos, const double *xb, unsigned
     d2 = dx * dx + dy * dy + dz * dz;
   }
   for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o));
-  if ((threadIdx.x & 63) == 0 && d2 > 0.0)
-    atomicMax(&ctl[0], (unsigned long long) __double_as_longlong(d2));
+  __shared__ double wmax[16];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int) (blockDim.x >> 6); w++) m = fmax(m, wmax[w]);
+    if (m > 0.0) atomicMax(&ctl[0], (unsigned long long) __double_as_longlong(m));
+  }
 }
 
 // wrap into the cell, remember build positions, bin the atoms
@@ -211,7 +217,7 @@ hipError_t nbr_step(const NbrBuild &nb, const Cell &cell, double4 *pos, uint32_t
 {
   int nbins = nb.nb[0] * nb.nb[1] * nb.nb[2];
   int blocks = (nb.n + 255) / 256;
-  k_nb_check<<<blocks, 256, 0, stream>>>(nb.n, pos, nb.x_build, nb.ctl);
+  k_nb_check<<<(nb.n + 1023) / 1024, 1024, 0, stream>>>(nb.n, pos, nb.x_build, nb.ctl);
   k_nb_bin<<<blocks, 256, 0, stream>>>(nb, cell, pos);
   k_nb_scan<<<1, 1024, 0, stream>>>(nb, nbins);
   k_nb_fill<<<blocks, 256, 0, stream>>>(nb);

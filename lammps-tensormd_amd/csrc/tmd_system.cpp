@@ -29,6 +29,10 @@ struct tmd_system {
   bool have_forces = false;
   double dt_cached = -1;
   int nbuilds_seen = 0;
+  // one captured MD step (velocity Verlet + neighbor check + forces + history)
+  hipGraphExec_t step_graph = nullptr;
+  bool graph_hist = false;
+  bool use_graph = true;
 };
 
 namespace {
@@ -107,6 +111,10 @@ int check_overflow(tmd_system *s, bool *overflow)
   if (*overflow) {
     int need = (int) ctl[2];
     int cap = ((need + need / 4 + 16) + 7) & ~7;
+    if (s->step_graph) {
+      hipGraphExecDestroy(s->step_graph);
+      s->step_graph = nullptr;
+    }
     int rc = alloc_list(s, cap);
     if (rc != TMD_OK) return rc;
     unsigned long long reset[3] = {0ull, 1ull, 0ull};
@@ -203,7 +211,7 @@ int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type
   if ((rc = chk(s->atominfo.ensure(sizeof(int) * 5 * (size_t) natoms), "atom bins alloc"))) return rc;
   if ((rc = chk(s->xbuild.ensure(sizeof(double) * 4 * (size_t) natoms), "xbuild alloc"))) return rc;
   if ((rc = chk(s->ctl.ensure(sizeof(unsigned long long) * 8), "ctl alloc"))) return rc;
-  if ((rc = chk(s->hist.ensure(sizeof(double) * 2 * 65536), "hist alloc"))) return rc;
+  if ((rc = chk(s->hist.ensure(sizeof(double) * (2 * 4096 + 4096 + 64)), "hist alloc"))) return rc;
   if ((rc = alloc_list(s.get(), cap))) return rc;
   int *bins = (int *) s->bins.p;
   s->nb.bin_count = bins;
@@ -237,6 +245,7 @@ void tmd_system_destroy(tmd_system *s)
   if (!s) return;
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
+  if (s->step_graph) hipGraphExecDestroy(s->step_graph);
   delete s;
 }
 
@@ -342,39 +351,80 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
     if ((e = hipMemcpy(s->dtfm.p, dtfm.data(), sizeof(double) * kMaxElts, hipMemcpyHostToDevice)) != hipSuccess)
       return hip_fail(s->ctx, e, "dtfm upload");
     s->dt_cached = dt;
+    if (s->step_graph) {  // dt is baked into the captured launch
+      hipGraphExecDestroy(s->step_graph);
+      s->step_graph = nullptr;
+    }
   }
   if (!s->have_forces) {
     int rc = tmd_system_compute(s);
     if (rc != TMD_OK) return rc;
   }
   double *hist = (double *) s->hist.p;
-  const int chunk = 4096;
+  const bool want_hist = epot || ekin;
+  // layout of the history buffer: [0, 2*kHist) pairs, then scratch
+  const int kHist = 4096;
+  double *ekin_out = hist + 2 * kHist;
+  unsigned long long *counter = (unsigned long long *) (hist + 2 * kHist + 4096);
+  auto enqueue_step = [&]() -> int {
+    hipError_t er;
+    if ((er = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
+                              (const double *) s->dtfm.p, dt, st)) != hipSuccess)
+      return hip_fail(s->ctx, er, "vv first");
+    int rc = sys_forces(s);
+    if (rc != TMD_OK) return rc;
+    if ((er = launch_vv_second(s->n, (const double4 *) s->w.pos.p, (double *) s->vel.p,
+                               (const double *) s->w.f.p, (const double *) s->dtfm.p, st)) != hipSuccess)
+      return hip_fail(s->ctx, er, "vv second");
+    if (want_hist) {
+      if ((er = launch_ekin(s->n, (const double4 *) s->w.pos.p, (const double *) s->vel.p,
+                            (const double *) s->mass.p, ekin_out, st)) != hipSuccess)
+        return hip_fail(s->ctx, er, "ekin");
+      if ((er = launch_record_step((const double *) s->w.red.p, ekin_out, hist, counter, kHist, st)) != hipSuccess)
+        return hip_fail(s->ctx, er, "history");
+    }
+    return TMD_OK;
+  };
+  if (s->use_graph && (!s->step_graph || s->graph_hist != want_hist)) {
+    if (s->step_graph) {
+      hipGraphExecDestroy(s->step_graph);
+      s->step_graph = nullptr;
+    }
+    hipGraph_t graph;
+    if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hip_fail(s->ctx, e, "graph capture");
+    int rc = enqueue_step();
+    e = hipStreamEndCapture(st, &graph);
+    if (rc != TMD_OK) return rc;
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "graph capture");
+    e = hipGraphInstantiate(&s->step_graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      s->step_graph = nullptr;
+      return hip_fail(s->ctx, e, "graph instantiate");
+    }
+    s->graph_hist = want_hist;
+  }
   for (int done = 0; done < nsteps;) {
-    int nrun = std::min(chunk, nsteps - done);
+    int nrun = std::min(kHist, nsteps - done);
+    if (want_hist) {
+      unsigned long long zero = 0ull;
+      if ((e = hipMemcpyAsync(counter, &zero, sizeof(zero), hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(s->ctx, e, "history reset");
+    }
     for (int q = 0; q < nrun; q++) {
-      if ((e = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
-                               (const double *) s->dtfm.p, dt, st)) != hipSuccess)
-        return hip_fail(s->ctx, e, "vv first");
-      int rc = sys_forces(s);
-      if (rc != TMD_OK) return rc;
-      if ((e = launch_vv_second(s->n, (const double4 *) s->w.pos.p, (double *) s->vel.p,
-                                (const double *) s->w.f.p, (const double *) s->dtfm.p, st)) != hipSuccess)
-        return hip_fail(s->ctx, e, "vv second");
-      if (epot && (e = launch_copy_scalar((const double *) s->w.red.p, hist + 2 * q, st)) != hipSuccess)
-        return hip_fail(s->ctx, e, "history");
-      if (ekin) {
-        if ((e = launch_ekin(s->n, (const double4 *) s->w.pos.p, (const double *) s->vel.p,
-                             (const double *) s->mass.p, hist + 2 * 4096 + 8 * 0 + 2 * 4096, st)) != hipSuccess)
-          return hip_fail(s->ctx, e, "ekin");
-        if ((e = launch_copy_scalar(hist + 4 * 4096, hist + 2 * q + 1, st)) != hipSuccess)
-          return hip_fail(s->ctx, e, "history");
+      if (s->use_graph) {
+        if ((e = hipGraphLaunch(s->step_graph, st)) != hipSuccess) return hip_fail(s->ctx, e, "graph launch");
+      } else {
+        int rc = enqueue_step();
+        if (rc != TMD_OK) return rc;
       }
     }
     bool over = false;
     int rc = check_overflow(s, &over);
     if (rc != TMD_OK) return rc;
     if (over) return sys_fail(s, TMD_ERR_CAPACITY, "neighbor list overflow during the run (list grown; rerun)");
-    if (epot || ekin) {
+    if (want_hist) {
       std::vector<double> h(2 * (size_t) nrun);
       if ((e = hipMemcpy(h.data(), hist, sizeof(double) * 2 * nrun, hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(s->ctx, e, "history copy");
@@ -385,6 +435,13 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
     }
     done += nrun;
   }
+  return TMD_OK;
+}
+
+int tmd_system_set_graph(tmd_system *s, int enable)
+{
+  if (!s) return TMD_ERR_ARG;
+  s->use_graph = enable != 0;
   return TMD_OK;
 }
 

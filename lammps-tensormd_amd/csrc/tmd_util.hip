@@ -42,14 +42,15 @@ __global__ void k_reduce_ev_part(const float *E, const double *wrow, const int *
   if (t < 7) part[(size_t) blockIdx.x * 7 + t] = sh[t][0];
 }
 
+// one wave per quantity; fixed shuffle tree (deterministic)
 __global__ void k_reduce_ev_final(const double *part, int nparts, double scale, double *out)
 {
-  int t = threadIdx.x;
-  if (t < 7) {
-    double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[(size_t) p * 7 + t];
-    out[t] = (t == 0) ? s * scale : s;
-  }
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (q >= 7) return;
+  double s = 0.0;
+  for (int p = lane; p < nparts; p += 64) s += part[(size_t) p * 7 + q];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) out[q] = (q == 0) ? s * scale : s;
 }
 
 __global__ void k_vv_first(int n, double4 *pos, double *v, const double *f, const double *dtfm,
@@ -102,11 +103,11 @@ __global__ void k_ekin_part(int n, const double4 *pos, const double *v, const do
 
 __global__ void k_ekin_final(const double *part, int nparts, double *out)
 {
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[p];
-    out[0] = 0.5 * 1.0364269e-4 * s;  // mvv2e, metal units
-  }
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int p = lane; p < nparts; p += 64) s += part[p];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (threadIdx.x == 0) out[0] = 0.5 * 1.0364269e-4 * s;  // mvv2e, metal units
 }
 
 __global__ void k_copy_scalar(const double *src, double *dst)
@@ -114,7 +115,35 @@ __global__ void k_copy_scalar(const double *src, double *dst)
   if (threadIdx.x == 0) dst[0] = src[0];
 }
 
+// hist[2*n] = epot, hist[2*n+1] = ekin, n = counter++ (graph-replay safe)
+__global__ void k_record_step(const double *epot, const double *ekin, double *hist,
+                              unsigned long long *counter, int cap)
+{
+  if (threadIdx.x == 0) {
+    unsigned long long n = counter[0];
+    if (n < (unsigned long long) cap) {
+      hist[2 * n] = epot[0];
+      hist[2 * n + 1] = ekin ? ekin[0] : 0.0;
+    }
+    counter[0] = n + 1;
+  }
+}
+
+__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
+{
+  size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t) gridDim.x * blockDim.x;
+  for (size_t q = i; q < n16; q += stride) dst[q] = src[q];
+}
+
 }  // namespace
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
+{
+  size_t n16 = bytes / 16;
+  k_copy<<<8192, 256, 0, s>>>((const uint4 *) src, (uint4 *) dst, n16);
+  return hipGetLastError();
+}
 
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s)
 {
@@ -130,7 +159,7 @@ hipError_t launch_reduce_ev(const float *E, const double *wrow, const int *row2a
 {
   double *part = out + 8;  // out holds 8 + kRedParts*7 doubles
   k_reduce_ev_part<<<kRedParts, 256, 0, s>>>(E, wrow, row2atom, nrows, part);
-  k_reduce_ev_final<<<1, 64, 0, s>>>(part, kRedParts, scale, out);
+  k_reduce_ev_final<<<1, 448, 0, s>>>(part, kRedParts, scale, out);
   return hipGetLastError();
 }
 
@@ -154,6 +183,13 @@ hipError_t launch_ekin(int n, const double4 *pos, const double *v, const double 
   double *part = out + 8;
   k_ekin_part<<<kRedParts, 256, 0, s>>>(n, pos, v, mass, part);
   k_ekin_final<<<1, 64, 0, s>>>(part, kRedParts, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_record_step(const double *epot, const double *ekin, double *hist,
+                              unsigned long long *counter, int cap, hipStream_t s)
+{
+  k_record_step<<<1, 64, 0, s>>>(epot, ekin, hist, counter, cap);
   return hipGetLastError();
 }
 

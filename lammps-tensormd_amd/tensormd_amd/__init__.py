@@ -76,6 +76,8 @@ def lib():
     L.tmd_md_run.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp]
     L.tmd_system_nbuilds.argtypes = [_vp]
     L.tmd_md_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp]
+    L.tmd_system_set_graph.argtypes = [_vp, ctypes.c_int]
+    L.tmd_bench_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
     L.tmd_system_neighbors.argtypes = [_vp, ctypes.c_long, _lp, _ip, _ip, _lp]
     L.tmd_system_descriptors.argtypes = [_vp, _fp, _fp]
     _lib = L
@@ -97,6 +99,13 @@ def _i32(a):
 def _check(rc, ctx=None):
     if rc != TMD_OK:
         raise TmdError(rc, lib().tmd_last_error(ctx).decode(errors="replace"))
+
+
+def bench_copy(nbytes, iters=20):
+    """Achieved HBM read+write GB/s of a plain device copy of `nbytes`."""
+    g = ctypes.c_double()
+    _check(lib().tmd_bench_copy(int(nbytes), int(iters), ctypes.byref(g)), None)
+    return g.value
 
 
 class TensorMD:
@@ -242,6 +251,9 @@ class System:
         _check(lib().tmd_md_run(self.h, int(nsteps), float(dt), _p(ep, _dp), _p(ek, _dp)),
                self.model.h)
         return ep, ek
+
+    def set_graph(self, enable):
+        _check(lib().tmd_system_set_graph(self.h, 1 if enable else 0), self.model.h)
 
     def profile(self, nsteps, dt):
         """Mean ms per step of each phase, timed with HIP events on the library stream."""

@@ -156,3 +156,29 @@ def test_nve_c1_trajectory(synth_models):
     # energy conservation of the GPU trajectory
     etot = ep + ek
     assert abs(etot - etot[0]).max() / n < 1e-5
+
+
+def test_graph_replay_matches_eager(synth_models):
+    """The captured-step graph gives the same trajectory as eager launches
+    (FP64 force atomics may reorder: tolerance, not bit equality)."""
+    say("case: graph vs eager, C2 model, bcc 6x6x6, 30 steps")
+    x, t, cell = W.crystal("bcc", 3.2, (6, 6, 6), nelt=4, sigma=0.08, seed=31)
+    v = W.maxwell_boltzmann(np.array([95.95, 92.906, 180.95, 183.84])[t - 1], 600.0, seed=32)
+    m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)
+    out = []
+    for graph in (True, False):
+        s = m.system(x, t, cell, skin=1.0)
+        s.set_graph(graph)
+        s.set_velocities(v)
+        s.compute()
+        ep, ek = [], []
+        for rep in range(3):  # several replays with host syncs in between
+            a, b = s.md_run(10, 0.001)
+            ep.append(a)
+            ek.append(b)
+        out.append((np.concatenate(ep), np.concatenate(ek), s.get()["x"]))
+        s.close()
+    n = len(x)
+    assert np.abs(out[0][0] - out[1][0]).max() / n < 1e-9
+    assert np.abs(out[0][1] - out[1][1]).max() / n < 1e-9
+    assert np.abs(out[0][2] - out[1][2]).max() < 1e-9

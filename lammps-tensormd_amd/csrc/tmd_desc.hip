@@ -100,16 +100,18 @@ __device__ inline unsigned lane_rank(unsigned long long m)
   return __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
 }
 
+constexpr int kChunk = 32;  // kept neighbors staged at a time
+
 struct LDSMap {
   uint32_t *kept;  // [cap] compacted, slot-sorted neighbor entries
-  float4 *rec;     // [64] ux, uy, uz, r of the current chunk
-  int *tcell;      // [64]
-  float2 *qp;      // [64]
-  float4 *drec;    // [64] fp32 displacement, j as bits (force)
-  float *M;        // [64][kMS]   (aliased by P/S after the accumulation)
-  float *H;        // [64][HS]
-  float *dH;       // [64][HS]    (force)
-  float *PS;       // [nslot][K][20] = M region
+  float4 *rec;     // [kChunk] ux, uy, uz, r of the current chunk
+  int *tcell;      // [kChunk]
+  float2 *qp;      // [kChunk]
+  float4 *drec;    // [kChunk] fp32 displacement, j as bits (force)
+  float *M;        // [kChunk][kMS]  (force: aliased by S after the accumulation)
+  float *H;        // [kChunk][HS]
+  float *dH;       // [kChunk][HS]   (force)
+  float *PS;       // [nslot][K][20] = M region (force)
 };
 
 template <int K>
@@ -122,16 +124,16 @@ __host__ __device__ inline size_t lds_layout(int nelt, int cap, bool force, size
     o += (bytes + 15) & ~(size_t) 15;
     return r;
   };
-  size_t mreg = (size_t) 64 * kMS * 4;
-  size_t preg = (size_t) nelt * K * kND * 4;
+  size_t mreg = (size_t) kChunk * kMS * 4;
+  size_t preg = force ? (size_t) nelt * K * kND * 4 : 0;
   offs[0] = take((size_t) cap * 4);
-  offs[1] = take(64 * 16);
-  offs[2] = take(64 * 4);
-  offs[3] = take(64 * 8);
-  offs[4] = take(force ? 64 * 16 : 16);
+  offs[1] = take(kChunk * 16);
+  offs[2] = take(kChunk * 4);
+  offs[3] = take(kChunk * 8);
+  offs[4] = take(force ? kChunk * 16 : 16);
   offs[5] = take(mreg > preg ? mreg : preg);
-  offs[6] = take((size_t) 64 * HS * 4);
-  offs[7] = take(force ? (size_t) 64 * HS * 4 : 16);
+  offs[6] = take((size_t) kChunk * HS * 4);
+  offs[7] = take(force ? (size_t) kChunk * HS * 4 : 16);
   return o;
 }
 
@@ -238,7 +240,7 @@ __device__ inline int compact(const ModelGeo &g, const Cell &cell, const double4
   return s;
 }
 
-// Phase 2 staging of kept neighbors [base, base+64): geometry, monomials, H (and dH)
+// Phase 2 staging of kept neighbors [base, base+kChunk): geometry, monomials, H (and dH)
 template <int K, bool FORCE>
 __device__ inline void stage(const ModelGeo &g, const Cell &cell, const double4 &xi,
                              const double4 *pos, const float4 *tab, int base, int nk, LDSMap &L,
@@ -247,7 +249,7 @@ __device__ inline void stage(const ModelGeo &g, const Cell &cell, const double4 
   const int lane = threadIdx.x & 63;
   constexpr int HS = Tiling<K>::HS;
   int c = base + lane;
-  if (c < nk) {
+  if (lane < kChunk && c < nk) {
     uint32_t e = L.kept[c];
     int j = (int) (e & kNbrJMask);
     double d[3];
@@ -271,9 +273,9 @@ __device__ inline void stage(const ModelGeo &g, const Cell &cell, const double4 
     }
   }
   __syncthreads();
-  const int cnt = min(64, nk - base);
+  const int cnt = min(kChunk, nk - base);
   const int total = cnt * K;
-  constexpr int UNR = 8;  // table loads in flight per lane
+  constexpr int UNR = 4;  // table loads in flight per lane
   for (int e0 = 0; e0 < total; e0 += 64 * UNR) {
     float4 t[UNR];
     float2 qp[UNR];
@@ -316,7 +318,7 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
 #pragma unroll
   for (int b = 0; b < NS; b++) {
     if (b >= g.nelt) break;
-    int c0 = max(start[b] - base, 0), c1 = min(start[b + 1] - base, 64);
+    int c0 = max(start[b] - base, 0), c1 = min(start[b + 1] - base, kChunk);
     for (int c = c0; c < c1; c++) {
       float h[T::KT], m[T::DT];
 #pragma unroll
@@ -331,42 +333,26 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
   }
 }
 
-// Phases 1-2 and the write of P to LDS (L.PS, [b][k][d]).  Returns nk.
+// Phases 1-2: P of every slot in the lane's register block.  Returns nk.
 template <int K, int NS, bool FORCE>
 __device__ inline int build_P(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
                               const uint32_t *list, int nraw, const double4 *pos, const float4 *tab,
-                              LDSMap &L, int *start)
+                              LDSMap &L, int *start, float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
 {
   using T = Tiling<K>;
-  const int lane = threadIdx.x & 63;
   int nk = compact(g, cell, xi, ei, list, nraw, pos, L.kept, start);
   __syncthreads();
-  float acc[NS][T::KT][T::DT];
 #pragma unroll
   for (int b = 0; b < NS; b++)
 #pragma unroll
     for (int a = 0; a < T::KT; a++)
 #pragma unroll
       for (int q = 0; q < T::DT; q++) acc[b][a][q] = 0.0f;
-  for (int base = 0; base < nk; base += 64) {
+  for (int base = 0; base < nk; base += kChunk) {
     stage<K, FORCE>(g, cell, xi, pos, tab, base, nk, L, true);
     accumulate<K, NS>(g, L, start, base, acc);
     __syncthreads();
   }
-  // registers -> LDS P[b][k][d] (overwrites the monomial rows)
-  if (lane < T::LANES) {
-    const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
-#pragma unroll
-    for (int b = 0; b < NS; b++) {
-      if (b >= g.nelt) break;
-#pragma unroll
-      for (int a = 0; a < T::KT; a++)
-#pragma unroll
-        for (int q = 0; q < T::DT; q++)
-          L.PS[((size_t) b * K + kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
-    }
-  }
-  __syncthreads();
   return nk;
 }
 
@@ -387,21 +373,42 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Cell cell, const 
     for (int e = lane; e < gw; e += 64) Grow[e] = 0.0f;
     return;
   }
+  using T = Tiling<K>;
   LDSMap L = carve<K>(smem, g.nelt, cap, false);
   int start[kMaxElts + 1];
   const double4 xi = pos[i];
-  build_P<K, NS, false>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start);
-  for (int e = lane; e < g.nelt * K; e += 64) {
-    const float *P = L.PS + (size_t) e * kND;
-    float Gv[kNM] = {0, 0, 0, 0};
+  float acc[NS][T::KT][T::DT];
+  build_P<K, NS, false>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start,
+                        acc);
+  // G[b][k][m] = sum_d T[m][d] P^2 (m >= 1): partial sums over the lane's d
+  // block, then a sum over the NDG lanes that share k; G[b][k][0] = P0
+  // (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
+  const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
+  const int gbase = kq * T::NDG;
 #pragma unroll
-    for (int d = 0; d < kND; d++) {
-      float p2 = P[d] * P[d];
+  for (int b = 0; b < NS; b++) {
+    if (b >= g.nelt) break;
 #pragma unroll
-      for (int m = 1; m < kNM; m++) Gv[m] += g.T[m][d] * p2;
+    for (int a = 0; a < T::KT; a++) {
+      float part[kNM] = {0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < T::DT; q++) {
+        const int d = dq * T::DT + q;
+        float p2 = acc[b][a][q] * acc[b][a][q];
+#pragma unroll
+        for (int m = 1; m < kNM; m++) part[m] += g.T[m][d] * p2;
+      }
+      float tot[kNM] = {0, 0, 0, 0};
+#pragma unroll
+      for (int t = 0; t < T::NDG; t++)
+#pragma unroll
+        for (int m = 1; m < kNM; m++) tot[m] += __shfl(part[m], gbase + t);
+      if (lane < T::LANES && dq == 0) {
+        const int k = kq * T::KT + a;
+        *reinterpret_cast<float4 *>(Grow + ((size_t) b * K + k) * kNM) =
+            make_float4(acc[b][a][0], tot[1], tot[2], tot[3]);
+      }
     }
-    Gv[0] = P[0];
-    *reinterpret_cast<float4 *>(Grow + (size_t) e * kNM) = make_float4(Gv[0], Gv[1], Gv[2], Gv[3]);
   }
 }
 
@@ -426,8 +433,23 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
   LDSMap L = carve<K>(smem, g.nelt, cap, true);
   int start[kMaxElts + 1];
   const double4 xi = pos[i];
-  const int nk = build_P<K, NS, true>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab,
-                                  L, start);
+  float acc[NS][T::KT][T::DT];
+  const int nk = build_P<K, NS, true>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos,
+                                      tab, L, start, acc);
+  // P -> LDS [b][k][d] (overwrites the monomial rows)
+  if (lane < T::LANES) {
+    const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
+#pragma unroll
+    for (int b = 0; b < NS; b++) {
+      if (b >= g.nelt) break;
+#pragma unroll
+      for (int a = 0; a < T::KT; a++)
+#pragma unroll
+        for (int q = 0; q < T::DT; q++)
+          L.PS[((size_t) b * K + kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
+    }
+  }
+  __syncthreads();
 
   // S[b][k][d] = P * sum_m 2 T[m][d] g'_m, in place
   const float *grow = dEdG + (size_t) row * g.nelt * K * kNM;
@@ -449,41 +471,50 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
 
   double fi[3] = {0, 0, 0};
   double wi[6] = {0, 0, 0, 0, 0, 0};
-  // chunks in reverse: the last staged chunk is still in LDS
-  const int last = nk > 0 ? ((nk - 1) / 64) * 64 : 0;
-  for (int base = last; base >= 0 && nk > 0; base -= 64) {
+  // two lanes per neighbor, each summing half of the filters; chunks in
+  // reverse so the last staged chunk is reused
+  const int c = lane >> 1, half = lane & 1;
+  constexpr int KH = K / 2;
+  const int last = nk > 0 ? ((nk - 1) / kChunk) * kChunk : 0;
+  for (int base = last; base >= 0 && nk > 0; base -= kChunk) {
     if (base != last) stage<K, true>(g, cell, xi, pos, tab, base, nk, L, false);
-    const int c = lane;
-    if (base + c < nk) {
-      int b = 0;
-      while (b + 1 < g.nelt && base + c >= start[b + 1]) b++;
-      float4 rc = L.rec[c];
-      const float x = rc.x, y = rc.y, z = rc.z;
-      float M[kND];
-      monomials20(x, y, z, M);
-      float V[kND];
+    const bool live = base + c < nk;
+    int b = 0;
+    while (b + 1 < g.nelt && base + c >= start[b + 1]) b++;
+    float4 rc = L.rec[c];
+    const float x = rc.x, y = rc.y, z = rc.z;
+    float M[kND];
+    monomials20(x, y, z, M);
+    float V[kND];
 #pragma unroll
-      for (int q = 0; q < kND; q++) V[q] = 0.0f;
-      float f1 = 0.0f;
-      const float *S = L.PS + (size_t) b * K * kND;
-      const float *Hr = L.H + c * HS;
-      const float *dHr = L.dH + c * HS;
+    for (int q = 0; q < kND; q++) V[q] = 0.0f;
+    float f1 = 0.0f;
+    if (live) {
+      const float *S = L.PS + ((size_t) b * K + half * KH) * kND;
+      const float *Hr = L.H + c * HS + half * KH;
+      const float *dHr = L.dH + c * HS + half * KH;
 #pragma unroll 4
-      for (int kk = 0; kk < K; kk++) {
+      for (int kk = 0; kk < KH; kk++) {
         float h = Hr[kk], dh = dHr[kk];
         const float4 *Sk = reinterpret_cast<const float4 *>(S + kk * kND);
         float U = 0.0f;
 #pragma unroll
         for (int q4 = 0; q4 < kND / 4; q4++) {
-          float4 s = Sk[q4];
-          U += s.x * M[4 * q4] + s.y * M[4 * q4 + 1] + s.z * M[4 * q4 + 2] + s.w * M[4 * q4 + 3];
-          V[4 * q4] += s.x * h;
-          V[4 * q4 + 1] += s.y * h;
-          V[4 * q4 + 2] += s.z * h;
-          V[4 * q4 + 3] += s.w * h;
+          float4 sv = Sk[q4];
+          U += sv.x * M[4 * q4] + sv.y * M[4 * q4 + 1] + sv.z * M[4 * q4 + 2] + sv.w * M[4 * q4 + 3];
+          V[4 * q4] += sv.x * h;
+          V[4 * q4 + 1] += sv.y * h;
+          V[4 * q4 + 2] += sv.z * h;
+          V[4 * q4 + 3] += sv.w * h;
         }
         f1 += U * dh;
       }
+    }
+    // combine the two halves
+#pragma unroll
+    for (int q = 0; q < kND; q++) V[q] += __shfl_xor(V[q], 1);
+    f1 += __shfl_xor(f1, 1);
+    if (live && half == 0) {
       // g_beta = sum_d V_d dM_d/du_beta
       float gx = V[1] + 2.f * x * V[4] + y * V[5] + z * V[6] + 3.f * x * x * V[10] +
           2.f * x * y * V[11] + 2.f * x * z * V[12] + y * y * V[13] + y * z * V[14] + z * z * V[15];

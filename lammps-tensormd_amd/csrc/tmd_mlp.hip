@@ -33,10 +33,17 @@ __device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (
 typedef float f32x8 __attribute__((ext_vector_type(8)));
 __device__ inline float bf2f(uint16_t v) { return __uint_as_float(((uint32_t) v) << 16); }
 
-// activation value and derivative (nnp.cu:9-49, activation.hpp:158-261)
-template <int ACT>
+// activation value and derivative (nnp.cu:9-49, activation.hpp:158-261).
+// FAST (BF16 mode only, whose operands are rounded to 8 bits anyway): tanh
+// from one exp and one reciprocal, tanh z = 1 - 2 / (exp(2z) + 1).
+template <int ACT, bool FAST = false>
 __device__ inline float act_f(float z, float &dz)
 {
+  if (ACT == 2 && FAST) {
+    float h = 1.0f - 2.0f * __frcp_rn(__expf(2.0f * z) + 1.0f);
+    dz = 1.0f - h * h;
+    return h;
+  }
   if (ACT == 1) {  // softplus; linear above -(log(eps_f32)+2) (nnp.cu:44-45)
     float ez = __expf(z);
     float h = (z < 13.942385f) ? log1pf(ez) : z;
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const f
           for (int i = 0; i < 4; i++) {
             int row = r * 16 + (lane >> 4) * 4 + i;
             float dz;
-            float h = act_f<ACT>(acc[r][j][i] + bv, dz);
+            float h = act_f<ACT, MODE == 1>(acc[r][j][i] + bv, dz);
             if (mp.res[l]) h += to_f<T>(Hin[row * ldi + col]);
             H[row * ldo + col] = from_f<T>(h);
             if (mp.storeD) D[row * ldo + col] = from_f<T>(dz);
@@ -414,7 +421,9 @@ hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const 
 {
   if (rl.nrows <= 0) return hipSuccess;
   const size_t limit = 160 * 1024;
-  static const int Rs[3] = {64, 32, 16};
+  // 32-row tiles: several workgroups per CU hide the LDS/L2 latency better
+  // than one 64-row tile, at twice the (L2-resident) weight traffic
+  static const int Rs[3] = {32, 16, 16};
   for (int q = 0; q < 3; q++) {
     int R = Rs[q];
     MlpParams m;

@@ -11,6 +11,13 @@ descriptor + MLP forward/backward + forces/virial + FP64 energy/virial sums.
 `value` is the BF16-MLP path's atom-steps/s; the plain FP32 path is timed in
 the same run (`fp32_atom_steps_per_s`).
 
+Serving (the `serving` object): 4096 single-point requests, periodic BCC/FCC
+supercells with log-uniform atom counts in [16, 4096] and random
+composition over the 4 elements (jitter 0.1 A), same model.  Measured:
+one-at-a-time FP32 requests (one slot, one structure per batch), the
+saturated BF16 request loop (all submitted at once), and Poisson arrivals at
+0.5x and 0.9x of that capacity; latency = submit -> results on the host.
+
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
@@ -34,6 +41,65 @@ def mlp_flops_per_atom(sizes):
     return 4 * sum(a * b for a, b in zip(sizes[:-1], sizes[1:]))
 
 
+def _lat(res, start="t_submit"):
+    lat = np.array([r["t_done"] - r[start] for r in res]) * 1e3
+    return dict(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                max_ms=float(lat.max()))
+
+
+def serve_bench(model, symbols, nreq, seed=11):
+    """Request-loop measurements (see the module docstring)."""
+    from tensormd_amd import TensorMD, workloads as W
+    reqs = W.serving_requests(nreq, seed=seed)
+    natoms = int(sum(len(r[0]) for r in reqs))
+    zero = np.zeros(nreq)
+    out = dict(requests=nreq, atoms=natoms, mean_atoms=natoms / nreq)
+
+    m32 = TensorMD(model, symbols, "fp32", 0.001, device=0)
+    srv = m32.serve(slots=1, max_batch_structs=1)
+    srv.replay(reqs[:64], zero[:64])  # warm: captures and tile tuning
+    t0 = time.perf_counter()
+    res, _ = srv.replay(reqs, zero)
+    dt = time.perf_counter() - t0
+    # one at a time: one slot, one structure per batch; all requests queue at
+    # once, so the latency reported is each request's own service time
+    # (batch launch -> results on the host)
+    serial = dict(structs_per_s=nreq / dt, **_lat(res, "t_start"))
+    assert all(r["status"] == 0 for r in res)
+    srv.stop()
+    m32.close()
+    out["serial_fp32"] = serial
+
+    mb = TensorMD(model, symbols, "bf16", 0.001, device=0)
+    srv = mb.serve()
+    srv.replay(reqs[:512], zero[:512])  # warm every bucket
+    t0 = time.perf_counter()
+    res, _ = srv.replay(reqs, zero)
+    dt = time.perf_counter() - t0
+    assert all(r["status"] == 0 for r in res)
+    cap = nreq / dt
+    out["saturated_bf16"] = dict(structs_per_s=cap, atoms_per_s=natoms / dt,
+                                 mean_batch=float(np.mean([r["batch_structs"] for r in res])),
+                                 **_lat(res))
+    rng = np.random.default_rng(seed + 1)
+    for frac in (0.5, 0.9):
+        rate = frac * cap
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+        arr -= arr[0]
+        t0 = time.perf_counter()
+        res, _ = srv.replay(reqs, arr)
+        dt = time.perf_counter() - t0
+        assert all(r["status"] == 0 for r in res)
+        out[f"poisson_{frac}x"] = dict(offered_per_s=rate, structs_per_s=nreq / dt,
+                                        mean_batch=float(np.mean([r["batch_structs"] for r in res])),
+                                        **_lat(res))
+    out["stats"] = srv.stats()
+    srv.stop()
+    mb.close()
+    out["batched_over_serial"] = out["saturated_bf16"]["structs_per_s"] / serial["structs_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -43,6 +109,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--precisions", default="fp32,bf16")
+    ap.add_argument("--serve-requests", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -109,6 +176,11 @@ def main():
     roof["measured_copy_gbs"] = copy_gbs
     roof["phases_ms"] = phases
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_bench(model, symbols, args.serve_requests)
+        print(f"bench: serving {json.dumps(serving)}", flush=True)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -150,6 +222,7 @@ def main():
         "bf16_over_fp32": bf["rate"] / fp["rate"],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -25,6 +25,9 @@
  *                       (src/neighbor.cpp:2293-2360), FixNVE
  *                       (src/fix_nve.cpp:64-130), kept on the device
  *   tmd_destroy         PairTensorMD::~PairTensorMD (pair_tensormd.cpp:68-83)
+ *   tmd_serve_*         no reference counterpart: batched single-point
+ *                       requests (each one a PairTensorMD::compute of a
+ *                       small periodic structure) with continuous batching
  */
 #ifndef TENSORMD_AMD_H
 #define TENSORMD_AMD_H
@@ -41,6 +44,7 @@ extern "C" {
 #define TMD_ERR_NOMEM (-4)    /* device allocation failed or over budget */
 #define TMD_ERR_CAPACITY (-5) /* a fixed capacity was exceeded */
 #define TMD_ERR_STATE (-6)    /* call not valid in the current state */
+#define TMD_PENDING 1         /* tmd_serve_poll: request not finished yet */
 
 /* ---- MLP arithmetic ------------------------------------------------------ */
 /* Descriptors and forces are FP32 with FP64 displacements and FP64
@@ -135,6 +139,82 @@ int tmd_system_neighbors(tmd_system *sys, long cap, long *off, int *j, int *shif
 /* Descriptor rows G[natoms][width] and dE/dG of the last compute (FP32,
  * atom order), for kernel-level parity tests. */
 int tmd_system_descriptors(tmd_system *sys, float *G, float *dEdG);
+
+/* ---- serving: batched single-point requests --------------------------------
+ * A request is one periodic structure (any triclinic cell at least rc/2
+ * wide in every direction); its energy, virial and forces are those of a
+ * tmd_system with skin 0.  Requests are packed into batches (continuous
+ * batching): a library thread launches a batch on a free slot (its own
+ * stream and device buffers) as soon as one is free, taking every queued
+ * request that fits the slot's capacities, so requests that arrive while
+ * others run are admitted as slots free up and batches on different slots
+ * overlap on the device.  Slot capacities are fixed at tmd_serve_start from
+ * the free device memory.  A batch runs as one captured HIP graph per
+ * capacity bucket (sizes are read on the device), and the MLP row tile of
+ * each bucket is chosen by timing the candidates on first use.
+ *
+ * Energies and virials of a structure do not depend on what else is in the
+ * batch (fixed-order sums); forces agree to FP64 rounding (atomics). */
+typedef long long tmd_ticket;
+
+typedef struct tmd_serve_options {
+  int slots;             /* batches in flight (default 2) */
+  int max_batch_atoms;   /* atoms per batch (default 65536; lowered to fit) */
+  int max_batch_structs; /* structures per batch (default 512) */
+  double hbm_fraction;   /* share of free device memory for the slots (0.25) */
+  int use_graph;         /* 1: one captured graph per capacity bucket (default) */
+  int autotune;          /* 1: time the MLP row tiles per bucket (default) */
+  int max_queue;         /* queued requests before submit refuses (65536) */
+} tmd_serve_options;
+
+typedef struct tmd_structure {
+  int natoms;
+  const double *x;   /* [natoms][3], copied at submit */
+  const int *type;   /* [natoms], 1-based LAMMPS types */
+  double cell[9];    /* rows a, b, c */
+  double *f;         /* out [natoms][3] or NULL; written before the request
+                        completes, must stay valid until then */
+  double *eatom;     /* out [natoms] or NULL (same rule) */
+} tmd_structure;
+
+typedef struct tmd_result {
+  int status;          /* TMD_OK or an error code for this request */
+  double energy;       /* sum of atomic energies (FP64) */
+  double virial[6];    /* sum over pairs of -d (x) F: xx yy zz xy xz yz */
+  double t_submit;     /* tmd_clock() at submit */
+  double t_start;      /* when its batch was launched */
+  double t_done;       /* when its results were written */
+  int batch_structs;   /* size of the batch it ran in */
+  int batch_atoms;
+} tmd_result;
+
+void tmd_serve_options_default(tmd_serve_options *opt);
+/* Start the request loop of `ctx` (one per context). */
+int tmd_serve_start(tmd_ctx *ctx, const tmd_serve_options *opt);
+/* Queue one structure; *ticket identifies it.  Refuses (TMD_ERR_CAPACITY)
+ * when max_queue requests are waiting and rejects structures larger than a
+ * slot (TMD_ERR_NOMEM) or with invalid types or cells (TMD_ERR_ARG). */
+int tmd_serve_submit(tmd_ctx *ctx, const tmd_structure *s, tmd_ticket *ticket);
+/* TMD_PENDING while running; TMD_OK once done (fills *res and releases the
+ * ticket); TMD_ERR_ARG for an unknown ticket. */
+int tmd_serve_poll(tmd_ctx *ctx, tmd_ticket ticket, tmd_result *res);
+/* Block until the request is done, then as tmd_serve_poll. */
+int tmd_serve_wait(tmd_ctx *ctx, tmd_ticket ticket, tmd_result *res);
+/* Finish every queued request, then stop the loop. */
+int tmd_serve_stop(tmd_ctx *ctx);
+/* st[0] batches, st[1] structures, st[2] atoms, st[3] graph captures,
+ * st[4] batches re-run after a neighbor-list overflow, st[5] summed device
+ * time of the batches (ms), st[6] slot capacity in atoms, st[7] slots,
+ * st[8..15] MLP row tile chosen per bucket (largest bucket first, 0 = not
+ * tuned yet), st[16..23] bucket capacities in atoms. */
+int tmd_serve_stats(tmd_ctx *ctx, double *st24);
+/* Load generator: submit s[k] at arrival[k] seconds after the call (from
+ * the calling thread; a full queue delays the submit), then wait for every
+ * request; res[k] receives each result (status carries submit errors). */
+int tmd_serve_replay(tmd_ctx *ctx, int n, const tmd_structure *s, const double *arrival,
+                     tmd_result *res);
+/* Monotonic clock in seconds (the clock of tmd_result times). */
+double tmd_clock(void);
 
 /* ---- measurement ----------------------------------------------------------
  * Device-to-device copy of `bytes` (read + write) repeated `iters` times on

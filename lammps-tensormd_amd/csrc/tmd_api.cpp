@@ -175,38 +175,53 @@ int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows)
   hipError_t e;
   if ((e = w.G.ensure((size_t) nrows * gw * 4)) != hipSuccess) return hip_fail(ctx, e, "G alloc");
   if ((e = w.dEdG.ensure((size_t) nrows * gw * 4)) != hipSuccess) return hip_fail(ctx, e, "dEdG alloc");
-  if ((e = w.E.ensure((size_t) nrows * 4)) != hipSuccess) return hip_fail(ctx, e, "E alloc");
-  if ((e = w.wrow.ensure((size_t) nrows * 6 * 8)) != hipSuccess) return hip_fail(ctx, e, "wrow alloc");
+  if ((e = w.E.ensure((size_t) nrows * 8)) != hipSuccess) return hip_fail(ctx, e, "E alloc");
   if ((e = w.row2atom.ensure((size_t) nrows * 4)) != hipSuccess) return hip_fail(ctx, e, "row alloc");
   if ((e = w.red.ensure(4096 * 8)) != hipSuccess) return hip_fail(ctx, e, "red alloc");
   return TMD_OK;
 }
 
+int ensure_atoms(tmd_ctx *ctx, Workspace &w, int natoms)
+{
+  hipError_t e = w.ew.ensure((size_t) std::max(natoms, 1) * 7 * sizeof(double));
+  return e == hipSuccess ? TMD_OK : hip_fail(ctx, e, "eatom/watom alloc");
+}
+
 // descriptor -> MLP fwd/bwd -> force kernel -> FP64 reductions
-int run_chain(tmd_ctx *ctx, Workspace &w, const Cell &cell, const double4 *pos, const uint32_t *nbr,
-              const int *nn, int cap, double scale, double *f, size_t fbytes, double *vatom,
-              size_t vbytes, hipStream_t s, PhaseEvents *pe)
+int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEvents *pe)
 {
   const int mode = ctx->opt.mlp_precision == TMD_MLP_FP32 ? 0 : 1;
   hipError_t e;
   const int *row2atom = (const int *) w.row2atom.p;
+  double *eatom = (double *) w.ew.p;
+  double *watom = eatom + c.natoms;
   if (pe) hipEventRecord(pe->ev[0], s);
-  e = launch_descriptor(ctx->geo, cell, pos, nbr, nn, cap, row2atom, w.rl.nrows,
+  e = launch_descriptor(ctx->geo, c.geo, c.pos, c.nbr, c.nn, c.cap, row2atom, w.rl.nrows,
                         (const float4 *) ctx->tab.p, (float *) w.G.p, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
   if (pe) hipEventRecord(pe->ev[1], s);
-  e = launch_mlp(mode, w.rl, ctx->mlp, (const float *) w.G.p, (float *) w.E.p, (float *) w.dEdG.p, s);
+  e = launch_mlp(mode, w.rl, ctx->mlp, (const float *) w.G.p, (double *) w.E.p, (float *) w.dEdG.p, s,
+                 c.rtile, c.dyn);
   if (e != hipSuccess) return hip_fail(ctx, e, "mlp launch");
   if (pe) hipEventRecord(pe->ev[2], s);
-  if ((e = launch_zero(f, fbytes, s)) != hipSuccess) return hip_fail(ctx, e, "zero f");
-  if (vatom && (e = launch_zero(vatom, vbytes, s)) != hipSuccess) return hip_fail(ctx, e, "zero vatom");
-  e = launch_force(ctx->geo, cell, pos, nbr, nn, cap, row2atom, w.rl.nrows,
-                   (const float4 *) ctx->tab.p, (const float *) w.dEdG.p, scale, f,
-                   (double *) w.wrow.p, vatom, s);
+  if (c.dyn)
+    e = launch_zero_atoms(c.f, 3, (int) (c.fbytes / (3 * sizeof(double))), c.dyn, s);
+  else
+    e = launch_zero(c.f, c.fbytes, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "zero f");
+  if (c.vatom && (e = launch_zero(c.vatom, c.vbytes, s)) != hipSuccess)
+    return hip_fail(ctx, e, "zero vatom");
+  if (c.zero_ew && (e = launch_zero(eatom, (size_t) c.natoms * 7 * sizeof(double), s)) != hipSuccess)
+    return hip_fail(ctx, e, "zero eatom");
+  e = launch_force(ctx->geo, c.geo, c.pos, c.nbr, c.nn, c.cap, row2atom, w.rl.nrows,
+                   (const float4 *) ctx->tab.p, (const float *) w.dEdG.p, (const double *) w.E.p,
+                   c.scale, c.f, eatom, watom, c.vatom, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "force launch");
   if (pe) hipEventRecord(pe->ev[3], s);
-  e = launch_reduce_ev((const float *) w.E.p, (const double *) w.wrow.p, row2atom, w.rl.nrows, scale,
-                       (double *) w.red.p, s);
+  if (c.struct_off)
+    e = launch_reduce_struct(eatom, watom, c.struct_off, c.nstruct, c.struct_out, s, c.dyn);
+  else
+    e = launch_reduce_ev(eatom, watom, c.natoms, (double *) w.red.p, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "reduce launch");
   if (pe) hipEventRecord(pe->ev[4], s);
   return TMD_OK;
@@ -318,6 +333,10 @@ void tmd_destroy(tmd_ctx *ctx)
 {
   if (!ctx) return;
   hipSetDevice(ctx->device);
+  if (ctx->server) {
+    server_destroy(ctx->server);
+    ctx->server = nullptr;
+  }
   hipStreamSynchronize(ctx->stream);
   ctx->list_ws = Workspace();
   ctx->tab.release();
@@ -444,30 +463,36 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   hipMemcpyAsync(w.nbr.p, list.data(), 4 * list.size(), hipMemcpyHostToDevice, s);
   hipMemcpyAsync(w.nn.p, nn.data(), 4 * nn.size(), hipMemcpyHostToDevice, s);
   hipMemcpyAsync(w.row2atom.p, row2atom.data(), 4 * row2atom.size(), hipMemcpyHostToDevice, s);
-  Cell cell;
-  std::memset(&cell, 0, sizeof(cell));
-  rc = run_chain(ctx, w, cell, (const double4 *) w.pos.p, (const uint32_t *) w.nbr.p,
-                 (const int *) w.nn.p, cap, scale, (double *) w.f.p, sizeof(double) * 3 * (size_t) nall,
-                 vatom ? (double *) w.vatom.p : nullptr, sizeof(double) * 6 * (size_t) nall, s);
+  // list mode: positions are already images (ghosts), the one cell is zero
+  CellInfo zero_cell;
+  std::memset(&zero_cell, 0, sizeof(zero_cell));
+  if ((e = w.cells.ensure(sizeof(CellInfo))) != hipSuccess) return hip_fail(ctx, e, "cell alloc");
+  hipMemcpyAsync(w.cells.p, &zero_cell, sizeof(CellInfo), hipMemcpyHostToDevice, s);
+  if ((rc = ensure_atoms(ctx, w, nlocal)) != TMD_OK) return rc;
+  Chain c;
+  c.geo.cells = (const CellInfo *) w.cells.p;
+  c.pos = (const double4 *) w.pos.p;
+  c.nbr = (const uint32_t *) w.nbr.p;
+  c.nn = (const int *) w.nn.p;
+  c.cap = cap;
+  c.scale = scale;
+  c.f = (double *) w.f.p;
+  c.fbytes = sizeof(double) * 3 * (size_t) nall;
+  c.vatom = vatom ? (double *) w.vatom.p : nullptr;
+  c.vbytes = sizeof(double) * 6 * (size_t) nall;
+  c.natoms = nlocal;
+  c.zero_ew = inum < nlocal;
+  rc = run_chain(ctx, w, c, s);
   if (rc != TMD_OK) return rc;
   double red[7];
   hipMemcpyAsync(f, w.f.p, sizeof(double) * 3 * (size_t) nall, hipMemcpyDeviceToHost, s);
   if (vatom) hipMemcpyAsync(vatom, w.vatom.p, sizeof(double) * 6 * (size_t) nall, hipMemcpyDeviceToHost, s);
   hipMemcpyAsync(red, w.red.p, sizeof(red), hipMemcpyDeviceToHost, s);
-  std::vector<float> E;
-  if (eatom) {
-    E.resize(w.rl.nrows);
-    hipMemcpyAsync(E.data(), w.E.p, 4 * (size_t) w.rl.nrows, hipMemcpyDeviceToHost, s);
-  }
+  if (eatom) hipMemcpyAsync(eatom, w.ew.p, sizeof(double) * (size_t) nlocal, hipMemcpyDeviceToHost, s);
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(ctx, e, "tmd_compute_list");
   *energy = red[0];
   if (virial)
     for (int q = 0; q < 6; q++) virial[q] = red[1 + q];
-  if (eatom) {
-    for (int i = 0; i < nlocal; i++) eatom[i] = 0.0;
-    for (int r = 0; r < w.rl.nrows; r++)
-      if (row2atom[r] >= 0) eatom[row2atom[r]] = (double) E[r] * scale;
-  }
   return TMD_OK;
 }
 

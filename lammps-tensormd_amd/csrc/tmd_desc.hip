@@ -38,8 +38,22 @@ struct Tiling {
   static constexpr int HS = K + 1;              // odd row stride: per-lane rows conflict-free
 };
 
+// Lattice rows of the cell an atom belongs to (uniform per workgroup).
+struct Box {
+  double h[9];
+};
+
+__device__ inline Box load_box(const Geom &geo, int i)
+{
+  const CellInfo &c = geo.cells[geo.atom_struct ? geo.atom_struct[i] : 0];
+  Box b;
+#pragma unroll
+  for (int q = 0; q < 9; q++) b.h[q] = c.h[q];
+  return b;
+}
+
 __device__ inline void displacement(const double4 &xi, const double4 &xj, uint32_t code,
-                                    const Cell &c, double d[3])
+                                    const Box &c, double d[3])
 {
 #pragma clang fp contract(off)
   int s0 = (int) (code / 49u) - 3, s1 = (int) ((code / 7u) % 7u) - 3, s2 = (int) (code % 7u) - 3;
@@ -156,7 +170,7 @@ __device__ inline LDSMap carve(unsigned char *base, int nelt, int cap, bool forc
 }
 
 // Phase 1: compact the kept entries (slot-sorted).  start[b] .. start[b+1].
-__device__ inline int kept_slot(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
+__device__ inline int kept_slot(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
                                 uint32_t e, const double4 &xj)
 {
   double d[3];
@@ -165,7 +179,7 @@ __device__ inline int kept_slot(const ModelGeo &g, const Cell &cell, const doubl
   return (r2 < g.rc2 && r2 > 1e-10) ? g.eltij[ei][(int) xj.w] : -1;  // kernel.cu:1163
 }
 
-__device__ inline int compact(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
+__device__ inline int compact(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
                               const uint32_t *list, int nraw, const double4 *pos, uint32_t *kept,
                               int *start)
 {
@@ -242,7 +256,7 @@ __device__ inline int compact(const ModelGeo &g, const Cell &cell, const double4
 
 // Phase 2 staging of kept neighbors [base, base+kChunk): geometry, monomials, H (and dH)
 template <int K, bool FORCE>
-__device__ inline void stage(const ModelGeo &g, const Cell &cell, const double4 &xi,
+__device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &xi,
                              const double4 *pos, const float4 *tab, int base, int nk, LDSMap &L,
                              bool want_M)
 {
@@ -335,7 +349,7 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
 
 // Phases 1-2: P of every slot in the lane's register block.  Returns nk.
 template <int K, int NS, bool FORCE>
-__device__ inline int build_P(const ModelGeo &g, const Cell &cell, const double4 &xi, int ei,
+__device__ inline int build_P(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
                               const uint32_t *list, int nraw, const double4 *pos, const float4 *tab,
                               LDSMap &L, int *start, float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
 {
@@ -357,14 +371,14 @@ __device__ inline int build_P(const ModelGeo &g, const Cell &cell, const double4
 }
 
 template <int K, int NS>
-__global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Cell cell, const double4 *pos,
+__global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const double4 *pos,
                                                    const uint32_t *nbr, const int *nn, int cap,
                                                    const int *row2atom, int nrows,
                                                    const float4 *tab, float *G)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   const int row = blockIdx.x;
-  if (row >= nrows) return;
+  if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int gw = g.nelt * K * kNM;
   float *Grow = G + (size_t) row * gw;
@@ -377,6 +391,7 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Cell cell, const 
   LDSMap L = carve<K>(smem, g.nelt, cap, false);
   int start[kMaxElts + 1];
   const double4 xi = pos[i];
+  const Box cell = load_box(geo, i);
   float acc[NS][T::KT][T::DT];
   build_P<K, NS, false>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start,
                         acc);
@@ -413,26 +428,25 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Cell cell, const 
 }
 
 template <int K, int NS>
-__global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const double4 *pos,
+__global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double4 *pos,
                                               const uint32_t *nbr, const int *nn, int cap,
                                               const int *row2atom, int nrows, const float4 *tab,
-                                              const float *dEdG, double scale, double *f,
-                                              double *wrow, double *vatom)
+                                              const float *dEdG, const double *E, double scale,
+                                              double *f, double *eatom, double *watom,
+                                              double *vatom)
 {
   using T = Tiling<K>;
   constexpr int HS = T::HS;
   extern __shared__ __align__(16) unsigned char smem[];
   const int row = blockIdx.x;
-  if (row >= nrows) return;
+  if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int i = row2atom[row];
-  if (i < 0) {
-    if (lane < 6) wrow[(size_t) row * 6 + lane] = 0.0;
-    return;
-  }
+  if (i < 0) return;  // padding row
   LDSMap L = carve<K>(smem, g.nelt, cap, true);
   int start[kMaxElts + 1];
   const double4 xi = pos[i];
+  const Box cell = load_box(geo, i);
   float acc[NS][T::KT][T::DT];
   const int nk = build_P<K, NS, true>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos,
                                       tab, L, start, acc);
@@ -561,13 +575,14 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Cell cell, const doubl
   }
   if (lane < 6) {
     double v = wi[lane];
-    wrow[(size_t) row * 6 + lane] = v;
+    watom[6 * (size_t) i + lane] = v;
     if (vatom) atomicAdd(&vatom[6 * (size_t) i + lane], 0.5 * v);
   }
+  if (lane == 6) eatom[i] = scale * E[row];
 }
 
 template <int K, int NS>
-hipError_t desc_impl(const ModelGeo &g, const Cell &cell, const double4 *pos, const uint32_t *nbr,
+hipError_t desc_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
                      const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
                      float *G, hipStream_t s)
 {
@@ -575,41 +590,42 @@ hipError_t desc_impl(const ModelGeo &g, const Cell &cell, const double4 *pos, co
   size_t lds = lds_layout<K>(g.nelt, cap, false, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
   hipFuncSetAttribute((const void *) k_descriptor<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-  k_descriptor<K, NS><<<nrows, 64, lds, s>>>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G);
+  k_descriptor<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G);
   return hipGetLastError();
 }
 
 template <int K, int NS>
-hipError_t force_impl(const ModelGeo &g, const Cell &cell, const double4 *pos, const uint32_t *nbr,
+hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
                       const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
-                      const float *dEdG, double scale, double *f, double *wrow, double *vatom,
-                      hipStream_t s)
+                      const float *dEdG, const double *E, double scale, double *f, double *eatom,
+                      double *watom, double *vatom, hipStream_t s)
 {
   size_t offs[8];
   size_t lds = lds_layout<K>(g.nelt, cap, true, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
   hipFuncSetAttribute((const void *) k_force<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-  k_force<K, NS><<<nrows, 64, lds, s>>>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, scale,
-                                        f, wrow, vatom);
+  k_force<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, E,
+                                        scale, f, eatom, watom, vatom);
   return hipGetLastError();
 }
 
 // slot capacity: the smallest of 1, 2, 4, 8 that holds nelt, within a
 // register budget of 128 accumulators per lane
 template <int K, bool FORCE>
-hipError_t dispatch(const ModelGeo &g, const Cell &cell, const double4 *pos, const uint32_t *nbr,
+hipError_t dispatch(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
                     const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
-                    float *G, const float *dEdG, double scale, double *f, double *wrow,
-                    double *vatom, hipStream_t s)
+                    float *G, const float *dEdG, const double *E, double scale, double *f,
+                    double *eatom, double *watom, double *vatom, hipStream_t s)
 {
   constexpr int per = Tiling<K>::KT * Tiling<K>::DT;
 #define TMD_NS_CASE(NS)                                                                            \
   if (g.nelt <= NS) {                                                                              \
     if (NS * per > 128) return hipErrorInvalidConfiguration;                                       \
     if (FORCE)                                                                                     \
-      return force_impl<K, (NS * per > 128 ? 1 : NS)>(g, cell, pos, nbr, nn, cap, row2atom, nrows, \
-                                                       tab, dEdG, scale, f, wrow, vatom, s);        \
-    return desc_impl<K, (NS * per > 128 ? 1 : NS)>(g, cell, pos, nbr, nn, cap, row2atom, nrows,    \
+      return force_impl<K, (NS * per > 128 ? 1 : NS)>(g, geo, pos, nbr, nn, cap, row2atom, nrows,  \
+                                                       tab, dEdG, E, scale, f, eatom, watom, vatom, \
+                                                       s);                                          \
+    return desc_impl<K, (NS * per > 128 ? 1 : NS)>(g, geo, pos, nbr, nn, cap, row2atom, nrows,     \
                                                    tab, G, s);                                      \
   }
   TMD_NS_CASE(1)
@@ -622,15 +638,15 @@ hipError_t dispatch(const ModelGeo &g, const Cell &cell, const double4 *pos, con
 
 }  // namespace
 
-hipError_t launch_descriptor(const ModelGeo &g, const Cell &cell, const double4 *pos,
+hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *pos,
                              const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
                              int nrows, const float4 *tab, float *G, hipStream_t stream)
 {
   if (nrows <= 0) return hipSuccess;
 #define TMD_K_CASE(K)                                                                          \
   case K:                                                                                      \
-    return dispatch<K, false>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, G, nullptr, 0, \
-                              nullptr, nullptr, nullptr, stream);
+    return dispatch<K, false>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, nullptr,     \
+                              nullptr, 0, nullptr, nullptr, nullptr, nullptr, stream);
   switch (g.k) {
     TMD_K_CASE(16)
     TMD_K_CASE(24)
@@ -642,16 +658,17 @@ hipError_t launch_descriptor(const ModelGeo &g, const Cell &cell, const double4 
 #undef TMD_K_CASE
 }
 
-hipError_t launch_force(const ModelGeo &g, const Cell &cell, const double4 *pos,
+hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos,
                         const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
-                        int nrows, const float4 *tab, const float *dEdG, double scale, double *f,
-                        double *wrow, double *vatom, hipStream_t stream)
+                        int nrows, const float4 *tab, const float *dEdG, const double *E,
+                        double scale, double *f, double *eatom, double *watom, double *vatom,
+                        hipStream_t stream)
 {
   if (nrows <= 0) return hipSuccess;
 #define TMD_K_CASE(K)                                                                           \
   case K:                                                                                       \
-    return dispatch<K, true>(g, cell, pos, nbr, nn, cap, row2atom, nrows, tab, nullptr, dEdG,   \
-                             scale, f, wrow, vatom, stream);
+    return dispatch<K, true>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, nullptr, dEdG, E, \
+                             scale, f, eatom, watom, vatom, stream);
   switch (g.k) {
     TMD_K_CASE(16)
     TMD_K_CASE(24)

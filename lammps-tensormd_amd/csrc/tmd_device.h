@@ -18,9 +18,15 @@ constexpr int kMaxElts = 8;
 constexpr int kND = 20;  // monomials for max_moment 3
 constexpr int kNM = 4;
 
-struct Cell {
+// One periodic cell (a batch of structures has one per structure).  The
+// neighbor-build fields are ignored by the descriptor/force kernels.
+struct CellInfo {
   double h[9];    // rows a, b, c
   double inv[9];  // inverse (fractional = x * inv)
+  int nb[3];      // bins per axis
+  int reach[3];   // stencil half-width per axis
+  int bin_base;   // first global bin of this cell
+  int pad;
 };
 
 // Per-model constants passed by value to the descriptor/force kernels.
@@ -41,6 +47,18 @@ constexpr int kRowTile = 64;
 struct RowLayout {
   int nrows;                     // padded total
   int row_off[kMaxElts + 1];     // element e occupies [row_off[e], row_off[e+1])
+};
+
+// Sizes of one batch of structures, kept in device memory so that one
+// captured graph (launched with grids sized for the capacities) serves
+// every batch up to those capacities.  Kernels given a non-null BatchDims
+// read their extents from it; workgroups beyond them exit at once.
+struct BatchDims {
+  int natoms;
+  int nbins;
+  int nstruct;
+  int pad;
+  RowLayout rl;
 };
 
 // The per-element MLP, all elements packed: layer l of element e starts at
@@ -74,8 +92,10 @@ struct NbrBuild {
   int cap;            // entries per atom
   double cut2;        // (rc + skin)^2
   double trig2;       // (skin/2)^2
-  int nb[3];          // bins per axis
-  int reach[3];       // stencil half-width per axis
+  int nbins;          // total bins over all cells
+  const CellInfo *cells;   // [ncells] (device)
+  const int *atom_struct;  // [n] cell of each atom, or null (one cell)
+  const BatchDims *dyn;    // device extents (n, nbins) when non-null
   int *bin_count;     // [nbins + 1]
   int *bin_start;     // [nbins + 1]
   int *bin_cursor;    // [nbins]
@@ -86,35 +106,102 @@ struct NbrBuild {
   unsigned long long *ctl;  // [0] max d^2 bits, [1] force flag, [2] overflow, [3] build count
 };
 
-hipError_t nbr_step(const NbrBuild &nb, const Cell &cell, double4 *pos, uint32_t *nbr, int *nn,
-                    hipStream_t stream);
+hipError_t nbr_step(const NbrBuild &nb, double4 *pos, uint32_t *nbr, int *nn, hipStream_t stream);
+// force the next nbr_step to rebuild (device-side flag; graph-safe)
+hipError_t nbr_force_rebuild(const NbrBuild &nb, hipStream_t stream);
 
 // ---- descriptor / force -------------------------------------------------------
-hipError_t launch_descriptor(const ModelGeo &g, const Cell &cell, const double4 *pos,
+// Geometry of an evaluation: cells[atom_struct[i]] (atom_struct null: cells[0]).
+struct Geom {
+  const CellInfo *cells;
+  const int *atom_struct;
+  const BatchDims *dyn;  // rows beyond dyn->rl.nrows are skipped when non-null
+};
+hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *pos,
                              const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
                              int nrows, const float4 *tab, float *G, hipStream_t stream);
 
-// Force/virial: dEdG rows -> f (atomics, FP64, must be zeroed), per-row
-// virial W[row][6] (sum over the atom's pairs of -d (x) F), optional vatom
+// Force/virial: dEdG rows -> f (atomics, FP64, must be zeroed); per atom:
+// eatom[i] = scale * E[row] and watom[i][6] = sum over the atom's pairs of
+// -d (x) F (both written only for atoms that own a row); optional vatom
 // (FP64 atomics; must be zeroed).
-hipError_t launch_force(const ModelGeo &g, const Cell &cell, const double4 *pos,
+hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos,
                         const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
-                        int nrows, const float4 *tab, const float *dEdG, double scale,
-                        double *f, double *wrow, double *vatom, hipStream_t stream);
+                        int nrows, const float4 *tab, const float *dEdG, const double *E,
+                        double scale, double *f, double *eatom, double *watom, double *vatom,
+                        hipStream_t stream);
 
 // ---- MLP ----------------------------------------------------------------------
-// mode: 0 fp32, 1 bf16.  E[row] (fp32) and dEdG[row][width] (fp32).
-hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, float *E,
-                      float *dEdG, hipStream_t stream);
+// mode: 0 fp32, 1 bf16.  E[row] (FP64 output sum) and dEdG[row][width]
+// (fp32).  rtile: rows per workgroup (16, 32 or 64; 0 = default).
+// With dyn, rl gives the capacity (grid) and dyn->rl the live layout.
+hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, double *E,
+                      float *dEdG, hipStream_t stream, int rtile = 0,
+                      const BatchDims *dyn = nullptr);
 size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out);
+
+// ---- batches of structures (serving) ---------------------------------------------
+// Capacities of one batch slot (grids of its captured graph).
+struct BatchCaps {
+  int atoms, rows, bins, structs;
+  int nelt;  // model elements (valid element indices in pos.w)
+};
+// Device arrays of a batch slot.
+struct BatchArrays {
+  BatchDims *dyn;
+  int *struct_off;    // [structs + 1]
+  CellInfo *cells;    // [structs]
+  int *atom_struct;   // [atoms]
+  int *row2atom;      // [rows]
+  double4 *pos;       // [atoms]
+  const double *struct_out;  // [structs][7]
+  const double *f;           // [atoms][3]
+  const double *eatom;       // [atoms]
+  unsigned long long *ctl;   // neighbor control word
+};
+// Dense host staging of a batch: BatchDims | struct_off[ns+1] | cells[ns] |
+// atom_struct[n] | row2atom[nrows] | pos[n].  Byte offsets from the start:
+// off[1] cells, off[2] atom_struct, off[3] row2atom, off[4] pos, off[5] and
+// off[6] the total size (struct_off follows the header directly).
+__host__ __device__ inline void batch_stage_offsets(int ns, int n, int nrows, size_t *off)
+{
+  auto up = [](size_t v) { return (v + 255) & ~(size_t) 255; };
+  off[0] = 0;
+  off[1] = up(sizeof(BatchDims) + 4 * (size_t) (ns + 1));
+  off[2] = up(off[1] + sizeof(CellInfo) * (size_t) ns);
+  off[3] = up(off[2] + 4 * (size_t) n);
+  off[4] = up(off[3] + 4 * (size_t) nrows);
+  off[5] = off[4] + sizeof(double4) * (size_t) n;
+  off[6] = off[5];
+}
+// Dense host output of a batch: 8 header doubles (list overflow count,
+// atoms, structures, error flag) | struct_out[ns][7] | f[n][3] | eatom[n].
+inline size_t batch_out_bytes(int ns, int n) { return 8 * (8 + 7 * (size_t) ns + 4 * (size_t) n); }
+// Copy a staged batch (host memory, read by the kernels) into the slot's
+// device arrays, clamping every count and index to the capacities.
+hipError_t launch_batch_unpack(const void *stage, const BatchCaps &caps, const BatchArrays &a,
+                               hipStream_t stream);
+// Write a finished batch's results to host memory `out` and clear the
+// list-overflow word.
+hipError_t launch_batch_pack(void *out, const BatchCaps &caps, const BatchArrays &a,
+                             hipStream_t stream);
 
 // ---- reductions / integration ---------------------------------------------------
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t stream);
 // plain device-to-device copy (bandwidth calibration)
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t stream);
-// out[0] = scale * sum_r E[r] over real rows; out[1..6] = sum_r wrow[r][*]
-hipError_t launch_reduce_ev(const float *E, const double *wrow, const int *row2atom, int nrows,
-                            double scale, double *out, hipStream_t stream);
+// out[0] = sum_i eatom[i]; out[1..6] = sum_i watom[i][*], i < n (fixed order)
+hipError_t launch_reduce_ev(const double *eatom, const double *watom, int n, double *out,
+                            hipStream_t stream);
+// per structure s: out[7 s + 0] = sum eatom, out[7 s + 1..6] = sum watom over
+// atoms [off[s], off[s+1])
+// (with dyn: nstruct is the capacity, dyn->nstruct the live count)
+hipError_t launch_reduce_struct(const double *eatom, const double *watom, const int *off,
+                                int nstruct, double *out, hipStream_t stream,
+                                const BatchDims *dyn = nullptr);
+// p[0 .. per_atom * dyn->natoms) = 0 (grid sized for cap_atoms)
+hipError_t launch_zero_atoms(double *p, int per_atom, int cap_atoms, const BatchDims *dyn,
+                             hipStream_t stream);
 hipError_t launch_vv_first(int n, double4 *pos, double *v, const double *f,
                            const double *dtfm_by_elem, double dt, hipStream_t stream);
 hipError_t launch_vv_second(int n, const double4 *pos, double *v, const double *f,

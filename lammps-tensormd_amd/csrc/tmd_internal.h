@@ -40,11 +40,15 @@ struct DeviceBuffer {
 };
 
 // Buffers of one evaluation: positions, list, MLP activations, outputs.
+// ew holds eatom[n] followed by watom[n][6].
 struct Workspace {
   RowLayout rl{};
-  DeviceBuffer pos, nbr, nn, f, vatom;
-  DeviceBuffer G, dEdG, E, wrow, row2atom, red;
+  DeviceBuffer pos, nbr, nn, f, vatom, ew;
+  DeviceBuffer G, dEdG, E, row2atom, red, cells;
 };
+
+struct Server;
+void server_destroy(Server *s);
 
 }  // namespace tmd
 
@@ -60,6 +64,7 @@ struct tmd_ctx {
   tmd::DeviceBuffer tab, params;
   tmd::MlpParams mlp{};
   tmd::Workspace list_ws;
+  tmd::Server *server = nullptr;  // request loop (tmd_serve_start)
 };
 
 namespace tmd {
@@ -71,9 +76,31 @@ struct PhaseEvents {
 void set_err(tmd_ctx *ctx, const std::string &m);
 int hip_fail(tmd_ctx *ctx, hipError_t e, const char *what);
 int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows);
-int run_chain(tmd_ctx *ctx, Workspace &w, const Cell &cell, const double4 *pos, const uint32_t *nbr,
-              const int *nn, int cap, double scale, double *f, size_t fbytes, double *vatom,
-              size_t vbytes, hipStream_t s, PhaseEvents *pe = nullptr);
+// Inputs and outputs of one descriptor -> MLP -> force -> reduction chain.
+struct Chain {
+  Geom geo{};
+  const double4 *pos = nullptr;
+  const uint32_t *nbr = nullptr;
+  const int *nn = nullptr;
+  int cap = 0;
+  double scale = 1.0;
+  double *f = nullptr;       // zeroed, then accumulated
+  size_t fbytes = 0;
+  double *vatom = nullptr;   // optional; zeroed, then accumulated
+  size_t vbytes = 0;
+  int natoms = 0;            // eatom/watom entries (in w.ew)
+  bool zero_ew = false;      // some atoms own no row: clear eatom/watom first
+  const int *struct_off = nullptr;  // per-structure sums into struct_out[7 s ..]
+  int nstruct = 0;
+  double *struct_out = nullptr;     // (null struct_off: one sum into w.red[0..6])
+  int rtile = 0;
+  const BatchDims *dyn = nullptr;   // device extents of a batch (capture-once graphs)
+};
+int ensure_atoms(tmd_ctx *ctx, Workspace &w, int natoms);
+int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEvents *pe = nullptr);
+bool invert3(const double *h, double *inv);
+double cell_volume(const double *h);
+int setup_cell(const double *h, double cut, CellInfo &c, std::string &err);
 void make_rows(int nelt, const std::vector<int> &atoms, const std::vector<int> &elem_of_atom,
                RowLayout &rl, std::vector<int> &row2atom);
 }  // namespace tmd

@@ -225,9 +225,10 @@ __device__ inline void wave_gemm(const typename Traits<MODE>::T *lds, int ld, co
 }  // namespace
 
 template <int MODE, int ACT, int R>
-__global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const float *Gd, float *E,
-                                             float *dEdG)
+__global__ __launch_bounds__(256) void k_mlp(RowLayout rl_cap, MlpParams mp, const float *Gd, double *E,
+                                             float *dEdG, const BatchDims *dyn)
 {
+  const RowLayout rl = dyn ? dyn->rl : rl_cap;
   using T = typename Traits<MODE>::T;
   constexpr int CT = Traits<MODE>::CT, RT = R / 16;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const f
       double s = 0.0;
       for (int k = lane; k < K; k += 64) s += (double) (to_f<T>(H[r * ld + k]) * w[k]);
       for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-      if (lane == 0) E[row0 + r] = (float) (s + (double) bo);
+      if (lane == 0) E[row0 + r] = s + (double) bo;
     }
   }
   // the seed below may overwrite H_{L-2} (when D aliases H): all output dots first
@@ -360,15 +361,15 @@ __global__ __launch_bounds__(256) void k_mlp(RowLayout rl, MlpParams mp, const f
 
 namespace {
 template <int MODE, int R>
-hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, float *E, float *dEdG,
-                    size_t lds, hipStream_t s)
+hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, double *E, float *dEdG,
+                    size_t lds, hipStream_t s, const BatchDims *dyn)
 {
   dim3 grid(rl.nrows / R);
 #define TMD_MLP_CASE(A)                                                                          \
   case A:                                                                                        \
     hipFuncSetAttribute((const void *) k_mlp<MODE, A, R>,                                        \
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                  \
-    k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG);                                \
+    k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                                \
     break;
   switch (mp.act) {
     TMD_MLP_CASE(0)
@@ -416,27 +417,29 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
   return off * esz;
 }
 
-hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, float *E,
-                      float *dEdG, hipStream_t s)
+hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, double *E,
+                      float *dEdG, hipStream_t s, int rtile, const BatchDims *dyn)
 {
   if (rl.nrows <= 0) return hipSuccess;
   const size_t limit = 160 * 1024;
-  // 32-row tiles: several workgroups per CU hide the LDS/L2 latency better
-  // than one 64-row tile, at twice the (L2-resident) weight traffic
-  static const int Rs[3] = {32, 16, 16};
+  // 32-row tiles by default: several workgroups per CU hide the LDS/L2
+  // latency better than one 64-row tile, at twice the (L2-resident) weight
+  // traffic.  rtile (16/32/64) overrides the first choice.
+  int Rs[3] = {32, 16, 16};
+  if (rtile == 16 || rtile == 32 || rtile == 64) Rs[0] = rtile;
   for (int q = 0; q < 3; q++) {
     int R = Rs[q];
     MlpParams m;
     size_t lds = mlp_lds_bytes(mp, mode, R, &m);
     if (lds > limit) continue;
     if (mode == 0) {
-      if (R == 64) return launch_r<0, 64>(rl, m, G, E, dEdG, lds, s);
-      if (R == 32) return launch_r<0, 32>(rl, m, G, E, dEdG, lds, s);
-      return launch_r<0, 16>(rl, m, G, E, dEdG, lds, s);
+      if (R == 64) return launch_r<0, 64>(rl, m, G, E, dEdG, lds, s, dyn);
+      if (R == 32) return launch_r<0, 32>(rl, m, G, E, dEdG, lds, s, dyn);
+      return launch_r<0, 16>(rl, m, G, E, dEdG, lds, s, dyn);
     }
-    if (R == 64) return launch_r<1, 64>(rl, m, G, E, dEdG, lds, s);
-    if (R == 32) return launch_r<1, 32>(rl, m, G, E, dEdG, lds, s);
-    return launch_r<1, 16>(rl, m, G, E, dEdG, lds, s);
+    if (R == 64) return launch_r<1, 64>(rl, m, G, E, dEdG, lds, s, dyn);
+    if (R == 32) return launch_r<1, 32>(rl, m, G, E, dEdG, lds, s, dyn);
+    return launch_r<1, 16>(rl, m, G, E, dEdG, lds, s, dyn);
   }
   return hipErrorInvalidConfiguration;
 }

@@ -11,6 +11,11 @@
 // The rebuild decision is taken on the device: every kernel here reads the
 // control word and exits when no rebuild is due, so a step never waits for
 // the host.
+//
+// One launch serves a batch of independent structures: each atom carries
+// its cell (atom_struct), each cell owns a contiguous range of global bins,
+// and the stencil of an atom only visits its own cell's bins, so atoms of
+// different structures never see each other.
 #include "tmd_device.h"
 
 namespace tmd {
@@ -18,7 +23,7 @@ namespace tmd {
 namespace {
 
 __device__ inline void displacement(const double4 &xi, const double4 &xj, int s0, int s1, int s2,
-                                    const Cell &c, double d[3])
+                                    const CellInfo &c, double d[3])
 {
 #pragma clang fp contract(off)
   // Same expression, same order as the oracle: (xj - xi) + ((s0 h0 + s1 h1) + s2 h2)
@@ -36,6 +41,14 @@ __device__ inline double dist2(const double d[3])
   return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
 }
 
+__device__ inline const CellInfo &cell_of(const NbrBuild &nb, int i)
+{
+  return nb.cells[nb.atom_struct ? nb.atom_struct[i] : 0];
+}
+
+__device__ inline int live_atoms(const NbrBuild &nb) { return nb.dyn ? nb.dyn->natoms : nb.n; }
+__device__ inline int live_bins(const NbrBuild &nb, int nbins) { return nb.dyn ? nb.dyn->nbins : nbins; }
+
 __device__ inline bool rebuild_due(const unsigned long long *ctl, double trig2)
 {
   double m = __longlong_as_double((long long) ctl[0]);
@@ -43,8 +56,10 @@ __device__ inline bool rebuild_due(const unsigned long long *ctl, double trig2)
 }
 
 // max |x - x_build|^2 over atoms -> ctl[0] (as ordered bits of a positive double)
-__global__ void k_nb_check(int n, const double4 *pos, const double *xb, unsigned long long *ctl)
+__global__ void k_nb_check(NbrBuild nb, const double4 *pos, const double *xb, unsigned long long *ctl)
 {
+  const int n = live_atoms(nb);
+  if ((int) (blockIdx.x * blockDim.x) >= n) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -64,11 +79,12 @@ __global__ void k_nb_check(int n, const double4 *pos, const double *xb, unsigned
 }
 
 // wrap into the cell, remember build positions, bin the atoms
-__global__ void k_nb_bin(NbrBuild nb, Cell c, double4 *pos)
+__global__ void k_nb_bin(NbrBuild nb, double4 *pos)
 {
   if (!rebuild_due(nb.ctl, nb.trig2)) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb.n) return;
+  if (i >= live_atoms(nb)) return;
+  const CellInfo &c = cell_of(nb, i);
   double4 p = pos[i];
   double f[3];
   f[0] = p.x * c.inv[0] + p.y * c.inv[3] + p.z * c.inv[6];
@@ -87,11 +103,11 @@ __global__ void k_nb_bin(NbrBuild nb, Cell c, double4 *pos)
   nb.x_build[4 * i + 2] = p.z;
   int b[3];
   for (int a = 0; a < 3; a++) {
-    int q = (int) floor(f[a] * nb.nb[a]);
-    b[a] = min(max(q, 0), nb.nb[a] - 1);
+    int q = (int) floor(f[a] * c.nb[a]);
+    b[a] = min(max(q, 0), c.nb[a] - 1);
     nb.atom_bc[3 * i + a] = b[a];
   }
-  int flat = (b[0] * nb.nb[1] + b[1]) * nb.nb[2] + b[2];
+  int flat = c.bin_base + (b[0] * c.nb[1] + b[1]) * c.nb[2] + b[2];
   nb.atom_bin[i] = flat;
   atomicAdd(&nb.bin_count[flat], 1);
 }
@@ -100,6 +116,7 @@ __global__ void k_nb_bin(NbrBuild nb, Cell c, double4 *pos)
 __global__ void k_nb_scan(NbrBuild nb, int nbins)
 {
   if (!rebuild_due(nb.ctl, nb.trig2)) return;
+  nbins = live_bins(nb, nbins);
   __shared__ int part[1024];
   int t = threadIdx.x;
   int per = (nbins + 1023) / 1024;
@@ -127,7 +144,7 @@ __global__ void k_nb_fill(NbrBuild nb)
 {
   if (!rebuild_due(nb.ctl, nb.trig2)) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb.n) return;
+  if (i >= live_atoms(nb)) return;
   int slot = atomicAdd(&nb.bin_cursor[nb.atom_bin[i]], 1);
   nb.bin_atoms[slot] = i;
 }
@@ -137,7 +154,7 @@ __global__ void k_nb_sortbins(NbrBuild nb, int nbins)
 {
   if (!rebuild_due(nb.ctl, nb.trig2)) return;
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
+  if (b >= live_bins(nb, nbins)) return;
   int lo = nb.bin_start[b], hi = nb.bin_start[b + 1];
   for (int p = lo + 1; p < hi; p++) {
     int v = nb.bin_atoms[p];
@@ -151,28 +168,30 @@ __global__ void k_nb_sortbins(NbrBuild nb, int nbins)
 }
 
 // One thread per atom walks the stencil in a fixed order.
-__global__ void k_nb_list(NbrBuild nb, Cell c, const double4 *pos, uint32_t *nbr, int *nn)
+__global__ void k_nb_list(NbrBuild nb, const double4 *pos, uint32_t *nbr, int *nn)
 {
   if (!rebuild_due(nb.ctl, nb.trig2)) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb.n) return;
+  if (i >= live_atoms(nb)) return;
+  const CellInfo &c = cell_of(nb, i);
   double4 xi = pos[i];
   int bc0 = nb.atom_bc[3 * i], bc1 = nb.atom_bc[3 * i + 1], bc2 = nb.atom_bc[3 * i + 2];
   int cnt = 0;
   uint32_t *out = nbr + (size_t) i * nb.cap;
-  for (int o0 = -nb.reach[0]; o0 <= nb.reach[0]; o0++) {
+  const int n0 = c.nb[0], n1 = c.nb[1], n2 = c.nb[2];
+  for (int o0 = -c.reach[0]; o0 <= c.reach[0]; o0++) {
     int q0 = bc0 + o0;
-    int s0 = (q0 >= 0) ? q0 / nb.nb[0] : -((-q0 + nb.nb[0] - 1) / nb.nb[0]);
-    int t0 = q0 - s0 * nb.nb[0];
-    for (int o1 = -nb.reach[1]; o1 <= nb.reach[1]; o1++) {
+    int s0 = (q0 >= 0) ? q0 / n0 : -((-q0 + n0 - 1) / n0);
+    int t0 = q0 - s0 * n0;
+    for (int o1 = -c.reach[1]; o1 <= c.reach[1]; o1++) {
       int q1 = bc1 + o1;
-      int s1 = (q1 >= 0) ? q1 / nb.nb[1] : -((-q1 + nb.nb[1] - 1) / nb.nb[1]);
-      int t1 = q1 - s1 * nb.nb[1];
-      for (int o2 = -nb.reach[2]; o2 <= nb.reach[2]; o2++) {
+      int s1 = (q1 >= 0) ? q1 / n1 : -((-q1 + n1 - 1) / n1);
+      int t1 = q1 - s1 * n1;
+      for (int o2 = -c.reach[2]; o2 <= c.reach[2]; o2++) {
         int q2 = bc2 + o2;
-        int s2 = (q2 >= 0) ? q2 / nb.nb[2] : -((-q2 + nb.nb[2] - 1) / nb.nb[2]);
-        int t2 = q2 - s2 * nb.nb[2];
-        int b = (t0 * nb.nb[1] + t1) * nb.nb[2] + t2;
+        int s2 = (q2 >= 0) ? q2 / n2 : -((-q2 + n2 - 1) / n2);
+        int t2 = q2 - s2 * n2;
+        int b = c.bin_base + (t0 * n1 + t1) * n2 + t2;
         int lo = nb.bin_start[b], hi = nb.bin_start[b + 1];
         for (int p = lo; p < hi; p++) {
           int j = nb.bin_atoms[p];
@@ -198,6 +217,7 @@ __global__ void k_nb_done(NbrBuild nb, int nbins)
 {
   // runs as one workgroup: clear counts for the next build, reset the control word
   bool due = rebuild_due(nb.ctl, nb.trig2);
+  nbins = live_bins(nb, nbins);
   __syncthreads();
   if (due) {
     for (int b = threadIdx.x; b <= nbins; b += blockDim.x) nb.bin_count[b] = 0;
@@ -210,19 +230,30 @@ __global__ void k_nb_done(NbrBuild nb, int nbins)
   }
 }
 
+__global__ void k_nb_force(unsigned long long *ctl)
+{
+  if (threadIdx.x == 0) ctl[1] = 1ull;
+}
+
 }  // namespace
 
-hipError_t nbr_step(const NbrBuild &nb, const Cell &cell, double4 *pos, uint32_t *nbr, int *nn,
-                    hipStream_t stream)
+hipError_t nbr_force_rebuild(const NbrBuild &nb, hipStream_t stream)
 {
-  int nbins = nb.nb[0] * nb.nb[1] * nb.nb[2];
+  k_nb_force<<<1, 64, 0, stream>>>(nb.ctl);
+  return hipGetLastError();
+}
+
+hipError_t nbr_step(const NbrBuild &nb, double4 *pos, uint32_t *nbr, int *nn, hipStream_t stream)
+{
+  if (nb.n <= 0) return hipSuccess;
+  const int nbins = nb.nbins;
   int blocks = (nb.n + 255) / 256;
-  k_nb_check<<<(nb.n + 1023) / 1024, 1024, 0, stream>>>(nb.n, pos, nb.x_build, nb.ctl);
-  k_nb_bin<<<blocks, 256, 0, stream>>>(nb, cell, pos);
+  k_nb_check<<<(nb.n + 1023) / 1024, 1024, 0, stream>>>(nb, pos, nb.x_build, nb.ctl);
+  k_nb_bin<<<blocks, 256, 0, stream>>>(nb, pos);
   k_nb_scan<<<1, 1024, 0, stream>>>(nb, nbins);
   k_nb_fill<<<blocks, 256, 0, stream>>>(nb);
   k_nb_sortbins<<<(nbins + 255) / 256, 256, 0, stream>>>(nb, nbins);
-  k_nb_list<<<(nb.n + 127) / 128, 128, 0, stream>>>(nb, cell, pos, nbr, nn);
+  k_nb_list<<<(nb.n + 127) / 128, 128, 0, stream>>>(nb, pos, nbr, nn);
   k_nb_done<<<1, 1024, 0, stream>>>(nb, nbins);
   return hipGetLastError();
 }

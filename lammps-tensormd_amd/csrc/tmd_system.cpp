@@ -20,7 +20,7 @@ struct tmd_system {
   tmd_ctx *ctx = nullptr;
   int n = 0;
   double skin = 0;
-  Cell cell{};
+  CellInfo cell{};
   NbrBuild nb{};
   int nbins = 0;
   Workspace w;
@@ -35,7 +35,7 @@ struct tmd_system {
   bool use_graph = true;
 };
 
-namespace {
+namespace tmd {
 
 bool invert3(const double *h, double *inv)
 {
@@ -73,6 +73,47 @@ void widths(const double *h, double w[3])
   w[2] = V / std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
 }
 
+// Bins and stencil reach of one periodic cell for the list cutoff `cut`;
+// bins are cells of the fractional lattice at least cut/2 wide.
+int setup_cell(const double *h, double cut, CellInfo &c, std::string &err)
+{
+  std::memset(&c, 0, sizeof(c));
+  std::memcpy(c.h, h, 9 * sizeof(double));
+  if (!invert3(c.h, c.inv)) {
+    err = "singular cell";
+    return TMD_ERR_ARG;
+  }
+  double w[3];
+  widths(c.h, w);
+  long nbins = 1;
+  for (int a = 0; a < 3; a++) {
+    c.nb[a] = std::max(1, (int) std::min(1024.0, std::floor(w[a] / (0.5 * cut))));
+    double alpha = cut * c.nb[a] / w[a];
+    c.reach[a] = (int) std::ceil(alpha + 1e-6);
+    // image shifts are encoded in [-3, 3]: |s| <= ceil(cut / w) + 1
+    if (std::ceil(cut / w[a]) + 1 > 3) {
+      err = "cell narrower than (rc+skin)/2; replicate it first";
+      return TMD_ERR_ARG;
+    }
+    nbins *= c.nb[a];
+  }
+  if (nbins > (1 << 26)) {
+    err = "too many bins";
+    return TMD_ERR_CAPACITY;
+  }
+  return TMD_OK;
+}
+
+double cell_volume(const double *h)
+{
+  return std::fabs(h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+                   h[2] * (h[3] * h[7] - h[4] * h[6]));
+}
+
+}  // namespace tmd
+
+namespace {
+
 int sys_fail(tmd_system *s, int code, const std::string &m)
 {
   set_err(s ? s->ctx : nullptr, m);
@@ -88,15 +129,26 @@ int alloc_list(tmd_system *s, int cap)
 }
 
 // One force evaluation at the current positions (neighbor list maintained).
+Chain sys_chain(tmd_system *s)
+{
+  Chain c;
+  c.geo.cells = (const CellInfo *) s->w.cells.p;
+  c.pos = (const double4 *) s->w.pos.p;
+  c.nbr = (const uint32_t *) s->w.nbr.p;
+  c.nn = (const int *) s->w.nn.p;
+  c.cap = s->nb.cap;
+  c.f = (double *) s->w.f.p;
+  c.fbytes = sizeof(double) * 3 * (size_t) s->n;
+  c.natoms = s->n;
+  return c;
+}
+
 int sys_forces(tmd_system *s)
 {
   hipStream_t st = s->ctx->stream;
-  hipError_t e = nbr_step(s->nb, s->cell, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p,
-                          (int *) s->w.nn.p, st);
+  hipError_t e = nbr_step(s->nb, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p, (int *) s->w.nn.p, st);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "neighbor build");
-  return run_chain(s->ctx, s->w, s->cell, (const double4 *) s->w.pos.p, (const uint32_t *) s->w.nbr.p,
-                   (const int *) s->w.nn.p, s->nb.cap, 1.0, (double *) s->w.f.p,
-                   sizeof(double) * 3 * (size_t) s->n, nullptr, 0, st);
+  return run_chain(s->ctx, s->w, sys_chain(s), st);
 }
 
 // overflow check after the queued work; grows the list and asks for a rebuild
@@ -144,37 +196,23 @@ int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type
   s->ctx = ctx;
   s->n = natoms;
   s->skin = skin;
-  std::memcpy(s->cell.h, cellv, 9 * sizeof(double));
-  if (!invert3(s->cell.h, s->cell.inv)) {
-    set_err(ctx, "tmd_system_create: singular cell");
-    return TMD_ERR_ARG;
-  }
   const Model &m = ctx->model;
   double cut = m.rc + skin;
-  double w[3];
-  widths(s->cell.h, w);
-  for (int a = 0; a < 3; a++) {
-    s->nb.nb[a] = std::max(1, (int) std::floor(w[a] / (0.5 * cut)));
-    double alpha = cut * s->nb.nb[a] / w[a];
-    s->nb.reach[a] = (int) std::ceil(alpha + 1e-6);
-    // image shifts are encoded in [-3, 3]: |s| <= ceil(cut / w) + 1
-    if (std::ceil(cut / w[a]) + 1 > 3) {
-      set_err(ctx, "tmd_system_create: cell narrower than (rc+skin)/2; replicate it first");
-      return TMD_ERR_ARG;
+  {
+    std::string err;
+    int rc = setup_cell(cellv, cut, s->cell, err);
+    if (rc != TMD_OK) {
+      set_err(ctx, "tmd_system_create: " + err);
+      return rc;
     }
   }
-  s->nbins = s->nb.nb[0] * s->nb.nb[1] * s->nb.nb[2];
-  if (s->nbins > (1 << 26)) {
-    set_err(ctx, "tmd_system_create: too many bins");
-    return TMD_ERR_CAPACITY;
-  }
+  s->nbins = s->cell.nb[0] * s->cell.nb[1] * s->cell.nb[2];
+  s->nb.nbins = s->nbins;
   s->nb.n = natoms;
   s->nb.cut2 = cut * cut;
   s->nb.trig2 = 0.25 * skin * skin;
   // expected neighbors within rc+skin, with margin for disorder
-  double V = std::fabs(s->cell.h[0] * (s->cell.h[4] * s->cell.h[8] - s->cell.h[5] * s->cell.h[7]) -
-                       s->cell.h[1] * (s->cell.h[3] * s->cell.h[8] - s->cell.h[5] * s->cell.h[6]) +
-                       s->cell.h[2] * (s->cell.h[3] * s->cell.h[7] - s->cell.h[4] * s->cell.h[6]));
+  double V = cell_volume(s->cell.h);
   double expect = 4.0 / 3.0 * M_PI * cut * cut * cut * natoms / V;
   int cap = (int) std::ceil(std::max(expect * 1.35, expect + 32.0));
   cap = (cap + 7) & ~7;
@@ -201,6 +239,8 @@ int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type
     if (er != hipSuccess) return hip_fail(ctx, er, what);
     return TMD_OK;
   };
+  if ((rc = ensure_atoms(ctx, s->w, natoms))) return rc;
+  if ((rc = chk(s->w.cells.ensure(sizeof(CellInfo)), "cell alloc"))) return rc;
   if ((rc = chk(s->w.pos.ensure(sizeof(double4) * natoms), "pos alloc"))) return rc;
   if ((rc = chk(s->w.nn.ensure(4 * (size_t) natoms), "nn alloc"))) return rc;
   if ((rc = chk(s->w.f.ensure(sizeof(double) * 3 * (size_t) natoms), "f alloc"))) return rc;
@@ -223,6 +263,8 @@ int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type
   s->nb.atom_bc = ai + 2 * (size_t) natoms;
   s->nb.x_build = (double *) s->xbuild.p;
   s->nb.ctl = (unsigned long long *) s->ctl.p;
+  s->nb.cells = (const CellInfo *) s->w.cells.p;
+  s->nb.atom_struct = nullptr;
   hipStream_t st = ctx->stream;
   std::vector<double> masses(kMaxElts, 1.0);
   for (int el = 0; el < m.nelt; el++) masses[el] = m.masses[el];
@@ -230,7 +272,8 @@ int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type
   if ((e = hipMemcpyAsync(s->w.pos.p, pos.data(), sizeof(double4) * natoms, hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(s->w.row2atom.p, s->row2atom.data(), 4 * s->row2atom.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(s->mass.p, masses.data(), sizeof(double) * kMaxElts, hipMemcpyHostToDevice, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(s->ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, st)) != hipSuccess)
+      (e = hipMemcpyAsync(s->ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(s->w.cells.p, &s->cell, sizeof(CellInfo), hipMemcpyHostToDevice, st)) != hipSuccess)
     return hip_fail(ctx, e, "system upload");
   if ((e = launch_zero(s->vel.p, sizeof(double) * 3 * (size_t) natoms, st)) != hipSuccess) return hip_fail(ctx, e, "zero v");
   if ((e = launch_zero(s->bins.p, s->bins.bytes, st)) != hipSuccess) return hip_fail(ctx, e, "zero bins");
@@ -306,7 +349,6 @@ int tmd_system_get(tmd_system *s, double *x, double *v, double *f, double *eatom
   hipSetDevice(s->ctx->device);
   hipStream_t st = s->ctx->stream;
   std::vector<double4> pos;
-  std::vector<float> E;
   double red[7];
   hipError_t e = hipSuccess;
   if (x) {
@@ -316,10 +358,8 @@ int tmd_system_get(tmd_system *s, double *x, double *v, double *f, double *eatom
   if (e == hipSuccess && v) e = hipMemcpyAsync(v, s->vel.p, sizeof(double) * 3 * (size_t) s->n, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess && f) e = hipMemcpyAsync(f, s->w.f.p, sizeof(double) * 3 * (size_t) s->n, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess && (energy || virial)) e = hipMemcpyAsync(red, s->w.red.p, sizeof(red), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess && eatom) {
-    E.resize(s->w.rl.nrows);
-    e = hipMemcpyAsync(E.data(), s->w.E.p, 4 * (size_t) s->w.rl.nrows, hipMemcpyDeviceToHost, st);
-  }
+  if (e == hipSuccess && eatom)
+    e = hipMemcpyAsync(eatom, s->w.ew.p, sizeof(double) * (size_t) s->n, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "tmd_system_get");
   if (x)
@@ -331,9 +371,6 @@ int tmd_system_get(tmd_system *s, double *x, double *v, double *f, double *eatom
   if (energy) *energy = red[0];
   if (virial)
     for (int q = 0; q < 6; q++) virial[q] = red[1 + q];
-  if (eatom)
-    for (int r = 0; r < s->w.rl.nrows; r++)
-      if (s->row2atom[r] >= 0) eatom[s->row2atom[r]] = (double) E[r];
   return TMD_OK;
 }
 
@@ -465,12 +502,10 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
                         (const double *) s->dtfm.p, dt, st);
     hipEventRecord(E[1], st);
     if (e == hipSuccess)
-      e = nbr_step(s->nb, s->cell, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p, (int *) s->w.nn.p, st);
+      e = nbr_step(s->nb, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p, (int *) s->w.nn.p, st);
     hipEventRecord(E[2], st);
     if (e == hipSuccess) {
-      rc = run_chain(s->ctx, s->w, s->cell, (const double4 *) s->w.pos.p, (const uint32_t *) s->w.nbr.p,
-                     (const int *) s->w.nn.p, s->nb.cap, 1.0, (double *) s->w.f.p,
-                     sizeof(double) * 3 * (size_t) s->n, nullptr, 0, st, &pe);
+      rc = run_chain(s->ctx, s->w, sys_chain(s), st, &pe);
       if (rc != TMD_OK) return rc;
     }
     if (e == hipSuccess)

@@ -21,16 +21,16 @@ __global__ void k_zero(uint4 *p, size_t n16, unsigned char *tail, size_t ntail)
 
 // Each block writes one partial; the final kernel adds partials in order,
 // so the result does not depend on scheduling.
-__global__ void k_reduce_ev_part(const float *E, const double *wrow, const int *row2atom, int nrows,
-                                 double *part)
+// sums of eatom[i], watom[i][0..5] over atoms [lo, hi) by one 256-thread
+// workgroup, in a fixed order; thread 0..6 of the result hold the 7 sums
+__device__ inline void block_sum7(const double *eatom, const double *watom, int lo, int hi,
+                                  int stride_lo, int stride, double (*sh)[256], double *res)
 {
-  __shared__ double sh[7][256];
   int t = threadIdx.x;
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int r = blockIdx.x * 256 + t; r < nrows; r += gridDim.x * 256) {
-    if (row2atom[r] < 0) continue;
-    v[0] += (double) E[r];
-    for (int q = 0; q < 6; q++) v[1 + q] += wrow[(size_t) r * 6 + q];
+  for (int i = lo + stride_lo + t; i < hi; i += stride) {
+    v[0] += eatom[i];
+    for (int q = 0; q < 6; q++) v[1 + q] += watom[(size_t) i * 6 + q];
   }
   for (int q = 0; q < 7; q++) sh[q][t] = v[q];
   __syncthreads();
@@ -39,18 +39,35 @@ __global__ void k_reduce_ev_part(const float *E, const double *wrow, const int *
       for (int q = 0; q < 7; q++) sh[q][t] += sh[q][t + o];
     __syncthreads();
   }
-  if (t < 7) part[(size_t) blockIdx.x * 7 + t] = sh[t][0];
+  if (t < 7) res[t] = sh[t][0];
+}
+
+__global__ void k_reduce_ev_part(const double *eatom, const double *watom, int n, double *part)
+{
+  __shared__ double sh[7][256];
+  block_sum7(eatom, watom, 0, n, blockIdx.x * 256, gridDim.x * 256, sh,
+             part + (size_t) blockIdx.x * 7);
 }
 
 // one wave per quantity; fixed shuffle tree (deterministic)
-__global__ void k_reduce_ev_final(const double *part, int nparts, double scale, double *out)
+__global__ void k_reduce_ev_final(const double *part, int nparts, double *out)
 {
   const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (q >= 7) return;
   double s = 0.0;
   for (int p = lane; p < nparts; p += 64) s += part[(size_t) p * 7 + q];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) out[q] = (q == 0) ? s * scale : s;
+  if (lane == 0) out[q] = s;
+}
+
+// one workgroup per structure
+__global__ void k_reduce_struct(const double *eatom, const double *watom, const int *off,
+                                int nstruct, double *out, const BatchDims *dyn)
+{
+  __shared__ double sh[7][256];
+  const int st = blockIdx.x;
+  if (st >= nstruct || (dyn && st >= dyn->nstruct)) return;
+  block_sum7(eatom, watom, off[st], off[st + 1], 0, 256, sh, out + (size_t) st * 7);
 }
 
 __global__ void k_vv_first(int n, double4 *pos, double *v, const double *f, const double *dtfm,
@@ -110,6 +127,13 @@ __global__ void k_ekin_final(const double *part, int nparts, double *out)
   if (threadIdx.x == 0) out[0] = 0.5 * 1.0364269e-4 * s;  // mvv2e, metal units
 }
 
+__global__ void k_zero_atoms(double *p, int per_atom, const BatchDims *dyn)
+{
+  size_t n = (size_t) per_atom * dyn->natoms;
+  size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 __global__ void k_copy_scalar(const double *src, double *dst)
 {
   if (threadIdx.x == 0) dst[0] = src[0];
@@ -154,12 +178,29 @@ hipError_t launch_zero(void *p, size_t bytes, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_ev(const float *E, const double *wrow, const int *row2atom, int nrows,
-                            double scale, double *out, hipStream_t s)
+hipError_t launch_reduce_ev(const double *eatom, const double *watom, int n, double *out,
+                            hipStream_t s)
 {
   double *part = out + 8;  // out holds 8 + kRedParts*7 doubles
-  k_reduce_ev_part<<<kRedParts, 256, 0, s>>>(E, wrow, row2atom, nrows, part);
-  k_reduce_ev_final<<<1, 448, 0, s>>>(part, kRedParts, scale, out);
+  k_reduce_ev_part<<<kRedParts, 256, 0, s>>>(eatom, watom, n, part);
+  k_reduce_ev_final<<<1, 448, 0, s>>>(part, kRedParts, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_struct(const double *eatom, const double *watom, const int *off,
+                                int nstruct, double *out, hipStream_t s, const BatchDims *dyn)
+{
+  if (nstruct <= 0) return hipSuccess;
+  k_reduce_struct<<<nstruct, 256, 0, s>>>(eatom, watom, off, nstruct, out, dyn);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_atoms(double *p, int per_atom, int cap_atoms, const BatchDims *dyn,
+                             hipStream_t s)
+{
+  size_t n = (size_t) per_atom * cap_atoms;
+  if (n == 0) return hipSuccess;
+  k_zero_atoms<<<(unsigned) ((n + 255) / 256), 256, 0, s>>>(p, per_atom, dyn);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libtensormd_amd.so")
 CSRC = os.path.join(os.path.dirname(_HERE), "csrc")
 
 TMD_OK = 0
+TMD_PENDING = 1
 ERRORS = {-1: "TMD_ERR_ARG", -2: "TMD_ERR_MODEL", -3: "TMD_ERR_DEVICE", -4: "TMD_ERR_NOMEM",
           -5: "TMD_ERR_CAPACITY", -6: "TMD_ERR_STATE"}
 PRECISION = {"fp32": 0, "bf16": 1, "fp8": 2}
@@ -34,6 +35,27 @@ class TmdError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("mlp_precision", ctypes.c_int), ("interp_dx", ctypes.c_double),
                 ("cmax_exact", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class ServeOptions(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("max_batch_atoms", ctypes.c_int),
+                ("max_batch_structs", ctypes.c_int), ("hbm_fraction", ctypes.c_double),
+                ("use_graph", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("max_queue", ctypes.c_int)]
+
+
+class Structure(ctypes.Structure):
+    _fields_ = [("natoms", ctypes.c_int), ("x", ctypes.POINTER(ctypes.c_double)),
+                ("type", ctypes.POINTER(ctypes.c_int)), ("cell", ctypes.c_double * 9),
+                ("f", ctypes.POINTER(ctypes.c_double)),
+                ("eatom", ctypes.POINTER(ctypes.c_double))]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("energy", ctypes.c_double),
+                ("virial", ctypes.c_double * 6), ("t_submit", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("batch_structs", ctypes.c_int), ("batch_atoms", ctypes.c_int)]
 
 
 _lib = None
@@ -80,6 +102,16 @@ def lib():
     L.tmd_bench_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
     L.tmd_system_neighbors.argtypes = [_vp, ctypes.c_long, _lp, _ip, _ip, _lp]
     L.tmd_system_descriptors.argtypes = [_vp, _fp, _fp]
+    L.tmd_serve_options_default.argtypes = [ctypes.POINTER(ServeOptions)]
+    L.tmd_serve_start.argtypes = [_vp, ctypes.POINTER(ServeOptions)]
+    L.tmd_serve_submit.argtypes = [_vp, ctypes.POINTER(Structure), ctypes.POINTER(ctypes.c_longlong)]
+    L.tmd_serve_poll.argtypes = [_vp, ctypes.c_longlong, ctypes.POINTER(Result)]
+    L.tmd_serve_wait.argtypes = [_vp, ctypes.c_longlong, ctypes.POINTER(Result)]
+    L.tmd_serve_stop.argtypes = [_vp]
+    L.tmd_serve_stats.argtypes = [_vp, _dp]
+    L.tmd_serve_replay.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Structure), _dp,
+                                   ctypes.POINTER(Result)]
+    L.tmd_clock.restype = ctypes.c_double
     _lib = L
     return L
 
@@ -195,6 +227,9 @@ class TensorMD:
     def system(self, x, types, cell, skin=1.0):
         return System(self, x, types, cell, skin)
 
+    def serve(self, **opts):
+        return Server(self, **opts)
+
 
 class System:
     """Device-resident periodic system (positions/velocities stay on the GPU)."""
@@ -283,3 +318,135 @@ class System:
         dG = np.zeros((self.n, width), dtype=np.float32)
         _check(lib().tmd_system_descriptors(self.h, _p(G, _fp), _p(dG, _fp)), self.model.h)
         return G, dG
+
+
+class Ticket:
+    """One submitted request; holds the output arrays until it completes."""
+
+    def __init__(self, server, tid, natoms, f, eatom, keep):
+        self.server = server
+        self.id = tid
+        self.natoms = natoms
+        self.f = f
+        self.eatom = eatom
+        self._keep = keep
+        self.result = None
+
+    def _fill(self, r):
+        self.result = dict(status=r.status, energy=r.energy, virial=np.array(r.virial[:]),
+                           f=self.f, eatom=self.eatom, t_submit=r.t_submit, t_start=r.t_start,
+                           t_done=r.t_done, batch_structs=r.batch_structs,
+                           batch_atoms=r.batch_atoms)
+        self._keep = None
+        return self.result
+
+
+class Server:
+    """Batched single-point requests on one model (continuous batching)."""
+
+    def __init__(self, model, slots=2, max_batch_atoms=65536, max_batch_structs=512,
+                 hbm_fraction=0.25, use_graph=True, autotune=True, max_queue=65536):
+        self.model = model
+        o = ServeOptions()
+        lib().tmd_serve_options_default(ctypes.byref(o))
+        o.slots, o.max_batch_atoms, o.max_batch_structs = slots, max_batch_atoms, max_batch_structs
+        o.hbm_fraction = hbm_fraction
+        o.use_graph = 1 if use_graph else 0
+        o.autotune = 1 if autotune else 0
+        o.max_queue = max_queue
+        _check(lib().tmd_serve_start(model.h, ctypes.byref(o)), model.h)
+        self.running = True
+        self._live = {}  # outstanding tickets keep their output arrays alive
+
+    def submit(self, x, types, cell, forces=True, eatom=False):
+        x = _f64(x).reshape(-1, 3)
+        types = _i32(types)
+        cell = _f64(cell).reshape(9)
+        n = x.shape[0]
+        f = np.zeros((n, 3)) if forces else None
+        ea = np.zeros(n) if eatom else None
+        st = Structure()
+        st.natoms = n
+        st.x = _p(x, _dp)
+        st.type = _p(types, _ip)
+        for q in range(9):
+            st.cell[q] = cell[q]
+        st.f = _p(f, _dp)
+        st.eatom = _p(ea, _dp)
+        tid = ctypes.c_longlong()
+        _check(lib().tmd_serve_submit(self.model.h, ctypes.byref(st), ctypes.byref(tid)),
+               self.model.h)
+        t = Ticket(self, tid.value, n, f, ea, (x, types))
+        self._live[t.id] = t
+        return t
+
+    def poll(self, ticket):
+        """Result dict if done, else None."""
+        if ticket.result is not None:
+            return ticket.result
+        r = Result()
+        rc = lib().tmd_serve_poll(self.model.h, ticket.id, ctypes.byref(r))
+        if rc == TMD_PENDING:
+            return None
+        _check(rc, self.model.h)
+        self._live.pop(ticket.id, None)
+        return ticket._fill(r)
+
+    def wait(self, ticket):
+        if ticket.result is not None:
+            return ticket.result
+        r = Result()
+        _check(lib().tmd_serve_wait(self.model.h, ticket.id, ctypes.byref(r)), self.model.h)
+        self._live.pop(ticket.id, None)
+        return ticket._fill(r)
+
+    def stats(self):
+        v = np.zeros(24)
+        _check(lib().tmd_serve_stats(self.model.h, _p(v, _dp)), self.model.h)
+        nb = int((v[16:24] > 0).sum())
+        return dict(batches=int(v[0]), structs=int(v[1]), atoms=int(v[2]), captures=int(v[3]),
+                    reruns=int(v[4]), gpu_ms=float(v[5]), slot_atoms=int(v[6]), slots=int(v[7]),
+                    row_tiles={int(v[16 + b]): int(v[8 + b]) for b in range(nb)})
+
+    def replay(self, structures, arrival, forces=True):
+        """Submit structures[k] = (x, types, cell) at arrival[k] s after the
+        call from a library loop (no Python in the timing), wait for all.
+        Returns (list of result dicts, force arrays)."""
+        n = len(structures)
+        arr = (Structure * max(n, 1))()
+        keep = []
+        fs = []
+        for k, (x, types, cell) in enumerate(structures):
+            x = _f64(x).reshape(-1, 3)
+            types = _i32(types)
+            cell = _f64(cell).reshape(9)
+            f = np.zeros((x.shape[0], 3)) if forces else None
+            keep.append((x, types))
+            fs.append(f)
+            arr[k].natoms = x.shape[0]
+            arr[k].x = _p(x, _dp)
+            arr[k].type = _p(types, _ip)
+            for q in range(9):
+                arr[k].cell[q] = cell[q]
+            arr[k].f = _p(f, _dp)
+            arr[k].eatom = None
+        at = _f64(arrival)
+        res = (Result * max(n, 1))()
+        _check(lib().tmd_serve_replay(self.model.h, n, arr, _p(at, _dp), res), self.model.h)
+        out = [dict(status=r.status, energy=r.energy, virial=np.array(r.virial[:]),
+                    t_submit=r.t_submit, t_start=r.t_start, t_done=r.t_done,
+                    batch_structs=r.batch_structs, batch_atoms=r.batch_atoms)
+               for r in res[:n]]
+        return out, fs
+
+    def stop(self):
+        """Finish every queued request, then stop the loop."""
+        if self.running:
+            self.running = False
+            _check(lib().tmd_serve_stop(self.model.h), self.model.h)
+            self._live.clear()
+
+
+def clock():
+    """The library's monotonic clock (seconds), the clock of request times."""
+    return lib().tmd_clock()

@@ -137,6 +137,31 @@ def crystal(kind, a, n, *, nelt=1, sigma=0.0, seed=0, equiatomic=True):
     return wrap(x, cell), types, cell
 
 
+def serving_requests(count, *, seed=0, nmin=16, nmax=4096, nelt=4, sigma=0.1,
+                     a_bcc=3.2, a_fcc=4.032):
+    """Single-point requests: periodic BCC or FCC supercells whose atom count
+    is drawn log-uniform in [nmin, nmax] and rounded to whole cells (at least
+    2 BCC cells per axis so that every cell is wider than rc/2), random
+    composition over `nelt` elements, Gaussian jitter `sigma`.
+
+    Returns a list of (x [N,3], types [N] 1-based, cell [3,3])."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        target = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        kind = "bcc" if rng.random() < 0.5 else "fcc"
+        basis, a, nlo = (2, a_bcc, 2) if kind == "bcc" else (4, a_fcc, 1)
+        cells = max(target / basis, nlo ** 3)
+        n1 = max(nlo, int(round(cells ** (1.0 / 3.0))))
+        n2 = max(nlo, int(round((cells / n1) ** 0.5)))
+        n3 = max(nlo, int(round(cells / (n1 * n2))))
+        x, cell = lattice(kind, a, (n1, n2, n3))
+        types = rng.integers(1, nelt + 1, size=x.shape[0]).astype(np.int32)
+        x = x + rng.normal(0.0, sigma, size=x.shape)
+        out.append((wrap(x, cell), types, cell))
+    return out
+
+
 def maxwell_boltzmann(masses, temperature, seed=0):
     """Velocities (A/ps) at exactly `temperature` (dof 3N-3), zero momentum."""
     rng = np.random.default_rng(seed)

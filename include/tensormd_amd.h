@@ -124,8 +124,9 @@ int tmd_system_get(tmd_system *sys, double *x, double *v, double *f, double *eat
 int tmd_md_run(tmd_system *sys, int nsteps, double dt, double *epot, double *ekin);
 /* Run nsteps MD steps with HIP events recorded on the library stream
  * between phases; ms[0..6] = mean ms per step spent in: neighbor
- * maintenance, descriptor kernel, MLP GEMMs, force kernel, FP64
- * reductions, integration, and the whole step. */
+ * maintenance, descriptor kernel, MLP kernel, force kernel, FP64
+ * reductions (with the force-buffer clears), integration, and the whole
+ * step. */
 int tmd_md_profile(tmd_system *sys, int nsteps, double dt, double *ms);
 /* Replay each MD step as one captured HIP graph (default 1) or launch its
  * kernels one by one (0); results are the same either way. */

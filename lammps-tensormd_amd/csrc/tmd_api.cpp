@@ -213,17 +213,18 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
     return hip_fail(ctx, e, "zero vatom");
   if (c.zero_ew && (e = launch_zero(eatom, (size_t) c.natoms * 7 * sizeof(double), s)) != hipSuccess)
     return hip_fail(ctx, e, "zero eatom");
+  if (pe) hipEventRecord(pe->ev[3], s);
   e = launch_force(ctx->geo, c.geo, c.pos, c.nbr, c.nn, c.cap, row2atom, w.rl.nrows,
                    (const float4 *) ctx->tab.p, (const float *) w.dEdG.p, (const double *) w.E.p,
                    c.scale, c.f, eatom, watom, c.vatom, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "force launch");
-  if (pe) hipEventRecord(pe->ev[3], s);
+  if (pe) hipEventRecord(pe->ev[4], s);
   if (c.struct_off)
     e = launch_reduce_struct(eatom, watom, c.struct_off, c.nstruct, c.struct_out, s, c.dyn);
   else
     e = launch_reduce_ev(eatom, watom, c.natoms, (double *) w.red.p, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "reduce launch");
-  if (pe) hipEventRecord(pe->ev[4], s);
+  if (pe) hipEventRecord(pe->ev[5], s);
   return TMD_OK;
 }
 

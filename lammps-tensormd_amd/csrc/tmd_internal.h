@@ -69,9 +69,10 @@ struct tmd_ctx {
 
 namespace tmd {
 // Optional events recorded between the phases of one evaluation:
-// [0] start, [1] after descriptor, [2] after MLP, [3] after force, [4] after reductions
+// [0] start, [1] after descriptor, [2] after MLP, [3] after the force-buffer
+// clears (just before the force kernel), [4] after force, [5] after reductions
 struct PhaseEvents {
-  hipEvent_t ev[5];
+  hipEvent_t ev[6];
 };
 void set_err(tmd_ctx *ctx, const std::string &m);
 int hip_fail(tmd_ctx *ctx, hipError_t e, const char *what);

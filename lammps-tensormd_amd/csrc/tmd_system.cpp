@@ -488,7 +488,7 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
   int rc = tmd_md_run(s, 1, dt, nullptr, nullptr);  // forces, dt constants, warm
   if (rc != TMD_OK) return rc;
   hipStream_t st = s->ctx->stream;
-  const int NE = 9;
+  const int NE = 10;
   std::vector<hipEvent_t> ev((size_t) nsteps * NE);
   for (auto &e : ev)
     if (hipEventCreate(&e) != hipSuccess) return sys_fail(s, TMD_ERR_DEVICE, "hipEventCreate");
@@ -496,7 +496,7 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
   for (int q = 0; q < nsteps && e == hipSuccess; q++) {
     hipEvent_t *E = &ev[(size_t) q * NE];
     PhaseEvents pe;
-    for (int k = 0; k < 5; k++) pe.ev[k] = E[3 + k];
+    for (int k = 0; k < 6; k++) pe.ev[k] = E[3 + k];
     hipEventRecord(E[0], st);
     e = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
                         (const double *) s->dtfm.p, dt, st);
@@ -511,11 +511,12 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
     if (e == hipSuccess)
       e = launch_vv_second(s->n, (const double4 *) s->w.pos.p, (double *) s->vel.p,
                            (const double *) s->w.f.p, (const double *) s->dtfm.p, st);
-    hipEventRecord(E[8], st);
+    hipEventRecord(E[9], st);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "profile");
-  // phases: integrate(0-1 + 7-8), neighbor(1-2), descriptor(3-4), mlp(4-5), force(5-6), reduce(6-7)
+  // phases: neighbor(1-2), descriptor(3-4), mlp(4-5), force kernel(6-7),
+  // reductions + force-buffer clears(7-8, 5-6), integrate(0-1, 8-9)
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   auto el = [&](hipEvent_t a, hipEvent_t b) {
     float t = 0.f;
@@ -527,10 +528,10 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
     acc[0] += el(E[1], E[2]);
     acc[1] += el(E[3], E[4]);
     acc[2] += el(E[4], E[5]);
-    acc[3] += el(E[5], E[6]);
-    acc[4] += el(E[6], E[7]);
-    acc[5] += el(E[0], E[1]) + el(E[7], E[8]);
-    acc[6] += el(E[0], E[8]);
+    acc[3] += el(E[6], E[7]);
+    acc[4] += el(E[7], E[8]) + el(E[5], E[6]);
+    acc[5] += el(E[0], E[1]) + el(E[8], E[9]);
+    acc[6] += el(E[0], E[9]);
   }
   for (auto &x : ev) hipEventDestroy(x);
   for (int k = 0; k < 7; k++) ms[k] = acc[k] / nsteps;

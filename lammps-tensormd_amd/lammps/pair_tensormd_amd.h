@@ -1,0 +1,58 @@
+/* -*- c++ -*- ----------------------------------------------------------
+   pair_style tensormd/amd: LAMMPS adaptor of libtensormd_amd (MI355X).
+
+   Same user interface and results as pair_style tensormd
+   (src/ML-TENSORMD/pair_tensormd.h:34-83 of the reference tree):
+     pair_style tensormd/amd
+     pair_coeff * * model.npz El1 El2 ... [interp dx] [etemp T] [cmax exact|default]
+   With `suffix amd` (or -sf amd) an input that says `pair_style tensormd`
+   selects this class.  Extra keyword: `precision fp32|bf16` (MLP GEMMs;
+   default fp32).
+------------------------------------------------------------------------- */
+
+#ifdef PAIR_CLASS
+// clang-format off
+PairStyle(tensormd/amd, PairTensorMDAMD);
+// clang-format on
+#else
+
+#ifndef LMP_PAIR_TENSORMD_AMD_H
+#define LMP_PAIR_TENSORMD_AMD_H
+
+#include "pair.h"
+
+#include <string>
+#include <vector>
+
+struct tmd_ctx;
+
+namespace LAMMPS_NS {
+
+class PairTensorMDAMD : public Pair {
+ public:
+  explicit PairTensorMDAMD(class LAMMPS *);
+  ~PairTensorMDAMD() override;
+  void compute(int, int) override;
+  void settings(int, char **) override;
+  void coeff(int, char **) override;
+  void init_style() override;
+  double init_one(int, int) override;
+  void *extract(const char *, int &) override;
+  double memory_usage() override;
+
+  double etemp;  // accepted for input compatibility; TDNP models are rejected
+
+ protected:
+  tmd_ctx *ctx;
+  double cutmax;
+  double **scale;  // fix adapt scaling (one factor for all types)
+  std::vector<std::string> kwargs;
+
+  void allocate();
+  void fail(int code, const char *where);
+};
+
+}    // namespace LAMMPS_NS
+
+#endif
+#endif

@@ -103,10 +103,43 @@ __device__ inline void kerley(float4 t, float q, float p, float dx, float &h, fl
   float c1 = t.z, c2 = t.w, s = t.y;
   float c3 = fabsf(c2 * p);
   float c4 = c3 + fabsf(c1 * q);
-  if (c4 > 0.0f) c3 = c3 / c4;
+  if (c4 > 0.0f) c3 = c3 * __builtin_amdgcn_rcpf(c4);  // 1-ulp reciprocal
   c4 = c2 + c3 * (c1 - c2);
   h = t.x + q * (s - p * c4);
   dh = s + (q - p) * c4 + dx * (c4 - c2) * (1.0f - c3);
+}
+
+// Cross-lane moves within a row of 16 lanes (DPP: no LDS traffic).
+// quad_perm [1,0,3,2] and [2,3,0,1] swap lanes 2i <-> 2i+1 and i <-> i^2;
+// row_ror:4 and row_ror:8 rotate a row of 16 lanes.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRor4 = 0x124, kDppRor8 = 0x128;
+
+template <int CTRL>
+__device__ inline float dpp_f(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ inline double dpp_d(double v)
+{
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_mov_dpp((int) (b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_mov_dpp((int) (b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long) hi << 32) | (unsigned int) lo);
+}
+
+// Sum over the 64 lanes, the result in every lane (fixed order: the same
+// lanes combine in the same order on every call).
+__device__ inline double wave_sum_d(double v)
+{
+  v += dpp_d<kDppXor1>(v);
+  v += dpp_d<kDppXor2>(v);
+  v += dpp_d<kDppRor4>(v);
+  v += dpp_d<kDppRor8>(v);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
 }
 
 __device__ inline unsigned lane_rank(unsigned long long m)
@@ -119,8 +152,6 @@ constexpr int kChunk = 32;  // kept neighbors staged at a time
 struct LDSMap {
   uint32_t *kept;  // [cap] compacted, slot-sorted neighbor entries
   float4 *rec;     // [kChunk] ux, uy, uz, r of the current chunk
-  int *tcell;      // [kChunk]
-  float2 *qp;      // [kChunk]
   float4 *drec;    // [kChunk] fp32 displacement, j as bits (force)
   float *M;        // [kChunk][kMS]  (force: aliased by S after the accumulation)
   float *H;        // [kChunk][HS]
@@ -142,8 +173,8 @@ __host__ __device__ inline size_t lds_layout(int nelt, int cap, bool force, size
   size_t preg = force ? (size_t) nelt * K * kND * 4 : 0;
   offs[0] = take((size_t) cap * 4);
   offs[1] = take(kChunk * 16);
-  offs[2] = take(kChunk * 4);
-  offs[3] = take(kChunk * 8);
+  offs[2] = take(0);
+  offs[3] = take(0);
   offs[4] = take(force ? kChunk * 16 : 16);
   offs[5] = take(mreg > preg ? mreg : preg);
   offs[6] = take((size_t) kChunk * HS * 4);
@@ -159,8 +190,6 @@ __device__ inline LDSMap carve(unsigned char *base, int nelt, int cap, bool forc
   LDSMap L;
   L.kept = (uint32_t *) (base + offs[0]);
   L.rec = (float4 *) (base + offs[1]);
-  L.tcell = (int *) (base + offs[2]);
-  L.qp = (float2 *) (base + offs[3]);
   L.drec = (float4 *) (base + offs[4]);
   L.M = (float *) (base + offs[5]);
   L.PS = L.M;
@@ -254,65 +283,78 @@ __device__ inline int compact(const ModelGeo &g, const Box &cell, const double4 
   return s;
 }
 
-// Phase 2 staging of kept neighbors [base, base+kChunk): geometry, monomials, H (and dH)
+// 1/sqrt(r2) in FP64: hardware estimate plus one Newton step (relative
+// error ~1e-15, far below the FP32 arithmetic that uses it)
+__device__ inline double rsqrt_fp64(double r2)
+{
+  double y = __builtin_amdgcn_rsq(r2);
+  return y * (1.5 - 0.5 * r2 * y * y);
+}
+
+// Phase 2 staging of kept neighbors [base, base+kChunk): two lanes per
+// neighbor (lane c and c + 32).  Both compute the neighbor's geometry; each
+// interpolates half of the filters from the table row of the neighbor's
+// cell, so the row loads use immediate offsets and no per-element index
+// arithmetic.  Writes u and r (rec), monomials (M), H (and dH, displacement
+// and j for the force kernel) to LDS.
 template <int K, bool FORCE>
 __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &xi,
                              const double4 *pos, const float4 *tab, int base, int nk, LDSMap &L,
                              bool want_M)
 {
+  static_assert(kChunk == 32, "two lanes per staged neighbor");
   const int lane = threadIdx.x & 63;
   constexpr int HS = Tiling<K>::HS;
-  int c = base + lane;
-  if (lane < kChunk && c < nk) {
-    uint32_t e = L.kept[c];
+  constexpr int KH = K / 2;
+  const int c = lane & 31, half = lane >> 5;
+  if (base + c < nk) {
+    uint32_t e = L.kept[base + c];
     int j = (int) (e & kNbrJMask);
     double d[3];
     displacement(xi, pos[j], e >> kNbrJBits, cell, d);
-    double r = sqrt(dist2(d));
-    double rinv = 1.0 / r;
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double rinv = rsqrt_fp64(r2);
+    double r = r2 * rinv;
     float ux = (float) (d[0] * rinv), uy = (float) (d[1] * rinv), uz = (float) (d[2] * rinv);
-    L.rec[lane] = make_float4(ux, uy, uz, (float) r);
-    int n = min(g.tab_n - 2, (int) (r * g.xdx));  // ration.cu:29-32, in FP64
-    double q = r - (double) n * g.dx;
-    L.tcell[lane] = n;
-    L.qp[lane] = make_float2((float) q, (float) (g.dx - q));
-    if (FORCE) L.drec[lane] = make_float4((float) d[0], (float) d[1], (float) d[2], __int_as_float(j));
-    if (want_M) {
-      float M[kND];
-      monomials20(ux, uy, uz, M);
-      float4 *Mr = reinterpret_cast<float4 *>(L.M + lane * kMS);
+    int n = min(g.tab_n - 2, (int) (r * g.xdx));  // ration.cu:29-32
+    double qd = r - (double) n * g.dx;
+    const float q = (float) qd, pq = (float) (g.dx - qd);
+    if (half == 0) {
+      L.rec[c] = make_float4(ux, uy, uz, (float) r);
+      if (FORCE) L.drec[c] = make_float4((float) d[0], (float) d[1], (float) d[2], __int_as_float(j));
+      if (want_M) {
+        float M[kND];
+        monomials20(ux, uy, uz, M);
+        float4 *Mr = reinterpret_cast<float4 *>(L.M + c * kMS);
 #pragma unroll
-      for (int q4 = 0; q4 < kND / 4; q4++)
-        Mr[q4] = make_float4(M[4 * q4], M[4 * q4 + 1], M[4 * q4 + 2], M[4 * q4 + 3]);
+        for (int q4 = 0; q4 < kND / 4; q4++)
+          Mr[q4] = make_float4(M[4 * q4], M[4 * q4 + 1], M[4 * q4 + 2], M[4 * q4 + 3]);
+      }
     }
-  }
-  __syncthreads();
-  const int cnt = min(kChunk, nk - base);
-  const int total = cnt * K;
-  constexpr int UNR = 4;  // table loads in flight per lane
-  for (int e0 = 0; e0 < total; e0 += 64 * UNR) {
-    float4 t[UNR];
-    float2 qp[UNR];
-    int cc[UNR], kk[UNR];
-    bool ok[UNR], valid[UNR];
+    float *Hc = L.H + c * HS + half * KH;
+    float *dHc = L.dH + c * HS + half * KH;
+    if (n > 0 && n < g.tab_n - 2) {  // the end cells are zero (ration.hpp:29)
+      const float4 *row = tab + (size_t) n * K + half * KH;
+      constexpr int B = KH < 4 ? KH : 4;  // table loads in flight
 #pragma unroll
-    for (int u = 0; u < UNR; u++) {
-      int e = e0 + lane + 64 * u;
-      valid[u] = e < total;
-      cc[u] = valid[u] ? e / K : 0;
-      kk[u] = e - cc[u] * K;
-      int n = L.tcell[cc[u]];
-      ok[u] = valid[u] && n > 0 && n < g.tab_n - 2;  // end cells are zero (ration.hpp:29)
-      t[u] = ok[u] ? tab[(size_t) n * K + kk[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
-      qp[u] = L.qp[cc[u]];
-    }
+      for (int k0 = 0; k0 < KH; k0 += B) {
+        float4 t[B];
 #pragma unroll
-    for (int u = 0; u < UNR; u++) {
-      if (!valid[u]) continue;
-      float h = 0.0f, dh = 0.0f;
-      if (ok[u]) kerley(t[u], qp[u].x, qp[u].y, g.dxf, h, dh);
-      L.H[cc[u] * HS + kk[u]] = h;
-      if (FORCE) L.dH[cc[u] * HS + kk[u]] = dh;
+        for (int u = 0; u < B; u++) t[u] = row[k0 + u];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+          float h, dh;
+          kerley(t[u], q, pq, g.dxf, h, dh);
+          Hc[k0 + u] = h;
+          if (FORCE) dHc[k0 + u] = dh;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < KH; k++) {
+        Hc[k] = 0.0f;
+        if (FORCE) dHc[k] = 0.0f;
+      }
     }
   }
   __syncthreads();
@@ -526,8 +568,8 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
     }
     // combine the two halves
 #pragma unroll
-    for (int q = 0; q < kND; q++) V[q] += __shfl_xor(V[q], 1);
-    f1 += __shfl_xor(f1, 1);
+    for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor1>(V[q]);
+    f1 += dpp_f<kDppXor1>(f1);
     if (live && half == 0) {
       // g_beta = sum_d V_d dM_d/du_beta
       float gx = V[1] + 2.f * x * V[4] + y * V[5] + z * V[6] + 3.f * x * x * V[10] +
@@ -562,12 +604,10 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
     __syncthreads();
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
+  for (int a = 0; a < 3; a++) fi[a] = wave_sum_d(fi[a]);
 #pragma unroll
-    for (int a = 0; a < 3; a++) fi[a] += __shfl_xor(fi[a], o);
-#pragma unroll
-    for (int q = 0; q < 6; q++) wi[q] += __shfl_xor(wi[q], o);
-  }
+  for (int q = 0; q < 6; q++) wi[q] = wave_sum_d(wi[q]);
+
   if (lane == 0) {
     atomicAdd(&f[3 * (size_t) i + 0], fi[0]);
     atomicAdd(&f[3 * (size_t) i + 1], fi[1]);

@@ -26,6 +26,7 @@ struct tmd_system {
   Workspace w;
   std::vector<int> row2atom;
   DeviceBuffer vel, dtfm, mass, bins, atominfo, xbuild, ctl, hist;
+  DeviceBuffer snap;  // x, v, f at the start of the current run segment
   bool have_forces = false;
   double dt_cached = -1;
   int nbuilds_seen = 0;
@@ -214,7 +215,9 @@ int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type
   // expected neighbors within rc+skin, with margin for disorder
   double V = cell_volume(s->cell.h);
   double expect = 4.0 / 3.0 * M_PI * cut * cut * cut * natoms / V;
-  int cap = (int) std::ceil(std::max(expect * 1.35, expect + 32.0));
+  // thermal motion widens the local-density spread: ~1.5x the mean plus a
+  // floor; an overflow during a run is still caught and the run rolled back
+  int cap = (int) std::ceil(expect * 1.5 + 16.0);
   cap = (cap + 7) & ~7;
 
   std::vector<double4> pos(natoms);
@@ -442,25 +445,65 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
     }
     s->graph_hist = want_hist;
   }
+  // Run in segments.  Each starts from a device snapshot of x, v and f; a
+  // neighbor-list overflow inside a segment (a truncated list) grows the
+  // list and reruns the segment from the snapshot, so no step ever uses a
+  // truncated list.
+  const int kSeg = 200;
+  const size_t xb = sizeof(double4) * (size_t) s->n, vb = 24 * (size_t) s->n;
+  if ((e = s->snap.ensure(xb + 2 * vb)) != hipSuccess) return hip_fail(s->ctx, e, "snapshot alloc");
+  unsigned char *snap = (unsigned char *) s->snap.p;
+  auto copy3 = [&](bool save) -> hipError_t {
+    hipError_t er;
+    void *a[3] = {s->w.pos.p, s->vel.p, s->w.f.p};
+    void *b[3] = {snap, snap + xb, snap + xb + vb};
+    size_t n[3] = {xb, vb, vb};
+    for (int q = 0; q < 3; q++)
+      if ((er = hipMemcpyAsync(save ? b[q] : a[q], save ? a[q] : b[q], n[q], hipMemcpyDeviceToDevice, st)) !=
+          hipSuccess)
+        return er;
+    return hipSuccess;
+  };
   for (int done = 0; done < nsteps;) {
-    int nrun = std::min(kHist, nsteps - done);
-    if (want_hist) {
-      unsigned long long zero = 0ull;
-      if ((e = hipMemcpyAsync(counter, &zero, sizeof(zero), hipMemcpyHostToDevice, st)) != hipSuccess)
-        return hip_fail(s->ctx, e, "history reset");
-    }
-    for (int q = 0; q < nrun; q++) {
-      if (s->use_graph) {
-        if ((e = hipGraphLaunch(s->step_graph, st)) != hipSuccess) return hip_fail(s->ctx, e, "graph launch");
-      } else {
-        int rc = enqueue_step();
-        if (rc != TMD_OK) return rc;
+    int nrun = std::min(kSeg, nsteps - done);
+    if ((e = copy3(true)) != hipSuccess) return hip_fail(s->ctx, e, "snapshot");
+    bool over = true;
+    for (int attempt = 0; attempt < 4 && over; attempt++) {
+      if (want_hist) {
+        unsigned long long zero = 0ull;
+        if ((e = hipMemcpyAsync(counter, &zero, sizeof(zero), hipMemcpyHostToDevice, st)) != hipSuccess)
+          return hip_fail(s->ctx, e, "history reset");
+      }
+      for (int q = 0; q < nrun; q++) {
+        if (s->use_graph) {
+          if ((e = hipGraphLaunch(s->step_graph, st)) != hipSuccess) return hip_fail(s->ctx, e, "graph launch");
+        } else {
+          int rc = enqueue_step();
+          if (rc != TMD_OK) return rc;
+        }
+      }
+      int rc = check_overflow(s, &over);  // grows the list and requests a rebuild
+      if (rc != TMD_OK) return rc;
+      if (over) {
+        if ((e = copy3(false)) != hipSuccess) return hip_fail(s->ctx, e, "snapshot restore");
+        if (s->use_graph && !s->step_graph) {  // the list grew: capture again
+          hipGraph_t graph;
+          if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return hip_fail(s->ctx, e, "graph capture");
+          rc = enqueue_step();
+          e = hipStreamEndCapture(st, &graph);
+          if (rc != TMD_OK) return rc;
+          if (e != hipSuccess) return hip_fail(s->ctx, e, "graph capture");
+          e = hipGraphInstantiate(&s->step_graph, graph, nullptr, nullptr, 0);
+          hipGraphDestroy(graph);
+          if (e != hipSuccess) {
+            s->step_graph = nullptr;
+            return hip_fail(s->ctx, e, "graph instantiate");
+          }
+        }
       }
     }
-    bool over = false;
-    int rc = check_overflow(s, &over);
-    if (rc != TMD_OK) return rc;
-    if (over) return sys_fail(s, TMD_ERR_CAPACITY, "neighbor list overflow during the run (list grown; rerun)");
+    if (over) return sys_fail(s, TMD_ERR_CAPACITY, "neighbor list overflow persists after growing the list");
     if (want_hist) {
       std::vector<double> h(2 * (size_t) nrun);
       if ((e = hipMemcpy(h.data(), hist, sizeof(double) * 2 * nrun, hipMemcpyDeviceToHost)) != hipSuccess)

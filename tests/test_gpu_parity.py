@@ -182,3 +182,33 @@ def test_graph_replay_matches_eager(synth_models):
     assert np.abs(out[0][0] - out[1][0]).max() / n < 1e-9
     assert np.abs(out[0][1] - out[1][1]).max() / n < 1e-9
     assert np.abs(out[0][2] - out[1][2]).max() < 1e-9
+
+
+def test_md_run_grows_list_and_rolls_back(synth_models):
+    """A converging flow raises the local density past the neighbor-list
+    capacity during the run: the run must grow the list, rerun the segment
+    from its snapshot and end with the forces of a fresh evaluation."""
+    say("case: list overflow during md_run, C1 2048 atoms, converging flow")
+    x, t, cell = W.crystal("fcc", 3.615, (8, 8, 8), sigma=0.05, seed=27)
+    L = np.diag(cell)
+    v = -1.5 * (x - 0.5 * L)  # A/ps toward the box center
+    m = TensorMD(synth_models["c1"], ["Cu"], "fp32", 0.001)
+    s = m.system(x, t, cell, skin=1.0)
+    s.set_velocities(v)
+    s.compute()
+    off0, _, _ = s.neighbors()
+    cap0 = int(np.diff(off0).max())
+    s.md_run(150, 0.001, history=False)
+    off, _, _ = s.neighbors()
+    out = s.get()
+    say("max list length before", cap0, "after", int(np.diff(off).max()), "builds", s.nbuilds)
+    # rc + skin = 7 A, 0.0847 atoms/A^3: the initial capacity is 1.5 x 121.7 + 16 -> 200
+    assert int(np.diff(off).max()) > 200
+    # a fresh system lists the neighbors in another order (FP32 sums differ
+    # in rounding only); a truncated list would be off by whole pair terms
+    fresh = m.system(out["x"], t, cell, skin=1.0)
+    ref = fresh.compute()
+    fmax = max(1.0, float(abs(ref["f"]).max()))
+    say("max |dF|", float(abs(out["f"] - ref["f"]).max()), "max |F|", fmax)
+    assert abs(out["f"] - ref["f"]).max() <= 1e-5 * fmax
+    assert abs(out["energy"] - ref["energy"]) / len(x) <= 1e-6 * max(1.0, abs(ref["energy"]) / len(x))

@@ -359,17 +359,327 @@ __global__ __launch_bounds__(256) void k_mlp(RowLayout rl_cap, MlpParams mp, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fast path, used when every hidden width is a multiple of 16 and every GEMM
+// inner dimension a multiple of 2 KS (all production models).  The weights
+// are the MFMA A operand and the activations the B operand, so the
+// accumulator of a lane holds 4 consecutive output columns of one row: the
+// epilogue reads and writes them with one vector access.  Operands are
+// double-buffered in registers (no copies) and loaded without bounds checks.
 namespace {
+
+template <int MODE>
+struct Frag;
+template <>
+struct Frag<0> {
+  using A = float;
+};
+template <>
+struct Frag<1> {
+  using A = bf16x8;
+};
+
+// A operand (weights).  BF16: row `orow + lane%16` of a row-major [rows][lda]
+// matrix, 8 consecutive inner indices; FP32: element (k, orow + lane%16) of
+// a [inner][lda] matrix.
+template <int MODE>
+__device__ inline typename Frag<MODE>::A ld_w(const void *W, int lda, int orow, int k)
+{
+  const int lane = threadIdx.x & 63;
+  if constexpr (MODE == 0)
+    return ((const float *) W)[(size_t) (k + (lane >> 4)) * lda + orow + (lane & 15)];
+  else
+    return *reinterpret_cast<const bf16x8 *>((const uint16_t *) W + (size_t) (orow + (lane & 15)) * lda +
+                                             k + (lane >> 4) * 8);
+}
+
+// B operand (activations of rows r0..r0+15) from a row-major LDS tile
+template <int MODE, typename T>
+__device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k)
+{
+  const int lane = threadIdx.x & 63;
+  if constexpr (MODE == 0)
+    return X[(r0 + (lane & 15)) * ld + k + (lane >> 4)];
+  else
+    return *reinterpret_cast<const bf16x8 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8);
+}
+
+// B operand from the FP32 descriptor rows in HBM (layer 0)
+template <int MODE>
+__device__ inline typename Frag<MODE>::A ld_g(const float *G, int gw, int grow, int k)
+{
+  const int lane = threadIdx.x & 63;
+  const float *p = G + (size_t) (grow + (lane & 15)) * gw + k;
+  if constexpr (MODE == 0) {
+    return p[lane >> 4];
+  } else {
+    const float4 *q = reinterpret_cast<const float4 *>(p + (lane >> 4) * 8);
+    float4 x0 = q[0], x1 = q[1];
+    f32x8 xv = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+    return __builtin_convertvector(xv, bf16x8);
+  }
+}
+
+template <int MODE>
+__device__ inline f32x4 mma(typename Frag<MODE>::A a, typename Frag<MODE>::A b, f32x4 c)
+{
+  if constexpr (MODE == 0)
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// 4 consecutive values of a row-major LDS tile (8- or 16-byte aligned)
+__device__ inline void ld4(const float *p, float v[4])
+{
+  float4 x = *reinterpret_cast<const float4 *>(p);
+  v[0] = x.x;
+  v[1] = x.y;
+  v[2] = x.z;
+  v[3] = x.w;
+}
+__device__ inline void ld4(const uint16_t *p, float v[4])
+{
+  uint2 x = *reinterpret_cast<const uint2 *>(p);
+  v[0] = __uint_as_float(x.x << 16);
+  v[1] = __uint_as_float(x.x & 0xffff0000u);
+  v[2] = __uint_as_float(x.y << 16);
+  v[3] = __uint_as_float(x.y & 0xffff0000u);
+}
+__device__ inline void st4(float *p, const float v[4])
+{
+  *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ inline void st4(uint16_t *p, const float v[4])
+{
+  uint2 x;
+  x.x = (uint32_t) f2bf(v[0]) | ((uint32_t) f2bf(v[1]) << 16);
+  x.y = (uint32_t) f2bf(v[2]) | ((uint32_t) f2bf(v[3]) << 16);
+  *reinterpret_cast<uint2 *>(p) = x;
+}
+
+// acc[j][r] = 16 output columns (tile t0 + 4 j) x 16 rows (r * 16 ..) of
+// W X^T, summed over the inner dimension K, for the wave's ntw tiles.
+template <int MODE, int R, bool XG, typename T>
+__device__ inline void gemm_t(const void *W, int lda, int t0, int ntw, const T *X, int ldx,
+                              const float *G, int gw, int grow0, int K, f32x4 (&acc)[4][R / 16])
+{
+  constexpr int KS = Traits<MODE>::KS, RT = R / 16;
+  using F = typename Frag<MODE>::A;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int r = 0; r < RT; r++) acc[j][r] = f32x4{0, 0, 0, 0};
+  F a0[4], a1[4], b0[RT], b1[RT];
+  auto load = [&](F(&a)[4], F(&b)[RT], int k) {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (j < ntw) a[j] = ld_w<MODE>(W, lda, (t0 + 4 * j) * 16, k);
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+      if constexpr (XG)
+        b[r] = ld_g<MODE>(G, gw, grow0 + r * 16, k);
+      else
+        b[r] = ld_x<MODE>(X, ldx, r * 16, k);
+    }
+  };
+  auto step = [&](F(&a)[4], F(&b)[RT]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (j < ntw)
+#pragma unroll
+        for (int r = 0; r < RT; r++) acc[j][r] = mma<MODE>(a[j], b[r], acc[j][r]);
+  };
+  load(a0, b0, 0);
+  for (int k = 0; k < K; k += 2 * KS) {
+    load(a1, b1, k + KS);
+    step(a0, b0);
+    if (k + 2 * KS < K) load(a0, b0, k + 2 * KS);
+    step(a1, b1);
+  }
+}
+
+}  // namespace
+
+template <int MODE, int ACT, int R>
+__global__ __launch_bounds__(256) void k_mlp_fast(RowLayout rl_cap, MlpParams mp, const float *Gd,
+                                                  double *E, float *dEdG, const BatchDims *dyn)
+{
+  const RowLayout rl = dyn ? dyn->rl : rl_cap;
+  using T = typename Traits<MODE>::T;
+  constexpr int RT = R / 16;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *sm = reinterpret_cast<T *>(smem_raw);
+  const int row0 = blockIdx.x * R;
+  if (row0 >= rl.nrows) return;
+  const int e = tile_elem(rl, row0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gw = mp.sizes[0];
+  const int L = mp.nl;
+  const int cq = 4 * (lane >> 4);  // first of the lane's 4 output columns in a tile
+  const int rq = lane & 15;        // the lane's row in a row tile
+
+  // ---------------- forward ----------------
+  for (int l = 0; l < L - 1; l++) {
+    const int K = mp.sizes[l], N = mp.sizes[l + 1];
+    const void *W = MODE == 0 ? (const void *) (mp.Wf32[l] + (size_t) e * mp.wstride[l])
+                              : (const void *) (mp.Wtb[l] + (size_t) e * mp.wstride[l]);
+    const int lda = MODE == 0 ? N : K;
+    const float *bias = mp.b[l] ? mp.b[l] + (size_t) e * mp.bstride[l] : nullptr;
+    T *H = sm + mp.offH[l];
+    T *D = sm + mp.offD[l];
+    const int ldo = mp.ld[l + 1];
+    const T *Hin = l > 0 ? sm + mp.offH[l - 1] : nullptr;
+    const int ldi = mp.ld[l];
+    const int NT = N / 16;
+    for (int tb = 0; tb < NT; tb += 16) {
+      const int t0 = tb + wave;
+      const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
+      if (ntw == 0) continue;
+      f32x4 acc[4][RT];
+      if (l == 0)
+        gemm_t<MODE, R, true, T>(W, lda, t0, ntw, nullptr, 0, Gd, gw, row0, K, acc);
+      else
+        gemm_t<MODE, R, false, T>(W, lda, t0, ntw, Hin, ldi, nullptr, gw, row0, K, acc);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (j >= ntw) continue;
+        const int c0 = (t0 + 4 * j) * 16 + cq;
+        float bv[4] = {0.f, 0.f, 0.f, 0.f};
+        if (bias) ld4(bias + c0, bv);
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+          const int row = r * 16 + rq;
+          float h[4], dz[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) h[i] = act_f<ACT, MODE == 1>(acc[j][r][i] + bv[i], dz[i]);
+          if (mp.res[l]) {
+            float x[4];
+            ld4(Hin + row * ldi + c0, x);
+#pragma unroll
+            for (int i = 0; i < 4; i++) h[i] += x[i];
+          }
+          st4(H + row * ldo + c0, h);
+          if (mp.storeD) st4(D + row * ldo + c0, dz);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---------------- output layer: E = H w + b (FP64 sum) ----------------
+  {
+    const int K = mp.sizes[L - 1];
+    const T *H = sm + mp.offH[L - 2];
+    const int ld = mp.ld[L - 1];
+    const float *w = mp.wout + (size_t) e * mp.wstride[L - 1];
+    const float bo = mp.bout ? mp.bout[(size_t) e * mp.bstride[L - 1]] : 0.0f;
+    for (int r = wave; r < R; r += 4) {
+      double s = 0.0;
+      for (int k = lane; k < K; k += 64) s += (double) (to_f<T>(H[r * ld + k]) * w[k]);
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      if (lane == 0) E[row0 + r] = s + (double) bo;
+    }
+  }
+  __syncthreads();
+  // seed: G_{L-2} = w_out * D_{L-2} (in the D slot); dE/dH = w_out for a resnet layer
+  {
+    const int N = mp.sizes[L - 1];
+    const float *w = mp.wout + (size_t) e * mp.wstride[L - 1];
+    T *D = sm + mp.offD[L - 2];
+    T *gH = sm + mp.offGH;
+    const int ld = mp.ld[L - 1];
+    for (int q = threadIdx.x; q < R * N; q += 256) {
+      int r = q / N, c = q - r * N;
+      float v = to_f<T>(D[r * ld + c]);
+      D[r * ld + c] = from_f<T>(w[c] * (mp.storeD ? v : 1.0f - v * v));
+      if (mp.res[L - 2]) gH[r * mp.ldGH + c] = from_f<T>(w[c]);
+    }
+  }
+  __syncthreads();
+
+  // ---------------- backward ----------------
+  for (int l = L - 2; l >= 0; l--) {
+    const int K = mp.sizes[l], N = mp.sizes[l + 1];
+    // Gin^T [K x R] = W_l [K x N] G_l^T [N x R]
+    const void *W = MODE == 0 ? (const void *) (mp.Wt32[l] + (size_t) e * mp.wstride[l])
+                              : (const void *) (mp.Wb16[l] + (size_t) e * mp.wstride[l]);
+    const int lda = MODE == 0 ? K : N;
+    const T *Gl = sm + mp.offD[l];
+    const int ldg = mp.ld[l + 1];
+    T *gH = sm + mp.offGH;
+    const int NT = K / 16;
+    for (int tb = 0; tb < NT; tb += 16) {
+      const int t0 = tb + wave;
+      const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
+      if (ntw == 0) continue;
+      f32x4 acc[4][RT];
+      gemm_t<MODE, R, false, T>(W, lda, t0, ntw, Gl, ldg, nullptr, gw, row0, N, acc);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (j >= ntw) continue;
+        const int c0 = (t0 + 4 * j) * 16 + cq;
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+          const int row = r * 16 + rq;
+          float v[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) v[i] = acc[j][r][i];
+          if (mp.res[l]) {  // dE/dH_l skip path
+            float x[4];
+            ld4(gH + row * mp.ldGH + c0, x);
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] += x[i];
+          }
+          if (l == 0) {
+            *reinterpret_cast<float4 *>(dEdG + (size_t) (row0 + row) * gw + c0) =
+                make_float4(v[0], v[1], v[2], v[3]);
+          } else {
+            T *D = sm + mp.offD[l - 1];
+            const int ld = mp.ld[l];
+            float d[4], o[4];
+            ld4(D + row * ld + c0, d);
+#pragma unroll
+            for (int i = 0; i < 4; i++) o[i] = v[i] * (mp.storeD ? d[i] : 1.0f - d[i] * d[i]);
+            st4(D + row * ld + c0, o);
+            if (mp.res[l - 1]) st4(gH + row * mp.ldGH + c0, v);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+namespace {
+// every hidden width a multiple of 16 and every GEMM inner size a multiple of 2 KS
+bool fast_ok(const MlpParams &mp, int mode)
+{
+  const int ks2 = mode == 0 ? 8 : 64;
+  for (int l = 0; l < mp.nl - 1; l++) {
+    if (mp.sizes[l + 1] % 16 || mp.sizes[l] % 16) return false;
+    if (mp.sizes[l] % ks2 || mp.sizes[l + 1] % ks2) return false;
+  }
+  return true;
+}
+
 template <int MODE, int R>
 hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, double *E, float *dEdG,
                     size_t lds, hipStream_t s, const BatchDims *dyn)
 {
   dim3 grid(rl.nrows / R);
+  const bool fast = fast_ok(mp, MODE);
 #define TMD_MLP_CASE(A)                                                                          \
   case A:                                                                                        \
-    hipFuncSetAttribute((const void *) k_mlp<MODE, A, R>,                                        \
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                  \
-    k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                                \
+    if (fast) {                                                                                  \
+      hipFuncSetAttribute((const void *) k_mlp_fast<MODE, A, R>,                                 \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                \
+      k_mlp_fast<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                    \
+    } else {                                                                                     \
+      hipFuncSetAttribute((const void *) k_mlp<MODE, A, R>,                                      \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                \
+      k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                         \
+    }                                                                                            \
     break;
   switch (mp.act) {
     TMD_MLP_CASE(0)

@@ -222,6 +222,9 @@ double tmd_clock(void);
  * the current device; *gbps = achieved read+write GB/s.  Re-measures the
  * HBM roofline and calibrates memory counters.  No reference counterpart. */
 int tmd_bench_copy(size_t bytes, int iters, double *gbps);
+/* Free and total device memory (bytes) of the current device, for
+ * footprint reports.  No reference counterpart. */
+int tmd_device_memory(size_t *free_bytes, size_t *total_bytes);
 
 #ifdef __cplusplus
 }

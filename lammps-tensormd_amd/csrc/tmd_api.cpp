@@ -97,8 +97,9 @@ int upload_model(tmd_ctx *ctx)
   if (L > kMaxLayers) return (set_err(ctx, "more than 8 MLP layers"), TMD_ERR_MODEL);
   size_t total = 0;
   struct Off {
-    size_t W, Wt, Wb, Wtb, b;
+    size_t W, Wt, Wb, Wtb, b, q8, qt8, swf, swb;
   };
+  const bool fp8 = ctx->opt.mlp_precision == TMD_MLP_FP8;
   std::vector<Off> offs(L);
   auto al = [](size_t x) { return (x + 255) & ~(size_t) 255; };
   for (int l = 0; l < L; l++) {
@@ -113,6 +114,16 @@ int upload_model(tmd_ctx *ctx)
     total = al(total + per * m.nelt * 2);
     offs[l].b = total;
     total = al(total + (size_t) n0.sizes[l + 1] * m.nelt * 4);
+    if (fp8) {  // e4m3 copies and their per-output-channel scales
+      offs[l].q8 = total;
+      total = al(total + per * m.nelt);
+      offs[l].qt8 = total;
+      total = al(total + per * m.nelt);
+      offs[l].swf = total;
+      total = al(total + (size_t) n0.sizes[l + 1] * m.nelt * 4);
+      offs[l].swb = total;
+      total = al(total + (size_t) n0.sizes[l] * m.nelt * 4);
+    }
   }
   e = ctx->params.ensure(total);
   if (e != hipSuccess) return hip_fail(ctx, e, "parameter alloc");
@@ -146,12 +157,35 @@ int upload_model(tmd_ctx *ctx)
         hasb = true;
         for (int j = 0; j < N; j++) b[el * N + j] = (float) net.B[l][j];
       }
+      if (fp8) {
+        // scale = max |w| / 440 over the output channel of each GEMM (forward:
+        // column n of W; backward: row k), so that w / scale fits e4m3
+        float *swf = (float *) &host[offs[l].swf] + (size_t) el * N;
+        float *swb = (float *) &host[offs[l].swb] + (size_t) el * K;
+        for (int j = 0; j < N; j++) swf[j] = 0.0f;
+        for (int i = 0; i < K; i++) {
+          float mx = 0.0f;
+          for (int j = 0; j < N; j++) {
+            float a = std::fabs((float) net.W[l][(size_t) i * N + j]);
+            mx = std::max(mx, a);
+            swf[j] = std::max(swf[j], a);
+          }
+          swb[i] = mx > 0.0f ? mx / 440.0f : 1.0f;
+        }
+        for (int j = 0; j < N; j++) swf[j] = swf[j] > 0.0f ? swf[j] / 440.0f : 1.0f;
+      }
     }
     mp.Wf32[l] = (const float *) (base + offs[l].W);
     mp.Wt32[l] = (const float *) (base + offs[l].Wt);
     mp.Wb16[l] = (const uint16_t *) (base + offs[l].Wb);
     mp.Wtb[l] = (const uint16_t *) (base + offs[l].Wtb);
     mp.b[l] = hasb ? (const float *) (base + offs[l].b) : nullptr;
+    if (fp8) {
+      mp.Wq8[l] = (const uint8_t *) (base + offs[l].q8);
+      mp.Wtq8[l] = (const uint8_t *) (base + offs[l].qt8);
+      mp.swf[l] = (const float *) (base + offs[l].swf);
+      mp.swb[l] = (const float *) (base + offs[l].swb);
+    }
     mp.wstride[l] = (long) per;
     mp.bstride[l] = N;
     mp.res[l] = n0.res_layer(l) ? 1 : 0;
@@ -160,10 +194,22 @@ int upload_model(tmd_ctx *ctx)
   mp.bout = mp.b[L - 1];
   mp.b[L - 1] = nullptr;
   // the fused kernel keeps one row tile's activations in LDS
-  if (mlp_lds_bytes(mp, ctx->opt.mlp_precision == TMD_MLP_FP32 ? 0 : 1, 16, nullptr) > 160 * 1024)
+  const int mode = ctx->opt.mlp_precision;
+  if (mlp_lds_bytes(mp, mode, 16, nullptr) > 160 * 1024)
     return (set_err(ctx, "MLP too wide for the fused kernel's LDS budget"), TMD_ERR_MODEL);
+  if (fp8 && !mlp_fast_ok(mp, mode))
+    return (set_err(ctx, "the FP8 MLP needs hidden and input widths that are multiples of 64"),
+            TMD_ERR_MODEL);
   e = hipMemcpy(ctx->params.p, host.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(ctx, e, "parameter upload");
+  if (fp8) {
+    for (int l = 0; l < L - 1; l++) {
+      e = launch_quant_fp8(mp.Wf32[l], mp.swf[l], mp.swb[l], (uint8_t *) mp.Wq8[l], (uint8_t *) mp.Wtq8[l],
+                           n0.sizes[l], n0.sizes[l + 1], m.nelt, ctx->stream);
+      if (e != hipSuccess) return hip_fail(ctx, e, "FP8 weight quantization");
+    }
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(ctx, e, "FP8 weights");
+  }
   return TMD_OK;
 }
 
@@ -190,7 +236,7 @@ int ensure_atoms(tmd_ctx *ctx, Workspace &w, int natoms)
 // descriptor -> MLP fwd/bwd -> force kernel -> FP64 reductions
 int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEvents *pe)
 {
-  const int mode = ctx->opt.mlp_precision == TMD_MLP_FP32 ? 0 : 1;
+  const int mode = ctx->opt.mlp_precision;  // 0 fp32, 1 bf16, 2 fp8
   hipError_t e;
   const int *row2atom = (const int *) w.row2atom.p;
   double *eatom = (double *) w.ew.p;
@@ -290,10 +336,7 @@ int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes
     set_err(nullptr, "tmd_create: unknown mlp_precision");
     return TMD_ERR_ARG;
   }
-  if (ctx->opt.mlp_precision == TMD_MLP_FP8) {
-    set_err(nullptr, "tmd_create: the FP8 MLP path is not available in this build");
-    return TMD_ERR_ARG;
-  }
+
   std::string err;
   if (!load_model(npz_path, ctx->opt.interp_dx, ctx->model, err)) {
     set_err(nullptr, err);
@@ -494,6 +537,14 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   *energy = red[0];
   if (virial)
     for (int q = 0; q < 6; q++) virial[q] = red[1 + q];
+  return TMD_OK;
+}
+
+int tmd_device_memory(size_t *free_bytes, size_t *total_bytes)
+{
+  if (!free_bytes || !total_bytes) return TMD_ERR_ARG;
+  hipError_t e = hipMemGetInfo(free_bytes, total_bytes);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "tmd_device_memory");
   return TMD_OK;
 }
 

@@ -75,6 +75,10 @@ struct MlpParams {
   const float *Wt32[kMaxLayers];
   const uint16_t *Wtb[kMaxLayers];
   const uint16_t *Wb16[kMaxLayers];
+  const uint8_t *Wq8[kMaxLayers];   // FP8 e4m3 W [K][N] / per-row scale (backward A operand)
+  const uint8_t *Wtq8[kMaxLayers];  // FP8 e4m3 W^T [N][K] / per-row scale (forward A operand)
+  const float *swf[kMaxLayers];     // forward per-output-channel scales [N]
+  const float *swb[kMaxLayers];     // backward per-output-channel scales [K]
   const float *b[kMaxLayers];    // hidden biases (or null)
   const float *wout;             // output layer weights [K_last]
   const float *bout;             // output bias (or null)
@@ -83,6 +87,7 @@ struct MlpParams {
   // LDS layout (elements of the mode's type)
   int offH[kMaxLayers], offD[kMaxLayers], ld[kMaxLayers + 1];
   int offGH, ldGH;
+  int offRS;   // FP8: per-row scale and inverse scale [2][R] (floats, byte offset)
   int storeD;  // D_l kept in LDS (otherwise recomputed from H_l: tanh)
 };
 
@@ -132,13 +137,21 @@ hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos,
                         hipStream_t stream);
 
 // ---- MLP ----------------------------------------------------------------------
-// mode: 0 fp32, 1 bf16.  E[row] (FP64 output sum) and dEdG[row][width]
-// (fp32).  rtile: rows per workgroup (16, 32 or 64; 0 = default).
+// mode: 0 fp32, 1 bf16, 2 fp8 (e4m3 operands: weights with per-output-channel
+// scales, activations and gradients with per-row scales; FP32 accumulation).
+// E[row] (FP64 output sum) and dEdG[row][width] (fp32).  rtile: rows per
+// workgroup (16, 32 or 64; 0 = default).
 // With dyn, rl gives the capacity (grid) and dyn->rl the live layout.
 hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, double *E,
                       float *dEdG, hipStream_t stream, int rtile = 0,
                       const BatchDims *dyn = nullptr);
 size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out);
+// the fast (aligned) MLP kernel applies: required by the FP8 mode
+bool mlp_fast_ok(const MlpParams &mp, int mode);
+// FP8 copies of one layer's weights for nelt elements: W [nelt][K][N] FP32,
+// per-output-channel scales swf [nelt][N] and swb [nelt][K]
+hipError_t launch_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Wq,
+                            uint8_t *Wtq, int K, int N, int nelt, hipStream_t stream);
 
 // ---- batches of structures (serving) ---------------------------------------------
 // Capacities of one batch slot (grids of its captured graph).

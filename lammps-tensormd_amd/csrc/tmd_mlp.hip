@@ -77,6 +77,12 @@ struct Traits<1> {
   static constexpr int KS = 32;
   static constexpr int CT = 4;
 };
+template <>  // FP8 operands, BF16 activations in LDS
+struct Traits<2> {
+  using T = uint16_t;
+  static constexpr int KS = 32;
+  static constexpr int CT = 4;
+};
 
 template <typename T>
 __device__ inline float to_f(T v);
@@ -378,6 +384,20 @@ template <>
 struct Frag<1> {
   using A = bf16x8;
 };
+template <>
+struct Frag<2> {
+  using A = long;  // 8 FP8 e4m3 values
+};
+
+// 8 floats -> 8 FP8 e4m3 (round to nearest even), first value in the low byte
+__device__ inline long pack_fp8(const float *v)
+{
+  int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+  lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+  int hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], 0, false);
+  hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
+  return (long) (((unsigned long long) (unsigned) hi << 32) | (unsigned) lo);
+}
 
 // A operand (weights).  BF16: row `orow + lane%16` of a row-major [rows][lda]
 // matrix, 8 consecutive inner indices; FP32: element (k, orow + lane%16) of
@@ -388,25 +408,39 @@ __device__ inline typename Frag<MODE>::A ld_w(const void *W, int lda, int orow, 
   const int lane = threadIdx.x & 63;
   if constexpr (MODE == 0)
     return ((const float *) W)[(size_t) (k + (lane >> 4)) * lda + orow + (lane & 15)];
-  else
+  else if constexpr (MODE == 1)
     return *reinterpret_cast<const bf16x8 *>((const uint16_t *) W + (size_t) (orow + (lane & 15)) * lda +
                                              k + (lane >> 4) * 8);
+  else
+    return *reinterpret_cast<const long *>((const uint8_t *) W + (size_t) (orow + (lane & 15)) * lda + k +
+                                           (lane >> 4) * 8);
 }
 
-// B operand (activations of rows r0..r0+15) from a row-major LDS tile
+// B operand (activations of rows r0..r0+15) from a row-major LDS tile;
+// FP8: scaled by the row's inverse scale ri
 template <int MODE, typename T>
-__device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k)
+__device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k, float ri)
 {
   const int lane = threadIdx.x & 63;
-  if constexpr (MODE == 0)
+  if constexpr (MODE == 0) {
     return X[(r0 + (lane & 15)) * ld + k + (lane >> 4)];
-  else
+  } else if constexpr (MODE == 1) {
     return *reinterpret_cast<const bf16x8 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8);
+  } else {
+    uint4 u = *reinterpret_cast<const uint4 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8);
+    float v[8] = {__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                  __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u),
+                  __uint_as_float(u.z << 16), __uint_as_float(u.z & 0xffff0000u),
+                  __uint_as_float(u.w << 16), __uint_as_float(u.w & 0xffff0000u)};
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] *= ri;
+    return pack_fp8(v);
+  }
 }
 
 // B operand from the FP32 descriptor rows in HBM (layer 0)
 template <int MODE>
-__device__ inline typename Frag<MODE>::A ld_g(const float *G, int gw, int grow, int k)
+__device__ inline typename Frag<MODE>::A ld_g(const float *G, int gw, int grow, int k, float ri)
 {
   const int lane = threadIdx.x & 63;
   const float *p = G + (size_t) (grow + (lane & 15)) * gw + k;
@@ -415,8 +449,14 @@ __device__ inline typename Frag<MODE>::A ld_g(const float *G, int gw, int grow, 
   } else {
     const float4 *q = reinterpret_cast<const float4 *>(p + (lane >> 4) * 8);
     float4 x0 = q[0], x1 = q[1];
-    f32x8 xv = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-    return __builtin_convertvector(xv, bf16x8);
+    if constexpr (MODE == 1) {
+      f32x8 xv = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+      return __builtin_convertvector(xv, bf16x8);
+    } else {
+      float v[8] = {x0.x * ri, x0.y * ri, x0.z * ri, x0.w * ri,
+                    x1.x * ri, x1.y * ri, x1.z * ri, x1.w * ri};
+      return pack_fp8(v);
+    }
   }
 }
 
@@ -425,8 +465,30 @@ __device__ inline f32x4 mma(typename Frag<MODE>::A a, typename Frag<MODE>::A b, 
 {
   if constexpr (MODE == 0)
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  else
+  else if constexpr (MODE == 1)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
+}
+
+// FP8: per-row scale of the next GEMM's B operand (the rows of X, or of the
+// descriptor rows G): rs[r] = max_c |x[r][c]| / 440 (1 for a zero row),
+// rs[R + r] = 1 / rs[r]; every value then fits e4m3 (|x| <= 448).
+template <bool XG>
+__device__ inline void row_scales(const uint16_t *X, int ld, const float *G, int gw, int grow0,
+                                  int width, int R, float *rs)
+{
+  const int tpr = 256 / R;  // threads per row: 16, 8 or 4 consecutive lanes
+  const int r = threadIdx.x / tpr, part = threadIdx.x - r * tpr;
+  float m = 0.0f;
+  for (int c = part; c < width; c += tpr)
+    m = fmaxf(m, fabsf(XG ? G[(size_t) (grow0 + r) * gw + c] : bf2f(X[r * ld + c])));
+  for (int o = 1; o < tpr; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if (part == 0) {
+    const float sc = m > 0.0f ? m * (1.0f / 440.0f) : 1.0f;
+    rs[r] = sc;
+    rs[R + r] = 1.0f / sc;
+  }
 }
 
 // 4 consecutive values of a row-major LDS tile (8- or 16-byte aligned)
@@ -462,7 +524,8 @@ __device__ inline void st4(uint16_t *p, const float v[4])
 // W X^T, summed over the inner dimension K, for the wave's ntw tiles.
 template <int MODE, int R, bool XG, typename T>
 __device__ inline void gemm_t(const void *W, int lda, int t0, int ntw, const T *X, int ldx,
-                              const float *G, int gw, int grow0, int K, f32x4 (&acc)[4][R / 16])
+                              const float *G, int gw, int grow0, int K, const float *rinv,
+                              f32x4 (&acc)[4][R / 16])
 {
   constexpr int KS = Traits<MODE>::KS, RT = R / 16;
   using F = typename Frag<MODE>::A;
@@ -470,6 +533,9 @@ __device__ inline void gemm_t(const void *W, int lda, int t0, int ntw, const T *
   for (int j = 0; j < 4; j++)
 #pragma unroll
     for (int r = 0; r < RT; r++) acc[j][r] = f32x4{0, 0, 0, 0};
+  float ri[RT];
+#pragma unroll
+  for (int r = 0; r < RT; r++) ri[r] = MODE == 2 ? rinv[r * 16 + (threadIdx.x & 15)] : 1.0f;
   F a0[4], a1[4], b0[RT], b1[RT];
   auto load = [&](F(&a)[4], F(&b)[RT], int k) {
 #pragma unroll
@@ -478,9 +544,9 @@ __device__ inline void gemm_t(const void *W, int lda, int t0, int ntw, const T *
 #pragma unroll
     for (int r = 0; r < RT; r++) {
       if constexpr (XG)
-        b[r] = ld_g<MODE>(G, gw, grow0 + r * 16, k);
+        b[r] = ld_g<MODE>(G, gw, grow0 + r * 16, k, ri[r]);
       else
-        b[r] = ld_x<MODE>(X, ldx, r * 16, k);
+        b[r] = ld_x<MODE>(X, ldx, r * 16, k, ri[r]);
     }
   };
   auto step = [&](F(&a)[4], F(&b)[RT]) {
@@ -518,12 +584,14 @@ __global__ __launch_bounds__(256) void k_mlp_fast(RowLayout rl_cap, MlpParams mp
   const int L = mp.nl;
   const int cq = 4 * (lane >> 4);  // first of the lane's 4 output columns in a tile
   const int rq = lane & 15;        // the lane's row in a row tile
+  float *rs = reinterpret_cast<float *>(smem_raw + mp.offRS);  // FP8 row scales [2][R]
 
   // ---------------- forward ----------------
   for (int l = 0; l < L - 1; l++) {
     const int K = mp.sizes[l], N = mp.sizes[l + 1];
-    const void *W = MODE == 0 ? (const void *) (mp.Wf32[l] + (size_t) e * mp.wstride[l])
-                              : (const void *) (mp.Wtb[l] + (size_t) e * mp.wstride[l]);
+    const void *W = MODE == 0   ? (const void *) (mp.Wf32[l] + (size_t) e * mp.wstride[l])
+                    : MODE == 1 ? (const void *) (mp.Wtb[l] + (size_t) e * mp.wstride[l])
+                                : (const void *) (mp.Wtq8[l] + (size_t) e * mp.wstride[l]);
     const int lda = MODE == 0 ? N : K;
     const float *bias = mp.b[l] ? mp.b[l] + (size_t) e * mp.bstride[l] : nullptr;
     T *H = sm + mp.offH[l];
@@ -532,27 +600,39 @@ __global__ __launch_bounds__(256) void k_mlp_fast(RowLayout rl_cap, MlpParams mp
     const T *Hin = l > 0 ? sm + mp.offH[l - 1] : nullptr;
     const int ldi = mp.ld[l];
     const int NT = N / 16;
+    if constexpr (MODE == 2) {
+      if (l == 0)
+        row_scales<true>(nullptr, 0, Gd, gw, row0, K, R, rs);
+      else
+        row_scales<false>((const uint16_t *) Hin, ldi, nullptr, 0, 0, K, R, rs);
+      __syncthreads();
+    }
+    const float *wsc = MODE == 2 ? mp.swf[l] + (size_t) e * N : nullptr;
     for (int tb = 0; tb < NT; tb += 16) {
       const int t0 = tb + wave;
       const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
       if (ntw == 0) continue;
       f32x4 acc[4][RT];
       if (l == 0)
-        gemm_t<MODE, R, true, T>(W, lda, t0, ntw, nullptr, 0, Gd, gw, row0, K, acc);
+        gemm_t<MODE, R, true, T>(W, lda, t0, ntw, nullptr, 0, Gd, gw, row0, K, rs + R, acc);
       else
-        gemm_t<MODE, R, false, T>(W, lda, t0, ntw, Hin, ldi, nullptr, gw, row0, K, acc);
+        gemm_t<MODE, R, false, T>(W, lda, t0, ntw, Hin, ldi, nullptr, gw, row0, K, rs + R, acc);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= ntw) continue;
         const int c0 = (t0 + 4 * j) * 16 + cq;
         float bv[4] = {0.f, 0.f, 0.f, 0.f};
         if (bias) ld4(bias + c0, bv);
+        float ws[4] = {1.f, 1.f, 1.f, 1.f};
+        if (MODE == 2) ld4(wsc + c0, ws);
 #pragma unroll
         for (int r = 0; r < RT; r++) {
           const int row = r * 16 + rq;
+          const float sr = MODE == 2 ? rs[row] : 1.0f;
           float h[4], dz[4];
 #pragma unroll
-          for (int i = 0; i < 4; i++) h[i] = act_f<ACT, MODE == 1>(acc[j][r][i] + bv[i], dz[i]);
+          for (int i = 0; i < 4; i++)
+            h[i] = act_f<ACT, MODE != 0>(acc[j][r][i] * (sr * ws[i]) + bv[i], dz[i]);
           if (mp.res[l]) {
             float x[4];
             ld4(Hin + row * ldi + c0, x);
@@ -602,29 +682,38 @@ __global__ __launch_bounds__(256) void k_mlp_fast(RowLayout rl_cap, MlpParams mp
   for (int l = L - 2; l >= 0; l--) {
     const int K = mp.sizes[l], N = mp.sizes[l + 1];
     // Gin^T [K x R] = W_l [K x N] G_l^T [N x R]
-    const void *W = MODE == 0 ? (const void *) (mp.Wt32[l] + (size_t) e * mp.wstride[l])
-                              : (const void *) (mp.Wb16[l] + (size_t) e * mp.wstride[l]);
+    const void *W = MODE == 0   ? (const void *) (mp.Wt32[l] + (size_t) e * mp.wstride[l])
+                    : MODE == 1 ? (const void *) (mp.Wb16[l] + (size_t) e * mp.wstride[l])
+                                : (const void *) (mp.Wq8[l] + (size_t) e * mp.wstride[l]);
     const int lda = MODE == 0 ? K : N;
     const T *Gl = sm + mp.offD[l];
     const int ldg = mp.ld[l + 1];
     T *gH = sm + mp.offGH;
     const int NT = K / 16;
+    if constexpr (MODE == 2) {
+      row_scales<false>((const uint16_t *) Gl, ldg, nullptr, 0, 0, N, R, rs);
+      __syncthreads();
+    }
+    const float *wsc = MODE == 2 ? mp.swb[l] + (size_t) e * K : nullptr;
     for (int tb = 0; tb < NT; tb += 16) {
       const int t0 = tb + wave;
       const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
       if (ntw == 0) continue;
       f32x4 acc[4][RT];
-      gemm_t<MODE, R, false, T>(W, lda, t0, ntw, Gl, ldg, nullptr, gw, row0, N, acc);
+      gemm_t<MODE, R, false, T>(W, lda, t0, ntw, Gl, ldg, nullptr, gw, row0, N, rs + R, acc);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= ntw) continue;
         const int c0 = (t0 + 4 * j) * 16 + cq;
+        float ws[4] = {1.f, 1.f, 1.f, 1.f};
+        if (MODE == 2) ld4(wsc + c0, ws);
 #pragma unroll
         for (int r = 0; r < RT; r++) {
           const int row = r * 16 + rq;
+          const float sr = MODE == 2 ? rs[row] : 1.0f;
           float v[4];
 #pragma unroll
-          for (int i = 0; i < 4; i++) v[i] = acc[j][r][i];
+          for (int i = 0; i < 4; i++) v[i] = acc[j][r][i] * (sr * ws[i]);
           if (mp.res[l]) {  // dE/dH_l skip path
             float x[4];
             ld4(gH + row * mp.ldGH + c0, x);
@@ -652,23 +741,30 @@ __global__ __launch_bounds__(256) void k_mlp_fast(RowLayout rl_cap, MlpParams mp
 }
 
 namespace {
-// every hidden width a multiple of 16 and every GEMM inner size a multiple of 2 KS
-bool fast_ok(const MlpParams &mp, int mode)
-{
-  const int ks2 = mode == 0 ? 8 : 64;
-  for (int l = 0; l < mp.nl - 1; l++) {
-    if (mp.sizes[l + 1] % 16 || mp.sizes[l] % 16) return false;
-    if (mp.sizes[l] % ks2 || mp.sizes[l + 1] % ks2) return false;
-  }
-  return true;
-}
-
 template <int MODE, int R>
 hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, double *E, float *dEdG,
                     size_t lds, hipStream_t s, const BatchDims *dyn)
 {
   dim3 grid(rl.nrows / R);
-  const bool fast = fast_ok(mp, MODE);
+  const bool fast = mlp_fast_ok(mp, MODE);
+  if constexpr (MODE == 2) {  // FP8 exists on the fast path only
+    if (!fast) return hipErrorInvalidConfiguration;
+#define TMD_MLP8_CASE(A)                                                                         \
+  case A:                                                                                        \
+    hipFuncSetAttribute((const void *) k_mlp_fast<2, A, R>,                                      \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                  \
+    k_mlp_fast<2, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                         \
+    break;
+    switch (mp.act) {
+      TMD_MLP8_CASE(0)
+      TMD_MLP8_CASE(1)
+      TMD_MLP8_CASE(2)
+      default:
+        TMD_MLP8_CASE(3)
+    }
+#undef TMD_MLP8_CASE
+    return hipGetLastError();
+  } else {
 #define TMD_MLP_CASE(A)                                                                          \
   case A:                                                                                        \
     if (fast) {                                                                                  \
@@ -681,23 +777,59 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
       k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                         \
     }                                                                                            \
     break;
-  switch (mp.act) {
-    TMD_MLP_CASE(0)
-    TMD_MLP_CASE(1)
-    TMD_MLP_CASE(2)
-    default:
-      TMD_MLP_CASE(3)
-  }
+    switch (mp.act) {
+      TMD_MLP_CASE(0)
+      TMD_MLP_CASE(1)
+      TMD_MLP_CASE(2)
+      default:
+        TMD_MLP_CASE(3)
+    }
 #undef TMD_MLP_CASE
-  return hipGetLastError();
+    return hipGetLastError();
+  }
+}
+
+__global__ void k_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Wq,
+                            uint8_t *Wtq, int K, int N, int nelt)
+{
+  const size_t per = (size_t) K * N;
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per * nelt) return;
+  const int el = (int) (i / per);
+  const size_t rem = i - (size_t) el * per;
+  const int k = (int) (rem / N), n = (int) (rem - (size_t) k * N);
+  const float w = W[i];
+  Wq[i] = (uint8_t) (__builtin_amdgcn_cvt_pk_fp8_f32(w / swb[(size_t) el * K + k], 0.0f, 0, false) & 0xff);
+  Wtq[(size_t) el * per + (size_t) n * K + k] =
+      (uint8_t) (__builtin_amdgcn_cvt_pk_fp8_f32(w / swf[(size_t) el * N + n], 0.0f, 0, false) & 0xff);
 }
 }  // namespace
+
+// every hidden width a multiple of 16 and every GEMM inner size a multiple of 2 KS
+bool mlp_fast_ok(const MlpParams &mp, int mode)
+{
+  const int ks2 = mode == 0 ? 8 : 64;
+  for (int l = 0; l < mp.nl - 1; l++) {
+    if (mp.sizes[l + 1] % 16 || mp.sizes[l] % 16) return false;
+    if (mp.sizes[l] % ks2 || mp.sizes[l + 1] % ks2) return false;
+  }
+  return true;
+}
+
+hipError_t launch_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Wq,
+                            uint8_t *Wtq, int K, int N, int nelt, hipStream_t stream)
+{
+  size_t total = (size_t) K * N * nelt;
+  if (total == 0) return hipSuccess;
+  k_quant_fp8<<<(unsigned) ((total + 255) / 256), 256, 0, stream>>>(W, swf, swb, Wq, Wtq, K, N, nelt);
+  return hipGetLastError();
+}
 
 // LDS layout for R rows: H_l and D_l per hidden layer (+ dE/dH for resnet).
 size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
 {
   MlpParams m = mp;
-  size_t esz = mode == 0 ? 4 : 2;
+  size_t esz = mode == 0 ? 4 : 2;  // BF16 activations in LDS for the BF16 and FP8 modes
   size_t off = 0;
   int maxw = 0;
   bool anyres = false;
@@ -723,8 +855,11 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
   m.ldGH = (mode == 0) ? maxw + 4 : maxw + 8;
   m.offGH = (int) off;
   if (anyres) off += (size_t) R * m.ldGH;
+  size_t bytes = (off * esz + 15) & ~(size_t) 15;
+  m.offRS = (int) bytes;
+  if (mode == 2) bytes += 2 * (size_t) R * sizeof(float);
   if (out) *out = m;
-  return off * esz;
+  return bytes;
 }
 
 hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, double *E,
@@ -746,6 +881,11 @@ hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const 
       if (R == 64) return launch_r<0, 64>(rl, m, G, E, dEdG, lds, s, dyn);
       if (R == 32) return launch_r<0, 32>(rl, m, G, E, dEdG, lds, s, dyn);
       return launch_r<0, 16>(rl, m, G, E, dEdG, lds, s, dyn);
+    }
+    if (mode == 2) {
+      if (R == 64) return launch_r<2, 64>(rl, m, G, E, dEdG, lds, s, dyn);
+      if (R == 32) return launch_r<2, 32>(rl, m, G, E, dEdG, lds, s, dyn);
+      return launch_r<2, 16>(rl, m, G, E, dEdG, lds, s, dyn);
     }
     if (R == 64) return launch_r<1, 64>(rl, m, G, E, dEdG, lds, s, dyn);
     if (R == 32) return launch_r<1, 32>(rl, m, G, E, dEdG, lds, s, dyn);

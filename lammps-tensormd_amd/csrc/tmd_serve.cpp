@@ -299,7 +299,7 @@ int enqueue_batch(Server *S, Slot &sl, const Bucket &b, int rtile)
 int tune_bucket(Server *S, Slot &sl, Bucket &b)
 {
   tmd_ctx *ctx = S->ctx;
-  const int mode = ctx->opt.mlp_precision == TMD_MLP_FP32 ? 0 : 1;
+  const int mode = ctx->opt.mlp_precision;  // 0 fp32, 1 bf16, 2 fp8
   RowLayout rl{};
   rl.nrows = b.caps.rows;
   const int nelt = ctx->model.nelt;

@@ -100,6 +100,7 @@ def lib():
     L.tmd_md_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp]
     L.tmd_system_set_graph.argtypes = [_vp, ctypes.c_int]
     L.tmd_bench_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
+    L.tmd_device_memory.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     L.tmd_system_neighbors.argtypes = [_vp, ctypes.c_long, _lp, _ip, _ip, _lp]
     L.tmd_system_descriptors.argtypes = [_vp, _fp, _fp]
     L.tmd_serve_options_default.argtypes = [ctypes.POINTER(ServeOptions)]
@@ -138,6 +139,13 @@ def bench_copy(nbytes, iters=20):
     g = ctypes.c_double()
     _check(lib().tmd_bench_copy(int(nbytes), int(iters), ctypes.byref(g)), None)
     return g.value
+
+
+def device_memory():
+    """(free, total) bytes of device memory on the current device."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    _check(lib().tmd_device_memory(ctypes.byref(f), ctypes.byref(t)), None)
+    return f.value, t.value
 
 
 class TensorMD:

@@ -164,7 +164,8 @@ class OracleModel:
                      want_desc=False):
         x = _f64(x).reshape(-1, 3)
         nall = x.shape[0]
-        elem = _i32(elem)
+        elem = _i32(elem).reshape(-1)
+        _same_len(elem, nall, "elem")
         ilist = _i32(ilist)
         numneigh = np.asarray(numneigh, dtype=np.int64)
         off = _i64(np.concatenate([[0], np.cumsum(numneigh[ilist])]))
@@ -195,7 +196,8 @@ class OracleModel:
         x = _f64(x).reshape(-1, 3)
         n = x.shape[0]
         cell = _f64(cell).reshape(3, 3)
-        elem = _i32(elem)
+        elem = _i32(elem).reshape(-1)
+        _same_len(elem, n, "elem")
         if nbr is None:
             nbr = neighbors(x, cell, self.rc)
         off, nj, ns = nbr
@@ -224,7 +226,9 @@ class OracleModel:
         x = _f64(x).reshape(-1, 3).copy()
         v = _f64(v).reshape(-1, 3).copy()
         n = x.shape[0]
-        elem = _i32(elem)
+        elem = _i32(elem).reshape(-1)
+        _same_len(elem, n, "elem")
+        _same_len(v, n, "v")
         cell = _f64(cell).reshape(3, 3)
         masses = _f64(self.masses)
         ep = np.zeros(nsteps + 1)
@@ -236,6 +240,12 @@ class OracleModel:
         if rc != 0:
             raise RuntimeError(lib().tmdo_last_error().decode())
         return dict(x=x, v=v, epot=ep, ekin=ek, forces=fh)
+
+
+def _same_len(a, n, name):
+    """The C functions read n entries: refuse shorter (or longer) arrays."""
+    if a.shape[0] != n:
+        raise ValueError(f"oracle: {name} has {a.shape[0]} rows, expected {n}")
 
 
 def neighbors(x, cell, cut, method="binned", cap_per_atom=400):

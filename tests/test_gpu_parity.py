@@ -212,3 +212,29 @@ def test_md_run_grows_list_and_rolls_back(synth_models):
     say("max |dF|", float(abs(out["f"] - ref["f"]).max()), "max |F|", fmax)
     assert abs(out["f"] - ref["f"]).max() <= 1e-5 * fmax
     assert abs(out["energy"] - ref["energy"]) / len(x) <= 1e-6 * max(1.0, abs(ref["energy"]) / len(x))
+
+
+def test_fp8_mlp_error(tmp_path):
+    """FP8 e4m3 MLP (weights with per-output-channel scales, activations and
+    gradients with per-row scales, FP32 accumulation) on the 256-256-128-1
+    architecture: errors against the FP64 oracle are reported beside the BF16
+    ones; the bar is loose (the experiment's result is the error itself)."""
+    say("case: fp8 vs bf16 MLP, 256-256-128-1 model, bcc 8x8x8 4 elements")
+    model = W.write_model(str(tmp_path / "c4.npz"), symbols=["Mo", "Nb", "Ta", "W"],
+                          masses=[95.95, 92.906, 180.95, 183.84], rmax=6.5, num_filters=24,
+                          layer_sizes=[256, 256, 128, 1], actfn="tanh", seed=44,
+                          radial_scale=0.2)
+    om = oracle.OracleModel(model, 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (8, 8, 8), nelt=4, sigma=0.08, seed=45)
+    ref = om.compute_periodic(x, t - 1, cell)
+    out = {}
+    for prec in ("bf16", "fp8"):
+        m = TensorMD(model, ["Mo", "Nb", "Ta", "W"], prec, 0.001)
+        got = m.system(x, t, cell, skin=1.0).compute()
+        e = _errs(got, ref, len(x))
+        e["rmsF"] = float(np.sqrt(((got["f"] - ref["f"]) ** 2).mean()))
+        out[prec] = e
+        say(prec, "errors", e, "max |F|", float(abs(ref["f"]).max()))
+    assert out["bf16"]["dF"] <= 5e-3
+    assert out["fp8"]["dF"] <= 0.1
+    assert out["fp8"]["dE"] <= 0.01

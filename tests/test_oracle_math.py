@@ -214,3 +214,13 @@ def test_list_mode_equals_periodic_mode(synth_models):
     assert np.allclose(f, per["f"], atol=1e-12)
     assert np.allclose(lst["virial"], per["virial"], atol=1e-9)
     assert np.allclose(lst["eatom"], per["eatom"], atol=1e-12)
+
+
+def test_wrapper_refuses_mismatched_arrays(c2):
+    """A per-atom array shorter than the positions is refused before the C
+    code reads it (a scalar element index used to be read past its end)."""
+    x, t, cell = W.crystal("bcc", 3.2, (3, 3, 3), nelt=4, seed=3)
+    with pytest.raises(ValueError):
+        c2.compute_periodic(x, 0, cell)
+    with pytest.raises(ValueError):
+        c2.nve(x, np.zeros_like(x), np.zeros(5, dtype=np.int32), cell, 0.001, 1)

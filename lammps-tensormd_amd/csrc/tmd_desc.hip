@@ -28,6 +28,8 @@ namespace {
 
 constexpr int kMS = 24;  // LDS stride of a monomial row (16-byte aligned)
 
+constexpr int kChunkRows = 32;  // neighbors staged at a time
+
 template <int K>
 struct Tiling {
   // each lane owns KT filters x DT monomials of every slot
@@ -35,7 +37,19 @@ struct Tiling {
   static constexpr int DT = (K == 24) ? 4 : 5;
   static constexpr int NDG = kND / DT;
   static constexpr int LANES = (K / KT) * NDG;  // 60 or 64
-  static constexpr int HS = K + 1;              // odd row stride: per-lane rows conflict-free
+  // H and dH: [half][kChunk][HS] with the filters of a half in a row of odd
+  // stride HS = K/2 + 1: the 64 lanes of a neighbor pass (32 rows x 2 halves)
+  // hit 64 distinct banks
+  static constexpr int KH = K / 2;
+  static constexpr int HS = KH + 1;
+  static constexpr int HALF = kChunkRows * HS;  // floats per half
+  // S rows of one slot, padded so that slots b and b + 2 start on different banks
+  static constexpr int SSTR = K * kND + 4;
+  // LDS offset of filter k of row c
+  __host__ __device__ static constexpr int hidx(int c, int k)
+  {
+    return (k >= KH ? HALF + k - KH : k) + c * HS;
+  }
 };
 
 // Lattice rows of the cell an atom belongs to (uniform per workgroup).
@@ -147,54 +161,43 @@ __device__ inline unsigned lane_rank(unsigned long long m)
   return __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
 }
 
-constexpr int kChunk = 32;  // kept neighbors staged at a time
+constexpr int kChunk = kChunkRows;
 
 struct LDSMap {
   uint32_t *kept;  // [cap] compacted, slot-sorted neighbor entries
-  float4 *rec;     // [kChunk] ux, uy, uz, r of the current chunk
-  float4 *drec;    // [kChunk] fp32 displacement, j as bits (force)
-  float *M;        // [kChunk][kMS]  (force: aliased by S after the accumulation)
-  float *H;        // [kChunk][HS]
-  float *dH;       // [kChunk][HS]   (force)
-  float *PS;       // [nslot][K][20] = M region (force)
+  float *M;        // [kChunk][kMS] monomials of the staged chunk
+  float *H;        // [2 halves][kChunk][HS] radial functions of the staged chunk
+  float *PS;       // force: [nslot][SSTR] P, then S (aliases M and H once P is built)
 };
 
 template <int K>
 __host__ __device__ inline size_t lds_layout(int nelt, int cap, bool force, size_t *offs)
 {
-  constexpr int HS = Tiling<K>::HS;
   size_t o = 0;
   auto take = [&](size_t bytes) {
     size_t r = o;
     o += (bytes + 15) & ~(size_t) 15;
     return r;
   };
-  size_t mreg = (size_t) kChunk * kMS * 4;
-  size_t preg = force ? (size_t) nelt * K * kND * 4 : 0;
+  const size_t mreg = (size_t) kChunk * kMS * 4;
+  const size_t hreg = (size_t) 2 * Tiling<K>::HALF * 4;
+  const size_t preg = force ? (size_t) nelt * Tiling<K>::SSTR * 4 : 0;
   offs[0] = take((size_t) cap * 4);
-  offs[1] = take(kChunk * 16);
-  offs[2] = take(0);
-  offs[3] = take(0);
-  offs[4] = take(force ? kChunk * 16 : 16);
-  offs[5] = take(mreg > preg ? mreg : preg);
-  offs[6] = take((size_t) kChunk * HS * 4);
-  offs[7] = take(force ? (size_t) kChunk * HS * 4 : 16);
+  offs[1] = take(mreg + hreg > preg ? mreg + hreg : preg);
+  offs[2] = offs[1] + mreg;
   return o;
 }
 
 template <int K>
 __device__ inline LDSMap carve(unsigned char *base, int nelt, int cap, bool force)
 {
-  size_t offs[8];
+  size_t offs[3];
   lds_layout<K>(nelt, cap, force, offs);
   LDSMap L;
   L.kept = (uint32_t *) (base + offs[0]);
-  L.rec = (float4 *) (base + offs[1]);
-  L.drec = (float4 *) (base + offs[4]);
-  L.M = (float *) (base + offs[5]);
+  L.M = (float *) (base + offs[1]);
+  L.H = (float *) (base + offs[2]);
   L.PS = L.M;
-  L.H = (float *) (base + offs[6]);
-  L.dH = (float *) (base + offs[7]);
   return L;
 }
 
@@ -291,16 +294,44 @@ __device__ inline double rsqrt_fp64(double r2)
   return y * (1.5 - 0.5 * r2 * y * y);
 }
 
-// Phase 2 staging of kept neighbors [base, base+kChunk): two lanes per
-// neighbor (lane c and c + 32).  Both compute the neighbor's geometry; each
-// interpolates half of the filters from the table row of the neighbor's
-// cell, so the row loads use immediate offsets and no per-element index
-// arithmetic.  Writes u and r (rec), monomials (M), H (and dH, displacement
-// and j for the force kernel) to LDS.
-template <int K, bool FORCE>
+// Geometry of one kept neighbor: FP64 displacement, unit vector and r, and
+// the radial-table cell n with the offsets q, dx - q (ration.cu:29-32).
+struct NbrGeo {
+  double d[3];
+  double r;
+  float ux, uy, uz;
+  float q, pq;
+  int n, j;
+};
+
+__device__ inline void nbr_geometry(const ModelGeo &g, const Box &cell, const double4 &xi,
+                                    const double4 *pos, uint32_t e, NbrGeo &o)
+{
+  o.j = (int) (e & kNbrJMask);
+  displacement(xi, pos[o.j], e >> kNbrJBits, cell, o.d);
+  double r2 = o.d[0] * o.d[0] + o.d[1] * o.d[1] + o.d[2] * o.d[2];
+  double rinv = rsqrt_fp64(r2);
+  o.r = r2 * rinv;
+  o.ux = (float) (o.d[0] * rinv);
+  o.uy = (float) (o.d[1] * rinv);
+  o.uz = (float) (o.d[2] * rinv);
+  o.n = min(g.tab_n - 2, (int) (o.r * g.xdx));
+  double qd = o.r - (double) o.n * g.dx;
+  o.q = (float) qd;
+  o.pq = (float) (g.dx - qd);
+}
+
+// the end cells of the table are zero (ration.hpp:29)
+__device__ inline bool table_cell_live(const ModelGeo &g, int n) { return n > 0 && n < g.tab_n - 2; }
+
+// Phase 2 staging of kept neighbors [base, base+kChunk) for the P
+// accumulation: two lanes per neighbor (lane c and c + 32).  Both compute the
+// neighbor's geometry; each interpolates half of the filters from the table
+// row of the neighbor's cell, so the row loads use immediate offsets and no
+// per-element index arithmetic.  Writes the monomials (M) and H to LDS.
+template <int K>
 __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &xi,
-                             const double4 *pos, const float4 *tab, int base, int nk, LDSMap &L,
-                             bool want_M)
+                             const double4 *pos, const float4 *tab, int base, int nk, LDSMap &L)
 {
   static_assert(kChunk == 32, "two lanes per staged neighbor");
   const int lane = threadIdx.x & 63;
@@ -308,33 +339,19 @@ __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &
   constexpr int KH = K / 2;
   const int c = lane & 31, half = lane >> 5;
   if (base + c < nk) {
-    uint32_t e = L.kept[base + c];
-    int j = (int) (e & kNbrJMask);
-    double d[3];
-    displacement(xi, pos[j], e >> kNbrJBits, cell, d);
-    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    double rinv = rsqrt_fp64(r2);
-    double r = r2 * rinv;
-    float ux = (float) (d[0] * rinv), uy = (float) (d[1] * rinv), uz = (float) (d[2] * rinv);
-    int n = min(g.tab_n - 2, (int) (r * g.xdx));  // ration.cu:29-32
-    double qd = r - (double) n * g.dx;
-    const float q = (float) qd, pq = (float) (g.dx - qd);
+    NbrGeo ng;
+    nbr_geometry(g, cell, xi, pos, L.kept[base + c], ng);
     if (half == 0) {
-      L.rec[c] = make_float4(ux, uy, uz, (float) r);
-      if (FORCE) L.drec[c] = make_float4((float) d[0], (float) d[1], (float) d[2], __int_as_float(j));
-      if (want_M) {
-        float M[kND];
-        monomials20(ux, uy, uz, M);
-        float4 *Mr = reinterpret_cast<float4 *>(L.M + c * kMS);
+      float M[kND];
+      monomials20(ng.ux, ng.uy, ng.uz, M);
+      float4 *Mr = reinterpret_cast<float4 *>(L.M + c * kMS);
 #pragma unroll
-        for (int q4 = 0; q4 < kND / 4; q4++)
-          Mr[q4] = make_float4(M[4 * q4], M[4 * q4 + 1], M[4 * q4 + 2], M[4 * q4 + 3]);
-      }
+      for (int q4 = 0; q4 < kND / 4; q4++)
+        Mr[q4] = make_float4(M[4 * q4], M[4 * q4 + 1], M[4 * q4 + 2], M[4 * q4 + 3]);
     }
-    float *Hc = L.H + c * HS + half * KH;
-    float *dHc = L.dH + c * HS + half * KH;
-    if (n > 0 && n < g.tab_n - 2) {  // the end cells are zero (ration.hpp:29)
-      const float4 *row = tab + (size_t) n * K + half * KH;
+    float *Hc = L.H + half * Tiling<K>::HALF + c * HS;
+    if (table_cell_live(g, ng.n)) {
+      const float4 *row = tab + (size_t) ng.n * K + half * KH;
       constexpr int B = KH < 4 ? KH : 4;  // table loads in flight
 #pragma unroll
       for (int k0 = 0; k0 < KH; k0 += B) {
@@ -344,17 +361,13 @@ __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &
 #pragma unroll
         for (int u = 0; u < B; u++) {
           float h, dh;
-          kerley(t[u], q, pq, g.dxf, h, dh);
+          kerley(t[u], ng.q, ng.pq, g.dxf, h, dh);
           Hc[k0 + u] = h;
-          if (FORCE) dHc[k0 + u] = dh;
         }
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < KH; k++) {
-        Hc[k] = 0.0f;
-        if (FORCE) dHc[k] = 0.0f;
-      }
+      for (int k = 0; k < KH; k++) Hc[k] = 0.0f;
     }
   }
   __syncthreads();
@@ -369,7 +382,7 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
   const int lane = threadIdx.x & 63;
   if (lane >= T::LANES) return;
   const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
-  const float *Hb = L.H + kq * T::KT;
+  const float *Hb = L.H + T::hidx(0, kq * T::KT);  // KT divides K/2: a lane's filters share a half
   const float *Mb = L.M + dq * T::DT;
 #pragma unroll
   for (int b = 0; b < NS; b++) {
@@ -390,7 +403,7 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
 }
 
 // Phases 1-2: P of every slot in the lane's register block.  Returns nk.
-template <int K, int NS, bool FORCE>
+template <int K, int NS>
 __device__ inline int build_P(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
                               const uint32_t *list, int nraw, const double4 *pos, const float4 *tab,
                               LDSMap &L, int *start, float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
@@ -405,7 +418,7 @@ __device__ inline int build_P(const ModelGeo &g, const Box &cell, const double4 
 #pragma unroll
       for (int q = 0; q < T::DT; q++) acc[b][a][q] = 0.0f;
   for (int base = 0; base < nk; base += kChunk) {
-    stage<K, FORCE>(g, cell, xi, pos, tab, base, nk, L, true);
+    stage<K>(g, cell, xi, pos, tab, base, nk, L);
     accumulate<K, NS>(g, L, start, base, acc);
     __syncthreads();
   }
@@ -435,8 +448,7 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
   const double4 xi = pos[i];
   const Box cell = load_box(geo, i);
   float acc[NS][T::KT][T::DT];
-  build_P<K, NS, false>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start,
-                        acc);
+  build_P<K, NS>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start, acc);
   // G[b][k][m] = sum_d T[m][d] P^2 (m >= 1): partial sums over the lane's d
   // block, then a sum over the NDG lanes that share k; G[b][k][0] = P0
   // (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
@@ -478,7 +490,6 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
                                               double *vatom)
 {
   using T = Tiling<K>;
-  constexpr int HS = T::HS;
   extern __shared__ __align__(16) unsigned char smem[];
   const int row = blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
@@ -490,9 +501,9 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
   const double4 xi = pos[i];
   const Box cell = load_box(geo, i);
   float acc[NS][T::KT][T::DT];
-  const int nk = build_P<K, NS, true>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos,
-                                      tab, L, start, acc);
-  // P -> LDS [b][k][d] (overwrites the monomial rows)
+  const int nk = build_P<K, NS>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L,
+                                start, acc);
+  // P -> LDS [b][k][d] (over the staging area; build_P ends with a barrier)
   if (lane < T::LANES) {
     const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
 #pragma unroll
@@ -502,7 +513,7 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
       for (int a = 0; a < T::KT; a++)
 #pragma unroll
         for (int q = 0; q < T::DT; q++)
-          L.PS[((size_t) b * K + kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
+          L.PS[(size_t) b * T::SSTR + (kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
     }
   }
   __syncthreads();
@@ -510,60 +521,70 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
   // S[b][k][d] = P * sum_m 2 T[m][d] g'_m, in place
   const float *grow = dEdG + (size_t) row * g.nelt * K * kNM;
   for (int e = lane; e < g.nelt * K; e += 64) {
-    float *P = L.PS + (size_t) e * kND;
+    float *P = L.PS + (size_t) (e / K) * T::SSTR + (e % K) * kND;
     float4 gv = *reinterpret_cast<const float4 *>(grow + (size_t) e * kNM);
     float G0 = P[0];
     float g0 = (fabsf(G0) < 1e-8f) ? 0.0f : gv.x / G0 * 0.5f;
     float gm[kNM] = {g0, gv.y, gv.z, gv.w};
 #pragma unroll
     for (int d = 0; d < kND; d++) {
-      float s = 0.0f;
+      float sd = 0.0f;
 #pragma unroll
-      for (int m = 0; m < kNM; m++) s += gm[m] * 2.0f * g.T[m][d];
-      P[d] = s * P[d];
+      for (int m = 0; m < kNM; m++) sd += gm[m] * 2.0f * g.T[m][d];
+      P[d] = sd * P[d];
     }
   }
   __syncthreads();
 
+  // Phase 3: two lanes per neighbor (adjacent lanes), each over half of the
+  // filters.  A lane recomputes its neighbor's geometry and interpolates
+  // H and dH/dr of its filters in registers, so the only LDS traffic is the
+  // broadcast read of S.
   double fi[3] = {0, 0, 0};
   double wi[6] = {0, 0, 0, 0, 0, 0};
-  // two lanes per neighbor, each summing half of the filters; chunks in
-  // reverse so the last staged chunk is reused
   const int c = lane >> 1, half = lane & 1;
   constexpr int KH = K / 2;
-  const int last = nk > 0 ? ((nk - 1) / kChunk) * kChunk : 0;
-  for (int base = last; base >= 0 && nk > 0; base -= kChunk) {
-    if (base != last) stage<K, true>(g, cell, xi, pos, tab, base, nk, L, false);
+  for (int base = 0; base < nk; base += kChunk) {
     const bool live = base + c < nk;
     int b = 0;
     while (b + 1 < g.nelt && base + c >= start[b + 1]) b++;
-    float4 rc = L.rec[c];
-    const float x = rc.x, y = rc.y, z = rc.z;
-    float M[kND];
-    monomials20(x, y, z, M);
+    NbrGeo ng;
+    ng.ux = ng.uy = ng.uz = 0.0f;
+    ng.r = 1.0;
     float V[kND];
 #pragma unroll
     for (int q = 0; q < kND; q++) V[q] = 0.0f;
     float f1 = 0.0f;
     if (live) {
-      const float *S = L.PS + ((size_t) b * K + half * KH) * kND;
-      const float *Hr = L.H + c * HS + half * KH;
-      const float *dHr = L.dH + c * HS + half * KH;
-#pragma unroll 4
-      for (int kk = 0; kk < KH; kk++) {
-        float h = Hr[kk], dh = dHr[kk];
-        const float4 *Sk = reinterpret_cast<const float4 *>(S + kk * kND);
-        float U = 0.0f;
+      nbr_geometry(g, cell, xi, pos, L.kept[base + c], ng);
+      if (table_cell_live(g, ng.n)) {
+        float M[kND];
+        monomials20(ng.ux, ng.uy, ng.uz, M);
+        const float4 *trow = tab + (size_t) ng.n * K + half * KH;
+        const float *S = L.PS + (size_t) b * T::SSTR + half * KH * kND;
+        constexpr int B = KH < 4 ? KH : 4;  // table loads in flight
+        for (int k0 = 0; k0 < KH; k0 += B) {
+          float4 t[B];
 #pragma unroll
-        for (int q4 = 0; q4 < kND / 4; q4++) {
-          float4 sv = Sk[q4];
-          U += sv.x * M[4 * q4] + sv.y * M[4 * q4 + 1] + sv.z * M[4 * q4 + 2] + sv.w * M[4 * q4 + 3];
-          V[4 * q4] += sv.x * h;
-          V[4 * q4 + 1] += sv.y * h;
-          V[4 * q4 + 2] += sv.z * h;
-          V[4 * q4 + 3] += sv.w * h;
+          for (int u = 0; u < B; u++) t[u] = trow[k0 + u];
+#pragma unroll
+          for (int u = 0; u < B; u++) {
+            float h, dh;
+            kerley(t[u], ng.q, ng.pq, g.dxf, h, dh);
+            const float4 *Sk = reinterpret_cast<const float4 *>(S + (k0 + u) * kND);
+            float U = 0.0f;
+#pragma unroll
+            for (int q4 = 0; q4 < kND / 4; q4++) {
+              float4 sv = Sk[q4];
+              U += sv.x * M[4 * q4] + sv.y * M[4 * q4 + 1] + sv.z * M[4 * q4 + 2] + sv.w * M[4 * q4 + 3];
+              V[4 * q4] += sv.x * h;
+              V[4 * q4 + 1] += sv.y * h;
+              V[4 * q4 + 2] += sv.z * h;
+              V[4 * q4 + 3] += sv.w * h;
+            }
+            f1 += U * dh;
+          }
         }
-        f1 += U * dh;
       }
     }
     // combine the two halves
@@ -571,6 +592,7 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
     for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor1>(V[q]);
     f1 += dpp_f<kDppXor1>(f1);
     if (live && half == 0) {
+      const float x = ng.ux, y = ng.uy, z = ng.uz;
       // g_beta = sum_d V_d dM_d/du_beta
       float gx = V[1] + 2.f * x * V[4] + y * V[5] + z * V[6] + 3.f * x * x * V[10] +
           2.f * x * y * V[11] + 2.f * x * z * V[12] + y * y * V[13] + y * z * V[14] + z * z * V[15];
@@ -579,20 +601,19 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
       float gz = V[3] + x * V[6] + y * V[8] + 2.f * z * V[9] + x * x * V[12] + x * y * V[14] +
           2.f * x * z * V[15] + y * y * V[17] + 2.f * y * z * V[18] + 3.f * z * z * V[19];
       float gu = gx * x + gy * y + gz * z;
-      float rinv = 1.0f / rc.w;
+      float rinv = (float) (1.0 / ng.r);
       // dE_i/dd = F1 + F2; dM/dd_a = sum_b dM/du_b (delta_ab - u_a u_b)/r
       double F0 = (double) (f1 * x + (gx - gu * x) * rinv) * scale;
       double F1v = (double) (f1 * y + (gy - gu * y) * rinv) * scale;
       double F2v = (double) (f1 * z + (gz - gu * z) * rinv) * scale;
-      float4 dr = L.drec[c];
-      int j = __float_as_int(dr.w);
+      const int j = ng.j;
       fi[0] += F0;
       fi[1] += F1v;
       fi[2] += F2v;
       atomicAdd(&f[3 * (size_t) j + 0], -F0);
       atomicAdd(&f[3 * (size_t) j + 1], -F1v);
       atomicAdd(&f[3 * (size_t) j + 2], -F2v);
-      double dx = dr.x, dy = dr.y, dz = dr.z;
+      const double dx = ng.d[0], dy = ng.d[1], dz = ng.d[2];
       double v[6] = {-dx * F0, -dy * F1v, -dz * F2v, -dx * F1v, -dx * F2v, -dy * F2v};
 #pragma unroll
       for (int q = 0; q < 6; q++) wi[q] += v[q];
@@ -601,7 +622,6 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
         for (int q = 0; q < 6; q++) atomicAdd(&vatom[6 * (size_t) j + q], 0.5 * v[q]);
       }
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int a = 0; a < 3; a++) fi[a] = wave_sum_d(fi[a]);

@@ -324,24 +324,34 @@ __device__ inline void nbr_geometry(const ModelGeo &g, const Box &cell, const do
 // the end cells of the table are zero (ration.hpp:29)
 __device__ inline bool table_cell_live(const ModelGeo &g, int n) { return n > 0 && n < g.tab_n - 2; }
 
-// Phase 2 staging of kept neighbors [base, base+kChunk) for the P
-// accumulation: two lanes per neighbor (lane c and c + 32).  Both compute the
-// neighbor's geometry; each interpolates half of the filters from the table
-// row of the neighbor's cell, so the row loads use immediate offsets and no
-// per-element index arithmetic.  Writes the monomials (M) and H to LDS.
+// Lanes per neighbor for a chunk when `rem` kept neighbors remain: 2 (a
+// chunk of 32 neighbors), 4 (16) or 8 (8).  A short last chunk spreads each
+// neighbor's filters over more lanes instead of leaving lanes idle.
+__device__ inline int lanes_per_nbr(int rem) { return rem > 16 ? 2 : (rem > 8 ? 4 : 8); }
+
+// table loads in flight per lane: divides K/2, K/4 and K/8
+template <int K>
+constexpr int kTabBatch = ((K / 8) % 4 == 0) ? 4 : K / 8;
+
+// Phase 2 staging of kept neighbors [base, base + 64/Lp) for the P
+// accumulation: Lp adjacent lanes per neighbor.  Each computes the
+// neighbor's geometry and interpolates K/Lp of the filters from the table
+// row of the neighbor's cell (immediate-offset loads, no per-element index
+// arithmetic).  Writes the monomials (M) and H to LDS.
 template <int K>
 __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &xi,
-                             const double4 *pos, const float4 *tab, int base, int nk, LDSMap &L)
+                             const double4 *pos, const float4 *tab, int base, int nk, int Lp,
+                             LDSMap &L)
 {
-  static_assert(kChunk == 32, "two lanes per staged neighbor");
+  static_assert(kChunk == 32, "at most 32 neighbors staged at a time");
+  using T = Tiling<K>;
   const int lane = threadIdx.x & 63;
-  constexpr int HS = Tiling<K>::HS;
-  constexpr int KH = K / 2;
-  const int c = lane & 31, half = lane >> 5;
+  const int c = lane / Lp, part = lane - c * Lp;
+  const int KL = K / Lp;
   if (base + c < nk) {
     NbrGeo ng;
     nbr_geometry(g, cell, xi, pos, L.kept[base + c], ng);
-    if (half == 0) {
+    if (part == 0) {
       float M[kND];
       monomials20(ng.ux, ng.uy, ng.uz, M);
       float4 *Mr = reinterpret_cast<float4 *>(L.M + c * kMS);
@@ -349,12 +359,12 @@ __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &
       for (int q4 = 0; q4 < kND / 4; q4++)
         Mr[q4] = make_float4(M[4 * q4], M[4 * q4 + 1], M[4 * q4 + 2], M[4 * q4 + 3]);
     }
-    float *Hc = L.H + half * Tiling<K>::HALF + c * HS;
+    // the lane's filters lie in one half of the H layout (K/Lp divides K/2)
+    float *Hc = L.H + T::hidx(c, part * KL);
+    constexpr int B = kTabBatch<K>;
     if (table_cell_live(g, ng.n)) {
-      const float4 *row = tab + (size_t) ng.n * K + half * KH;
-      constexpr int B = KH < 4 ? KH : 4;  // table loads in flight
-#pragma unroll
-      for (int k0 = 0; k0 < KH; k0 += B) {
+      const float4 *row = tab + (size_t) ng.n * K + part * KL;
+      for (int k0 = 0; k0 < KL; k0 += B) {
         float4 t[B];
 #pragma unroll
         for (int u = 0; u < B; u++) t[u] = row[k0 + u];
@@ -366,8 +376,7 @@ __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &
         }
       }
     } else {
-#pragma unroll
-      for (int k = 0; k < KH; k++) Hc[k] = 0.0f;
+      for (int k = 0; k < KL; k++) Hc[k] = 0.0f;
     }
   }
   __syncthreads();
@@ -417,10 +426,12 @@ __device__ inline int build_P(const ModelGeo &g, const Box &cell, const double4 
     for (int a = 0; a < T::KT; a++)
 #pragma unroll
       for (int q = 0; q < T::DT; q++) acc[b][a][q] = 0.0f;
-  for (int base = 0; base < nk; base += kChunk) {
-    stage<K>(g, cell, xi, pos, tab, base, nk, L);
+  for (int base = 0; base < nk;) {
+    const int Lp = lanes_per_nbr(nk - base);
+    stage<K>(g, cell, xi, pos, tab, base, nk, Lp, L);
     accumulate<K, NS>(g, L, start, base, acc);
     __syncthreads();
+    base += 64 / Lp;
   }
   return nk;
 }
@@ -481,6 +492,113 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
   }
 }
 
+// One chunk of the chain-rule pass: LP adjacent lanes per neighbor (neighbors
+// [base, base + 64/LP)), each over K/LP of the filters.  A lane recomputes
+// its neighbor's geometry and interpolates H and dH/dr of its filters in
+// registers, so the only LDS traffic is the broadcast read of S.  Adds the
+// neighbor forces to f_j (FP64 atomics) and to the atom's fi, wi.
+template <int K, int LP>
+__device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const double4 &xi,
+                                   const double4 *pos, const float4 *tab, const LDSMap &L,
+                                   const int *start, int base, int nk, double scale, double *f,
+                                   double *vatom, double fi[3], double wi[6])
+{
+  using T = Tiling<K>;
+  const int lane = threadIdx.x & 63;
+  const int c = lane / LP, sub = lane - c * LP;
+  constexpr int KL = K / LP;
+  {
+    const bool live = base + c < nk;
+    int b = 0;
+    while (b + 1 < g.nelt && base + c >= start[b + 1]) b++;
+    NbrGeo ng;
+    ng.ux = ng.uy = ng.uz = 0.0f;
+    ng.r = 1.0;
+    float V[kND];
+#pragma unroll
+    for (int q = 0; q < kND; q++) V[q] = 0.0f;
+    float f1 = 0.0f;
+    if (live) {
+      nbr_geometry(g, cell, xi, pos, L.kept[base + c], ng);
+      if (table_cell_live(g, ng.n)) {
+        float M[kND];
+        monomials20(ng.ux, ng.uy, ng.uz, M);
+        const float4 *trow = tab + (size_t) ng.n * K + sub * KL;
+        const float *S = L.PS + (size_t) b * T::SSTR + sub * KL * kND;
+        constexpr int B = kTabBatch<K>;
+#pragma unroll 1
+        for (int k0 = 0; k0 < KL; k0 += B) {
+          float4 t[B];
+#pragma unroll
+          for (int u = 0; u < B; u++) t[u] = trow[k0 + u];
+#pragma unroll
+          for (int u = 0; u < B; u++) {
+            float h, dh;
+            kerley(t[u], ng.q, ng.pq, g.dxf, h, dh);
+            const float4 *Sk = reinterpret_cast<const float4 *>(S + (k0 + u) * kND);
+            float U = 0.0f;
+#pragma unroll
+            for (int q4 = 0; q4 < kND / 4; q4++) {
+              float4 sv = Sk[q4];
+              U += sv.x * M[4 * q4] + sv.y * M[4 * q4 + 1] + sv.z * M[4 * q4 + 2] + sv.w * M[4 * q4 + 3];
+              V[4 * q4] += sv.x * h;
+              V[4 * q4 + 1] += sv.y * h;
+              V[4 * q4 + 2] += sv.z * h;
+              V[4 * q4 + 3] += sv.w * h;
+            }
+            f1 += U * dh;
+          }
+        }
+      }
+    }
+    // combine the Lp partial sums into the neighbor's first lane
+#pragma unroll
+    for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor1>(V[q]);
+    f1 += dpp_f<kDppXor1>(f1);
+    if constexpr (LP >= 4) {
+#pragma unroll
+      for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor2>(V[q]);
+      f1 += dpp_f<kDppXor2>(f1);
+    }
+    if constexpr (LP == 8) {
+#pragma unroll
+      for (int q = 0; q < kND; q++) V[q] += __shfl_xor(V[q], 4);
+      f1 += __shfl_xor(f1, 4);
+    }
+    if (live && sub == 0) {
+      const float x = ng.ux, y = ng.uy, z = ng.uz;
+      // g_beta = sum_d V_d dM_d/du_beta
+      float gx = V[1] + 2.f * x * V[4] + y * V[5] + z * V[6] + 3.f * x * x * V[10] +
+          2.f * x * y * V[11] + 2.f * x * z * V[12] + y * y * V[13] + y * z * V[14] + z * z * V[15];
+      float gy = V[2] + x * V[5] + 2.f * y * V[7] + z * V[8] + x * x * V[11] + 2.f * x * y * V[13] +
+          x * z * V[14] + 3.f * y * y * V[16] + 2.f * y * z * V[17] + z * z * V[18];
+      float gz = V[3] + x * V[6] + y * V[8] + 2.f * z * V[9] + x * x * V[12] + x * y * V[14] +
+          2.f * x * z * V[15] + y * y * V[17] + 2.f * y * z * V[18] + 3.f * z * z * V[19];
+      float gu = gx * x + gy * y + gz * z;
+      float rinv = (float) (1.0 / ng.r);
+      // dE_i/dd = F1 + F2; dM/dd_a = sum_b dM/du_b (delta_ab - u_a u_b)/r
+      double F0 = (double) (f1 * x + (gx - gu * x) * rinv) * scale;
+      double F1v = (double) (f1 * y + (gy - gu * y) * rinv) * scale;
+      double F2v = (double) (f1 * z + (gz - gu * z) * rinv) * scale;
+      const int j = ng.j;
+      fi[0] += F0;
+      fi[1] += F1v;
+      fi[2] += F2v;
+      atomicAdd(&f[3 * (size_t) j + 0], -F0);
+      atomicAdd(&f[3 * (size_t) j + 1], -F1v);
+      atomicAdd(&f[3 * (size_t) j + 2], -F2v);
+      const double dx = ng.d[0], dy = ng.d[1], dz = ng.d[2];
+      double v[6] = {-dx * F0, -dy * F1v, -dz * F2v, -dx * F1v, -dx * F2v, -dy * F2v};
+#pragma unroll
+      for (int q = 0; q < 6; q++) wi[q] += v[q];
+      if (vatom) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) atomicAdd(&vatom[6 * (size_t) j + q], 0.5 * v[q]);
+      }
+    }
+  }
+}
+
 template <int K, int NS>
 __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double4 *pos,
                                               const uint32_t *nbr, const int *nn, int cap,
@@ -536,92 +654,19 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
   }
   __syncthreads();
 
-  // Phase 3: two lanes per neighbor (adjacent lanes), each over half of the
-  // filters.  A lane recomputes its neighbor's geometry and interpolates
-  // H and dH/dr of its filters in registers, so the only LDS traffic is the
-  // broadcast read of S.
+  // Phase 3: the chain rule, chunk by chunk (2 lanes per neighbor, or 4 / 8
+  // for a short last chunk)
   double fi[3] = {0, 0, 0};
   double wi[6] = {0, 0, 0, 0, 0, 0};
-  const int c = lane >> 1, half = lane & 1;
-  constexpr int KH = K / 2;
-  for (int base = 0; base < nk; base += kChunk) {
-    const bool live = base + c < nk;
-    int b = 0;
-    while (b + 1 < g.nelt && base + c >= start[b + 1]) b++;
-    NbrGeo ng;
-    ng.ux = ng.uy = ng.uz = 0.0f;
-    ng.r = 1.0;
-    float V[kND];
-#pragma unroll
-    for (int q = 0; q < kND; q++) V[q] = 0.0f;
-    float f1 = 0.0f;
-    if (live) {
-      nbr_geometry(g, cell, xi, pos, L.kept[base + c], ng);
-      if (table_cell_live(g, ng.n)) {
-        float M[kND];
-        monomials20(ng.ux, ng.uy, ng.uz, M);
-        const float4 *trow = tab + (size_t) ng.n * K + half * KH;
-        const float *S = L.PS + (size_t) b * T::SSTR + half * KH * kND;
-        constexpr int B = KH < 4 ? KH : 4;  // table loads in flight
-        for (int k0 = 0; k0 < KH; k0 += B) {
-          float4 t[B];
-#pragma unroll
-          for (int u = 0; u < B; u++) t[u] = trow[k0 + u];
-#pragma unroll
-          for (int u = 0; u < B; u++) {
-            float h, dh;
-            kerley(t[u], ng.q, ng.pq, g.dxf, h, dh);
-            const float4 *Sk = reinterpret_cast<const float4 *>(S + (k0 + u) * kND);
-            float U = 0.0f;
-#pragma unroll
-            for (int q4 = 0; q4 < kND / 4; q4++) {
-              float4 sv = Sk[q4];
-              U += sv.x * M[4 * q4] + sv.y * M[4 * q4 + 1] + sv.z * M[4 * q4 + 2] + sv.w * M[4 * q4 + 3];
-              V[4 * q4] += sv.x * h;
-              V[4 * q4 + 1] += sv.y * h;
-              V[4 * q4 + 2] += sv.z * h;
-              V[4 * q4 + 3] += sv.w * h;
-            }
-            f1 += U * dh;
-          }
-        }
-      }
-    }
-    // combine the two halves
-#pragma unroll
-    for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor1>(V[q]);
-    f1 += dpp_f<kDppXor1>(f1);
-    if (live && half == 0) {
-      const float x = ng.ux, y = ng.uy, z = ng.uz;
-      // g_beta = sum_d V_d dM_d/du_beta
-      float gx = V[1] + 2.f * x * V[4] + y * V[5] + z * V[6] + 3.f * x * x * V[10] +
-          2.f * x * y * V[11] + 2.f * x * z * V[12] + y * y * V[13] + y * z * V[14] + z * z * V[15];
-      float gy = V[2] + x * V[5] + 2.f * y * V[7] + z * V[8] + x * x * V[11] + 2.f * x * y * V[13] +
-          x * z * V[14] + 3.f * y * y * V[16] + 2.f * y * z * V[17] + z * z * V[18];
-      float gz = V[3] + x * V[6] + y * V[8] + 2.f * z * V[9] + x * x * V[12] + x * y * V[14] +
-          2.f * x * z * V[15] + y * y * V[17] + 2.f * y * z * V[18] + 3.f * z * z * V[19];
-      float gu = gx * x + gy * y + gz * z;
-      float rinv = (float) (1.0 / ng.r);
-      // dE_i/dd = F1 + F2; dM/dd_a = sum_b dM/du_b (delta_ab - u_a u_b)/r
-      double F0 = (double) (f1 * x + (gx - gu * x) * rinv) * scale;
-      double F1v = (double) (f1 * y + (gy - gu * y) * rinv) * scale;
-      double F2v = (double) (f1 * z + (gz - gu * z) * rinv) * scale;
-      const int j = ng.j;
-      fi[0] += F0;
-      fi[1] += F1v;
-      fi[2] += F2v;
-      atomicAdd(&f[3 * (size_t) j + 0], -F0);
-      atomicAdd(&f[3 * (size_t) j + 1], -F1v);
-      atomicAdd(&f[3 * (size_t) j + 2], -F2v);
-      const double dx = ng.d[0], dy = ng.d[1], dz = ng.d[2];
-      double v[6] = {-dx * F0, -dy * F1v, -dz * F2v, -dx * F1v, -dx * F2v, -dy * F2v};
-#pragma unroll
-      for (int q = 0; q < 6; q++) wi[q] += v[q];
-      if (vatom) {
-#pragma unroll
-        for (int q = 0; q < 6; q++) atomicAdd(&vatom[6 * (size_t) j + q], 0.5 * v[q]);
-      }
-    }
+  for (int base = 0; base < nk;) {
+    const int Lp = lanes_per_nbr(nk - base);
+    if (Lp == 2)
+      force_chunk<K, 2>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
+    else if (Lp == 4)
+      force_chunk<K, 4>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
+    else
+      force_chunk<K, 8>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
+    base += 64 / Lp;
   }
 #pragma unroll
   for (int a = 0; a < 3; a++) fi[a] = wave_sum_d(fi[a]);

@@ -54,8 +54,10 @@ __device__ inline float act_f(float z, float &dz)
     dz = 1.0f - h * h;
     return h;
   } else if (ACT == 3) {
+    // squareplus (z + sqrt(z^2 + 4)) / 2; for z < 0 the same value as
+    // 2 / (sqrt(z^2 + 4) - z), which does not cancel in FP32
     float s = sqrtf(z * z + 4.0f);
-    float h = 0.5f * (z + s);
+    float h = z >= 0.0f ? 0.5f * (z + s) : 2.0f / (s - z);
     dz = h / s;
     return h;
   }

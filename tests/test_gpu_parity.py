@@ -85,11 +85,11 @@ def test_neighbor_set_bit_exact(synth_models):
         assert a == b, i
 
 
-@pytest.mark.parametrize("case", ["W_k32", "MoNi_m3"])
+@pytest.mark.parametrize("case", ["W_k32", "W_m3_k64", "MoNi_m3", "MoNi_squareplus_500k"])
 def test_golden_models(goldens, case):
     say("case: golden model", case)
     c = goldens["cases"][case]
-    if case == "W_k32":
+    if case.startswith("W_"):
         x, cell = W.lattice("bcc", c["a"], (3, 3, 3))
         types = np.ones(len(x), dtype=np.int32)
         symbols = ["W"]
@@ -117,9 +117,32 @@ def test_golden_models(goldens, case):
     assert e["dF"] <= F_TOL
     assert e["dE"] <= e_tol
     assert abs(got["energy"] / len(x) - c["pe"] / c["natoms"]) <= e_tol + 5e-7
-    if case == "MoNi_m3":
+    if c["model"] == goldens["forces_MoNi"]["model"]:  # the force dump's model
         fy = np.array([r[2:] for r in goldens["forces_MoNi"]["rows"]])
         assert abs(got["f"][:16] - fy).max() <= F_TOL
+
+
+def test_k128_model_vs_oracle():
+    """The widest radial basis shipped with the reference (k = 128, four
+    128-wide squareplus resnet layers), jittered so that forces are not zero
+    by symmetry: FP32 path vs the FP64 oracle."""
+    say("case: W_k128_3H_ver_2c, bcc 4x4x4 jittered")
+    path = model_path("W_k128_3H_ver_2c.npz")
+    x, t, cell = W.crystal("bcc", 3.189, (4, 4, 4), sigma=0.06, seed=46)
+    om = oracle.OracleModel(path, 0.001)
+    ref = om.compute_periodic(x, t - 1, cell)
+    m = TensorMD(path, ["W"], "fp32", 0.001)
+    got = m.system(x, t, cell, skin=1.0).compute()
+    e = _errs(got, ref, len(x))
+    fmax = float(abs(ref["f"]).max())
+    say("errors", e, "E/atom", ref["energy"] / len(x), "max |F|", fmax)
+    # this trained model is stiff (|F| up to ~3.4 eV/A at this jitter): the
+    # force bar is 1e-5 per eV/A of the largest force, i.e. the absolute
+    # 1e-5 eV/A bar scaled to the force magnitude (FP32 sums of ~60
+    # neighbors x 128 filters); the energy bar is relative as for the goldens
+    e_tol = max(E_TOL, 4e-7 * abs(ref["energy"]) / len(x))
+    assert e["dF"] <= F_TOL * max(1.0, fmax)
+    assert e["dE"] <= e_tol
 
 
 def test_bf16_mlp_error(synth_models):

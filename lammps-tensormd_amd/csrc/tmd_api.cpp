@@ -75,10 +75,21 @@ int upload_model(tmd_ctx *ctx)
   g.nelt = m.nelt;
   g.k = m.k;
   g.tab_n = m.tab_n;
-  for (int a = 0; a < m.nelt; a++)
-    for (int b = 0; b < m.nelt; b++) g.eltij[a][b] = m.eltij[a][b];
+  for (int a = 0; a < m.nelt; a++) {
+    g.slots[a] = 0;
+    for (int b = 0; b < m.nelt; b++) g.slots[a] |= (uint32_t) m.eltij[a][b] << (4 * b);
+  }
+  // the kernels hold T as the fixed multiplicity table plus the T-symmetric
+  // extras; check that the model's T is exactly that
+  g.tsym = m.T_sym ? 1 : 0;
+  if (m.nm != kNM || m.nd != kND) return (set_err(ctx, "max_moment 3 models only"), TMD_ERR_MODEL);
   for (int mm = 0; mm < kNM; mm++)
-    for (int d = 0; d < kND; d++) g.T[mm][d] = (float) m.T[mm * m.nd + d];
+    for (int d = 0; d < kND; d++) {
+      double t = kDeg[d] == mm ? (double) kMult[d] : 0.0;
+      if (g.tsym && mm == 2 && d == 0) t = -1.0 / 3.0;
+      if (g.tsym && mm == 3 && d >= 1 && d <= 3) t = -3.0 / 5.0;
+      if (m.T[mm * m.nd + d] != t) return (set_err(ctx, "unexpected descriptor T table"), TMD_ERR_MODEL);
+    }
   g.rc2 = m.rc * m.rc;
   g.dx = m.dx;
   g.xdx = m.xdx;

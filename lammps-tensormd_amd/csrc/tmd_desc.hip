@@ -110,17 +110,50 @@ __device__ inline void monomials20(float x, float y, float z, float *M)
   M[19] = z * zz;
 }
 
-// Kerley rational interpolation of one filter (gpu/cuda/ration.cu:59-68);
-// the caller zeroes the two end cells.
-__device__ inline void kerley(float4 t, float q, float p, float dx, float &h, float &dh)
+// Kerley rational interpolation of one filter (gpu/cuda/ration.cu:59-68),
+// table entry t = (y, s, c1, c2), q = r - n dx, p = dx - q, qp = q - p:
+//   w = |c2 p| / (|c2 p| + |c1 q|)  (0 when both vanish)
+//   c = c2 + w (c1 - c2),  h = y + q (s - p c),
+//   dh/dr = s + (q - p) c + dx w (1 - w) (c1 - c2)
+// (the last term is the reference's dx (c - c2)(1 - w)).  The max() stands
+// for its "denominator > 0" test: a zero denominator has a zero numerator.
+// The caller zeroes the two end cells.
+__device__ inline float kerley_w(float4 t, float q, float p)
 {
-  float c1 = t.z, c2 = t.w, s = t.y;
-  float c3 = fabsf(c2 * p);
-  float c4 = c3 + fabsf(c1 * q);
-  if (c4 > 0.0f) c3 = c3 * __builtin_amdgcn_rcpf(c4);  // 1-ulp reciprocal
-  c4 = c2 + c3 * (c1 - c2);
-  h = t.x + q * (s - p * c4);
-  dh = s + (q - p) * c4 + dx * (c4 - c2) * (1.0f - c3);
+  float a = fabsf(t.w) * fabsf(p);
+  float den = fmaf(fabsf(t.z), fabsf(q), a);
+  return a * __builtin_amdgcn_rcpf(fmaxf(den, 1.17549435e-38f));  // 1-ulp reciprocal
+}
+
+__device__ inline float kerley_h(float4 t, float q, float p)
+{
+  float c = fmaf(kerley_w(t, q, p), t.z - t.w, t.w);
+  return fmaf(q, fmaf(-p, c, t.y), t.x);
+}
+
+__device__ inline void kerley(float4 t, float q, float p, float qp, float dx, float &h, float &dh)
+{
+  float w = kerley_w(t, q, p);
+  float dc = t.z - t.w;
+  float c = fmaf(w, dc, t.w);
+  h = fmaf(q, fmaf(-p, c, t.y), t.x);
+  dh = fmaf(dx * dc, fmaf(-w, w, w), fmaf(qp, c, t.y));
+}
+
+// T[m][d] of a max_moment-3 descriptor (kDeg, kMult, plus the T-symmetric
+// extras T[2][0] = -1/3, T[3][1..3] = -3/5) for a lane-dependent d.
+__device__ inline float t_entry(int m, int d, int tsym)
+{
+  constexpr unsigned long long kMultBits = [] {
+    unsigned long long b = 0;
+    for (int i = 0; i < kND; i++) b |= (unsigned long long) kMult[i] << (3 * i);
+    return b;
+  }();
+  const int deg = d >= 10 ? 3 : (d >= 4 ? 2 : (d >= 1 ? 1 : 0));
+  float t = deg == m ? (float) ((kMultBits >> (3 * d)) & 7ull) : 0.0f;
+  if (tsym && m == 2 && d == 0) t = (float) (-1.0 / 3.0);
+  if (tsym && m == 3 && d >= 1 && d <= 3) t = (float) (-3.0 / 5.0);
+  return t;
 }
 
 // Cross-lane moves within a row of 16 lanes (DPP: no LDS traffic).
@@ -208,7 +241,8 @@ __device__ inline int kept_slot(const ModelGeo &g, const Box &cell, const double
   double d[3];
   displacement(xi, xj, e >> kNbrJBits, cell, d);
   double r2 = dist2(d);
-  return (r2 < g.rc2 && r2 > 1e-10) ? g.eltij[ei][(int) xj.w] : -1;  // kernel.cu:1163
+  return (r2 < g.rc2 && r2 > 1e-10) ? (int) ((g.slots[ei] >> (4 * (int) xj.w)) & 15u)
+                                    : -1;  // kernel.cu:1163
 }
 
 __device__ inline int compact(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
@@ -329,6 +363,11 @@ __device__ inline bool table_cell_live(const ModelGeo &g, int n) { return n > 0 
 // neighbor's filters over more lanes instead of leaving lanes idle.
 __device__ inline int lanes_per_nbr(int rem) { return rem > 16 ? 2 : (rem > 8 ? 4 : 8); }
 
+// Lanes per neighbor in the chain-rule pass: one lane per neighbor (a chunk
+// of 64) while more than 48 remain, then as lanes_per_nbr.  One wide chunk
+// runs the per-neighbor set-up and the force scatter once instead of twice.
+__device__ inline int chain_lanes(int rem) { return rem > 48 ? 1 : lanes_per_nbr(rem); }
+
 // table loads in flight per lane: divides K/2, K/4 and K/8
 template <int K>
 constexpr int kTabBatch = ((K / 8) % 4 == 0) ? 4 : K / 8;
@@ -369,11 +408,7 @@ __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &
 #pragma unroll
         for (int u = 0; u < B; u++) t[u] = row[k0 + u];
 #pragma unroll
-        for (int u = 0; u < B; u++) {
-          float h, dh;
-          kerley(t[u], ng.q, ng.pq, g.dxf, h, dh);
-          Hc[k0 + u] = h;
-        }
+        for (int u = 0; u < B; u++) Hc[k0 + u] = kerley_h(t[u], ng.q, ng.pq);
       }
     } else {
       for (int k = 0; k < KL; k++) Hc[k] = 0.0f;
@@ -397,6 +432,7 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
   for (int b = 0; b < NS; b++) {
     if (b >= g.nelt) break;
     int c0 = max(start[b] - base, 0), c1 = min(start[b + 1] - base, kChunk);
+#pragma unroll 4
     for (int c = c0; c < c1; c++) {
       float h[T::KT], m[T::DT];
 #pragma unroll
@@ -465,6 +501,11 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
   // (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
   const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
   const int gbase = kq * T::NDG;
+  float tc[kNM][T::DT];
+#pragma unroll
+  for (int m = 1; m < kNM; m++)
+#pragma unroll
+    for (int q = 0; q < T::DT; q++) tc[m][q] = t_entry(m, dq * T::DT + q, g.tsym);
 #pragma unroll
   for (int b = 0; b < NS; b++) {
     if (b >= g.nelt) break;
@@ -473,10 +514,9 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
       float part[kNM] = {0, 0, 0, 0};
 #pragma unroll
       for (int q = 0; q < T::DT; q++) {
-        const int d = dq * T::DT + q;
         float p2 = acc[b][a][q] * acc[b][a][q];
 #pragma unroll
-        for (int m = 1; m < kNM; m++) part[m] += g.T[m][d] * p2;
+        for (int m = 1; m < kNM; m++) part[m] += tc[m][q] * p2;
       }
       float tot[kNM] = {0, 0, 0, 0};
 #pragma unroll
@@ -523,6 +563,7 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
       if (table_cell_live(g, ng.n)) {
         float M[kND];
         monomials20(ng.ux, ng.uy, ng.uz, M);
+        const float qp = ng.q - ng.pq;
         const float4 *trow = tab + (size_t) ng.n * K + sub * KL;
         const float *S = L.PS + (size_t) b * T::SSTR + sub * KL * kND;
         constexpr int B = kTabBatch<K>;
@@ -534,27 +575,34 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
 #pragma unroll
           for (int u = 0; u < B; u++) {
             float h, dh;
-            kerley(t[u], ng.q, ng.pq, g.dxf, h, dh);
+            kerley(t[u], ng.q, ng.pq, qp, g.dxf, h, dh);
             const float4 *Sk = reinterpret_cast<const float4 *>(S + (k0 + u) * kND);
-            float U = 0.0f;
+            float sv[kND];
 #pragma unroll
             for (int q4 = 0; q4 < kND / 4; q4++) {
-              float4 sv = Sk[q4];
-              U += sv.x * M[4 * q4] + sv.y * M[4 * q4 + 1] + sv.z * M[4 * q4 + 2] + sv.w * M[4 * q4 + 3];
-              V[4 * q4] += sv.x * h;
-              V[4 * q4 + 1] += sv.y * h;
-              V[4 * q4 + 2] += sv.z * h;
-              V[4 * q4 + 3] += sv.w * h;
+              float4 v4 = Sk[q4];
+              sv[4 * q4] = v4.x;
+              sv[4 * q4 + 1] = v4.y;
+              sv[4 * q4 + 2] = v4.z;
+              sv[4 * q4 + 3] = v4.w;
             }
-            f1 += U * dh;
+            // U_k = sum_d S_kd M_d (M_0 = 1), V_d += S_kd H_k
+            float U = sv[0];
+#pragma unroll
+            for (int q = 1; q < kND; q++) U = fmaf(sv[q], M[q], U);
+#pragma unroll
+            for (int q = 0; q < kND; q++) V[q] = fmaf(sv[q], h, V[q]);
+            f1 = fmaf(U, dh, f1);
           }
         }
       }
     }
     // combine the Lp partial sums into the neighbor's first lane
+    if constexpr (LP >= 2) {
 #pragma unroll
-    for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor1>(V[q]);
-    f1 += dpp_f<kDppXor1>(f1);
+      for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor1>(V[q]);
+      f1 += dpp_f<kDppXor1>(f1);
+    }
     if constexpr (LP >= 4) {
 #pragma unroll
       for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor2>(V[q]);
@@ -636,31 +684,38 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
   }
   __syncthreads();
 
-  // S[b][k][d] = P * sum_m 2 T[m][d] g'_m, in place
+  // S[b][k][d] = P * sum_m 2 T[m][d] g'_m, in place: T[m][d] is the
+  // multiplicity of monomial d in its degree m, plus the T-symmetric extras
   const float *grow = dEdG + (size_t) row * g.nelt * K * kNM;
   for (int e = lane; e < g.nelt * K; e += 64) {
     float *P = L.PS + (size_t) (e / K) * T::SSTR + (e % K) * kND;
     float4 gv = *reinterpret_cast<const float4 *>(grow + (size_t) e * kNM);
     float G0 = P[0];
     float g0 = (fabsf(G0) < 1e-8f) ? 0.0f : gv.x / G0 * 0.5f;
-    float gm[kNM] = {g0, gv.y, gv.z, gv.w};
+    float w[kNM] = {2.0f * g0, 2.0f * gv.y, 2.0f * gv.z, 2.0f * gv.w};
+    float w0 = w[0], w1 = w[1];
+    if (g.tsym) {
+      w0 = fmaf(w[2], (float) (-1.0 / 3.0), w0);
+      w1 = fmaf(w[3], (float) (-3.0 / 5.0), w1);
+    }
+    P[0] *= w0;
 #pragma unroll
-    for (int d = 0; d < kND; d++) {
-      float sd = 0.0f;
-#pragma unroll
-      for (int m = 0; m < kNM; m++) sd += gm[m] * 2.0f * g.T[m][d];
-      P[d] = sd * P[d];
+    for (int d = 1; d < kND; d++) {
+      const float wd = kDeg[d] == 1 ? w1 : (kMult[d] == 1 ? w[kDeg[d]] : (float) kMult[d] * w[kDeg[d]]);
+      P[d] *= wd;
     }
   }
   __syncthreads();
 
-  // Phase 3: the chain rule, chunk by chunk (2 lanes per neighbor, or 4 / 8
+  // Phase 3: the chain rule, chunk by chunk (1 lane per neighbor, or 2 / 4 / 8
   // for a short last chunk)
   double fi[3] = {0, 0, 0};
   double wi[6] = {0, 0, 0, 0, 0, 0};
   for (int base = 0; base < nk;) {
-    const int Lp = lanes_per_nbr(nk - base);
-    if (Lp == 2)
+    const int Lp = chain_lanes(nk - base);
+    if (Lp == 1)
+      force_chunk<K, 1>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
+    else if (Lp == 2)
       force_chunk<K, 2>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
     else if (Lp == 4)
       force_chunk<K, 4>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);

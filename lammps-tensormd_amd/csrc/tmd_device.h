@@ -29,13 +29,22 @@ struct CellInfo {
   int pad;
 };
 
+// Monomials of max_moment 3 in the order degree 0, 1, 2, 3 and, within a
+// degree, x-power then y-power descending (tensormd.cpp:205-276): degree and
+// multinomial multiplicity of monomial d.  T[m][d] = kMult[d] if kDeg[d] == m;
+// a T-symmetric model adds T[2][0] = -1/3 and T[3][1..3] = -3/5.
+constexpr int kDeg[kND] = {0, 1, 1, 1, 2, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3, 3, 3};
+constexpr int kMult[kND] = {1, 1, 1, 1, 1, 2, 2, 1, 2, 1, 1, 3, 3, 3, 6, 3, 1, 3, 3, 1};
+
 // Per-model constants passed by value to the descriptor/force kernels.
+// slots[ei] packs the slot of neighbor element ej in bits 4ej..4ej+3
+// (eltij of tensormd.cpp:165-176).
 struct ModelGeo {
   int nelt;
   int k;
   int tab_n;
-  int eltij[kMaxElts][kMaxElts];
-  float T[kNM][kND];
+  int tsym;  // 1: T-symmetric descriptor
+  uint32_t slots[kMaxElts];
   double rc2;
   double dx, xdx;
   float dxf;

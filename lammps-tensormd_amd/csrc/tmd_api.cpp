@@ -238,6 +238,12 @@ int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows)
   return TMD_OK;
 }
 
+int ensure_kept(tmd_ctx *ctx, Workspace &w, int nrows, int cap)
+{
+  hipError_t e = w.kl.ensure((size_t) std::max(nrows, 1) * kept_cache_words(cap) * 4);
+  return e == hipSuccess ? TMD_OK : hip_fail(ctx, e, "kept-list cache alloc");
+}
+
 int ensure_atoms(tmd_ctx *ctx, Workspace &w, int natoms)
 {
   hipError_t e = w.ew.ensure((size_t) std::max(natoms, 1) * 7 * sizeof(double));
@@ -252,9 +258,13 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
   const int *row2atom = (const int *) w.row2atom.p;
   double *eatom = (double *) w.ew.p;
   double *watom = eatom + c.natoms;
+  // the kept-list cache is sized before any capture; never allocate here
+  if (w.kl.bytes < (size_t) w.rl.nrows * kept_cache_words(c.cap) * 4)
+    return (set_err(ctx, "internal: kept-list cache not allocated for this list"), TMD_ERR_STATE);
+  uint32_t *kl = (uint32_t *) w.kl.p;
   if (pe) hipEventRecord(pe->ev[0], s);
   e = launch_descriptor(ctx->geo, c.geo, c.pos, c.nbr, c.nn, c.cap, row2atom, w.rl.nrows,
-                        (const float4 *) ctx->tab.p, (float *) w.G.p, s);
+                        (const float4 *) ctx->tab.p, (float *) w.G.p, kl, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
   if (pe) hipEventRecord(pe->ev[1], s);
   e = launch_mlp(mode, w.rl, ctx->mlp, (const float *) w.G.p, (double *) w.E.p, (float *) w.dEdG.p, s,
@@ -271,9 +281,9 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
   if (c.zero_ew && (e = launch_zero(eatom, (size_t) c.natoms * 7 * sizeof(double), s)) != hipSuccess)
     return hip_fail(ctx, e, "zero eatom");
   if (pe) hipEventRecord(pe->ev[3], s);
-  e = launch_force(ctx->geo, c.geo, c.pos, c.nbr, c.nn, c.cap, row2atom, w.rl.nrows,
-                   (const float4 *) ctx->tab.p, (const float *) w.dEdG.p, (const double *) w.E.p,
-                   c.scale, c.f, eatom, watom, c.vatom, s);
+  e = launch_force(ctx->geo, c.geo, c.pos, c.cap, row2atom, w.rl.nrows, (const float4 *) ctx->tab.p,
+                   kl, (const float *) w.dEdG.p, (const double *) w.E.p, c.scale, c.f, eatom,
+                   watom, c.vatom, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "force launch");
   if (pe) hipEventRecord(pe->ev[4], s);
   if (c.struct_off)
@@ -507,6 +517,7 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   }
   int rc = ensure_rows(ctx, w, w.rl.nrows);
   if (rc != TMD_OK) return rc;
+  if ((rc = ensure_kept(ctx, w, w.rl.nrows, cap)) != TMD_OK) return rc;
   hipError_t e;
   if ((e = w.pos.ensure(sizeof(double4) * pos.size())) != hipSuccess) return hip_fail(ctx, e, "pos alloc");
   if ((e = w.nbr.ensure(4 * list.size())) != hipSuccess) return hip_fail(ctx, e, "nbr alloc");

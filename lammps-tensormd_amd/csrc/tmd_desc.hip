@@ -200,11 +200,11 @@ struct LDSMap {
   uint32_t *kept;  // [cap] compacted, slot-sorted neighbor entries
   float *M;        // [kChunk][kMS] monomials of the staged chunk
   float *H;        // [2 halves][kChunk][HS] radial functions of the staged chunk
-  float *PS;       // force: [nslot][SSTR] P, then S (aliases M and H once P is built)
 };
 
+// descriptor LDS: kept list, monomials, radial functions
 template <int K>
-__host__ __device__ inline size_t lds_layout(int nelt, int cap, bool force, size_t *offs)
+__host__ __device__ inline size_t lds_layout(int cap, size_t *offs)
 {
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -212,25 +212,35 @@ __host__ __device__ inline size_t lds_layout(int nelt, int cap, bool force, size
     o += (bytes + 15) & ~(size_t) 15;
     return r;
   };
-  const size_t mreg = (size_t) kChunk * kMS * 4;
-  const size_t hreg = (size_t) 2 * Tiling<K>::HALF * 4;
-  const size_t preg = force ? (size_t) nelt * Tiling<K>::SSTR * 4 : 0;
   offs[0] = take((size_t) cap * 4);
-  offs[1] = take(mreg + hreg > preg ? mreg + hreg : preg);
-  offs[2] = offs[1] + mreg;
+  offs[1] = take((size_t) kChunk * kMS * 4);
+  offs[2] = take((size_t) 2 * Tiling<K>::HALF * 4);
   return o;
 }
 
+// force LDS: the staging area (monomials, radial functions), later P and
+// then S of every slot [nelt][SSTR] over it; the kept list stays in HBM
 template <int K>
-__device__ inline LDSMap carve(unsigned char *base, int nelt, int cap, bool force)
+__host__ __device__ inline size_t force_lds_layout(int nelt, size_t *offs)
+{
+  const size_t mreg = (size_t) kChunk * kMS * 4;
+  const size_t hreg = (size_t) 2 * Tiling<K>::HALF * 4;
+  const size_t preg = (size_t) nelt * Tiling<K>::SSTR * 4;
+  offs[0] = 0;
+  offs[1] = 0;
+  offs[2] = mreg;
+  return mreg + hreg > preg ? mreg + hreg : preg;
+}
+
+template <int K>
+__device__ inline LDSMap carve(unsigned char *base, int cap)
 {
   size_t offs[3];
-  lds_layout<K>(nelt, cap, force, offs);
+  lds_layout<K>(cap, offs);
   LDSMap L;
   L.kept = (uint32_t *) (base + offs[0]);
   L.M = (float *) (base + offs[1]);
   L.H = (float *) (base + offs[2]);
-  L.PS = L.M;
   return L;
 }
 
@@ -447,15 +457,15 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
   }
 }
 
-// Phases 1-2: P of every slot in the lane's register block.  Returns nk.
+// Phase 2: P of every slot in the lane's register block over the kept list
+// L.kept[0, nk) with slot starts start[].
 template <int K, int NS>
-__device__ inline int build_P(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
-                              const uint32_t *list, int nraw, const double4 *pos, const float4 *tab,
-                              LDSMap &L, int *start, float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
+__device__ inline void accumulate_P(const ModelGeo &g, const Box &cell, const double4 &xi,
+                                    const double4 *pos, const float4 *tab, LDSMap &L,
+                                    const int *start, int nk,
+                                    float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
 {
   using T = Tiling<K>;
-  int nk = compact(g, cell, xi, ei, list, nraw, pos, L.kept, start);
-  __syncthreads();
 #pragma unroll
   for (int b = 0; b < NS; b++)
 #pragma unroll
@@ -469,6 +479,17 @@ __device__ inline int build_P(const ModelGeo &g, const Box &cell, const double4 
     __syncthreads();
     base += 64 / Lp;
   }
+}
+
+// Phases 1-2: compaction, then P.  Returns nk.
+template <int K, int NS>
+__device__ inline int build_P(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
+                              const uint32_t *list, int nraw, const double4 *pos, const float4 *tab,
+                              LDSMap &L, int *start, float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
+{
+  int nk = compact(g, cell, xi, ei, list, nraw, pos, L.kept, start);
+  __syncthreads();
+  accumulate_P<K, NS>(g, cell, xi, pos, tab, L, start, nk, acc);
   return nk;
 }
 
@@ -476,7 +497,7 @@ template <int K, int NS>
 __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const double4 *pos,
                                                    const uint32_t *nbr, const int *nn, int cap,
                                                    const int *row2atom, int nrows,
-                                                   const float4 *tab, float *G)
+                                                   const float4 *tab, float *G, uint32_t *kl)
 {
   extern __shared__ __align__(16) unsigned char smem[];
   const int row = blockIdx.x;
@@ -490,12 +511,22 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
     return;
   }
   using T = Tiling<K>;
-  LDSMap L = carve<K>(smem, g.nelt, cap, false);
+  LDSMap L = carve<K>(smem, cap);
   int start[kMaxElts + 1];
   const double4 xi = pos[i];
   const Box cell = load_box(geo, i);
   float acc[NS][T::KT][T::DT];
-  build_P<K, NS>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start, acc);
+  const int nk =
+      build_P<K, NS>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start, acc);
+  // the row's kept list with its slot starts, for the force kernel
+  {
+    uint32_t *kr = kl + (size_t) row * kept_cache_words(cap);
+    if (lane == 0) {
+      kr[0] = (uint32_t) nk;
+      for (int b = 0; b <= g.nelt; b++) kr[1 + b] = (uint32_t) start[b];
+    }
+    for (int e = lane; e < nk; e += 64) kr[kKeptHdr + e] = L.kept[e];
+  }
   // G[b][k][m] = sum_d T[m][d] P^2 (m >= 1): partial sums over the lane's d
   // block, then a sum over the NDG lanes that share k; G[b][k][0] = P0
   // (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
@@ -539,9 +570,9 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
 // neighbor forces to f_j (FP64 atomics) and to the atom's fi, wi.
 template <int K, int LP>
 __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const double4 &xi,
-                                   const double4 *pos, const float4 *tab, const LDSMap &L,
-                                   const int *start, int base, int nk, double scale, double *f,
-                                   double *vatom, double fi[3], double wi[6])
+                                   const double4 *pos, const float4 *tab, const float *PS,
+                                   const uint32_t *kept, const int *start, int base, int nk,
+                                   double scale, double *f, double *vatom, double fi[3], double wi[6])
 {
   using T = Tiling<K>;
   const int lane = threadIdx.x & 63;
@@ -559,13 +590,13 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
     for (int q = 0; q < kND; q++) V[q] = 0.0f;
     float f1 = 0.0f;
     if (live) {
-      nbr_geometry(g, cell, xi, pos, L.kept[base + c], ng);
+      nbr_geometry(g, cell, xi, pos, kept[base + c], ng);
       if (table_cell_live(g, ng.n)) {
         float M[kND];
         monomials20(ng.ux, ng.uy, ng.uz, M);
         const float qp = ng.q - ng.pq;
         const float4 *trow = tab + (size_t) ng.n * K + sub * KL;
-        const float *S = L.PS + (size_t) b * T::SSTR + sub * KL * kND;
+        const float *S = PS + (size_t) b * T::SSTR + sub * KL * kND;
         constexpr int B = kTabBatch<K>;
 #pragma unroll 1
         for (int k0 = 0; k0 < KL; k0 += B) {
@@ -647,13 +678,20 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
   }
 }
 
+// 4 waves per SIMD (at most 128 VGPRs): the chain-rule pass hides its table
+// and LDS latency with more waves; the few spilled values live outside its loops
+// Waves per SIMD asked of the force kernel: 4 (at most 128 VGPRs, a few
+// values spilled outside the loops) where the P block of a lane is small;
+// the wider variants keep their registers and run fewer waves.
 template <int K, int NS>
-__global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double4 *pos,
-                                              const uint32_t *nbr, const int *nn, int cap,
+constexpr int kForceWaves = (K <= 24 && NS * Tiling<K>::KT * Tiling<K>::DT <= 40) ? 4 : 1;
+
+template <int K, int NS>
+__global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, Geom geo, const double4 *pos, int cap,
                                               const int *row2atom, int nrows, const float4 *tab,
-                                              const float *dEdG, const double *E, double scale,
-                                              double *f, double *eatom, double *watom,
-                                              double *vatom)
+                                              const uint32_t *kl, const float *dEdG,
+                                              const double *E, double scale, double *f,
+                                              double *eatom, double *watom, double *vatom)
 {
   using T = Tiling<K>;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -662,14 +700,24 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
   const int lane = threadIdx.x & 63;
   const int i = row2atom[row];
   if (i < 0) return;  // padding row
-  LDSMap L = carve<K>(smem, g.nelt, cap, true);
-  int start[kMaxElts + 1];
+  size_t offs[3];
+  force_lds_layout<K>(g.nelt, offs);
+  LDSMap L;
+  L.M = reinterpret_cast<float *>(smem + offs[1]);
+  L.H = reinterpret_cast<float *>(smem + offs[2]);
+  float *PS = reinterpret_cast<float *>(smem);  // over the staging area once P is built
   const double4 xi = pos[i];
   const Box cell = load_box(geo, i);
+  // the descriptor kernel's kept list of this row (slot-sorted, within rc)
+  const uint32_t *kr = kl + (size_t) row * kept_cache_words(cap);
+  const int nk = (int) kr[0];
+  int start[kMaxElts + 1];
+  for (int b = 0; b <= g.nelt; b++) start[b] = (int) kr[1 + b];
+  const uint32_t *kept = kr + kKeptHdr;
+  L.kept = const_cast<uint32_t *>(kept);
   float acc[NS][T::KT][T::DT];
-  const int nk = build_P<K, NS>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L,
-                                start, acc);
-  // P -> LDS [b][k][d] (over the staging area; build_P ends with a barrier)
+  accumulate_P<K, NS>(g, cell, xi, pos, tab, L, start, nk, acc);
+  // P -> LDS [b][k][d] (over the staging area; accumulate_P ends with a barrier)
   if (lane < T::LANES) {
     const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
 #pragma unroll
@@ -679,7 +727,7 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
       for (int a = 0; a < T::KT; a++)
 #pragma unroll
         for (int q = 0; q < T::DT; q++)
-          L.PS[(size_t) b * T::SSTR + (kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
+          PS[(size_t) b * T::SSTR + (kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
     }
   }
   __syncthreads();
@@ -688,7 +736,7 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
   // multiplicity of monomial d in its degree m, plus the T-symmetric extras
   const float *grow = dEdG + (size_t) row * g.nelt * K * kNM;
   for (int e = lane; e < g.nelt * K; e += 64) {
-    float *P = L.PS + (size_t) (e / K) * T::SSTR + (e % K) * kND;
+    float *P = PS + (size_t) (e / K) * T::SSTR + (e % K) * kND;
     float4 gv = *reinterpret_cast<const float4 *>(grow + (size_t) e * kNM);
     float G0 = P[0];
     float g0 = (fabsf(G0) < 1e-8f) ? 0.0f : gv.x / G0 * 0.5f;
@@ -714,13 +762,13 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
   for (int base = 0; base < nk;) {
     const int Lp = chain_lanes(nk - base);
     if (Lp == 1)
-      force_chunk<K, 1>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
+      force_chunk<K, 1>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, fi, wi);
     else if (Lp == 2)
-      force_chunk<K, 2>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
+      force_chunk<K, 2>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, fi, wi);
     else if (Lp == 4)
-      force_chunk<K, 4>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
+      force_chunk<K, 4>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, fi, wi);
     else
-      force_chunk<K, 8>(g, cell, xi, pos, tab, L, start, base, nk, scale, f, vatom, fi, wi);
+      force_chunk<K, 8>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, fi, wi);
     base += 64 / Lp;
   }
 #pragma unroll
@@ -744,28 +792,28 @@ __global__ __launch_bounds__(64) void k_force(ModelGeo g, Geom geo, const double
 template <int K, int NS>
 hipError_t desc_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
                      const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
-                     float *G, hipStream_t s)
+                     float *G, uint32_t *kl, hipStream_t s)
 {
-  size_t offs[8];
-  size_t lds = lds_layout<K>(g.nelt, cap, false, offs);
+  size_t offs[3];
+  size_t lds = lds_layout<K>(cap, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
   hipFuncSetAttribute((const void *) k_descriptor<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-  k_descriptor<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G);
+  k_descriptor<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl);
   return hipGetLastError();
 }
 
 template <int K, int NS>
-hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
-                      const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
+hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
+                      const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
                       const float *dEdG, const double *E, double scale, double *f, double *eatom,
                       double *watom, double *vatom, hipStream_t s)
 {
-  size_t offs[8];
-  size_t lds = lds_layout<K>(g.nelt, cap, true, offs);
+  size_t offs[3];
+  const size_t lds = force_lds_layout<K>(g.nelt, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
   hipFuncSetAttribute((const void *) k_force<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
-  k_force<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, dEdG, E,
-                                        scale, f, eatom, watom, vatom);
+  k_force<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale,
+                                        f, eatom, watom, vatom);
   return hipGetLastError();
 }
 
@@ -774,19 +822,18 @@ hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, co
 template <int K, bool FORCE>
 hipError_t dispatch(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
                     const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
-                    float *G, const float *dEdG, const double *E, double scale, double *f,
-                    double *eatom, double *watom, double *vatom, hipStream_t s)
+                    float *G, uint32_t *kl, const float *dEdG, const double *E, double scale,
+                    double *f, double *eatom, double *watom, double *vatom, hipStream_t s)
 {
   constexpr int per = Tiling<K>::KT * Tiling<K>::DT;
-#define TMD_NS_CASE(NS)                                                                            \
-  if (g.nelt <= NS) {                                                                              \
-    if (NS * per > 128) return hipErrorInvalidConfiguration;                                       \
-    if (FORCE)                                                                                     \
-      return force_impl<K, (NS * per > 128 ? 1 : NS)>(g, geo, pos, nbr, nn, cap, row2atom, nrows,  \
-                                                       tab, dEdG, E, scale, f, eatom, watom, vatom, \
-                                                       s);                                          \
-    return desc_impl<K, (NS * per > 128 ? 1 : NS)>(g, geo, pos, nbr, nn, cap, row2atom, nrows,     \
-                                                   tab, G, s);                                      \
+#define TMD_NS_CASE(NS)                                                                             \
+  if (g.nelt <= NS) {                                                                               \
+    if (NS * per > 128) return hipErrorInvalidConfiguration;                                        \
+    if (FORCE)                                                                                      \
+      return force_impl<K, (NS * per > 128 ? 1 : NS)>(g, geo, pos, cap, row2atom, nrows, tab, kl,   \
+                                                       dEdG, E, scale, f, eatom, watom, vatom, s);  \
+    return desc_impl<K, (NS * per > 128 ? 1 : NS)>(g, geo, pos, nbr, nn, cap, row2atom, nrows,      \
+                                                   tab, G, kl, s);                                   \
   }
   TMD_NS_CASE(1)
   TMD_NS_CASE(2)
@@ -800,12 +847,13 @@ hipError_t dispatch(const ModelGeo &g, const Geom &geo, const double4 *pos, cons
 
 hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *pos,
                              const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
-                             int nrows, const float4 *tab, float *G, hipStream_t stream)
+                             int nrows, const float4 *tab, float *G, uint32_t *kl,
+                             hipStream_t stream)
 {
   if (nrows <= 0) return hipSuccess;
-#define TMD_K_CASE(K)                                                                          \
-  case K:                                                                                      \
-    return dispatch<K, false>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, nullptr,     \
+#define TMD_K_CASE(K)                                                                         \
+  case K:                                                                                     \
+    return dispatch<K, false>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, nullptr, \
                               nullptr, 0, nullptr, nullptr, nullptr, nullptr, stream);
   switch (g.k) {
     TMD_K_CASE(16)
@@ -818,17 +866,17 @@ hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *
 #undef TMD_K_CASE
 }
 
-hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos,
-                        const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
-                        int nrows, const float4 *tab, const float *dEdG, const double *E,
-                        double scale, double *f, double *eatom, double *watom, double *vatom,
-                        hipStream_t stream)
+hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
+                        const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
+                        const float *dEdG, const double *E, double scale, double *f, double *eatom,
+                        double *watom, double *vatom, hipStream_t stream)
 {
   if (nrows <= 0) return hipSuccess;
-#define TMD_K_CASE(K)                                                                           \
-  case K:                                                                                       \
-    return dispatch<K, true>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, nullptr, dEdG, E, \
-                             scale, f, eatom, watom, vatom, stream);
+#define TMD_K_CASE(K)                                                                            \
+  case K:                                                                                        \
+    return dispatch<K, true>(g, geo, pos, nullptr, nullptr, cap, row2atom, nrows, tab, nullptr,  \
+                             const_cast<uint32_t *>(kl), dEdG, E, scale, f, eatom, watom, vatom, \
+                             stream);
   switch (g.k) {
     TMD_K_CASE(16)
     TMD_K_CASE(24)

@@ -131,19 +131,27 @@ struct Geom {
   const int *atom_struct;
   const BatchDims *dyn;  // rows beyond dyn->rl.nrows are skipped when non-null
 };
+// The descriptor kernel's compaction of each row's list, reused by the force
+// kernel of the same evaluation: kl[row][kKeptHdr + cap], kl[0] = nk,
+// kl[1 + b] = first entry of slot b (b = 0..nelt), then the nk kept entries
+// (within rc, sorted by slot).
+constexpr int kKeptHdr = 16;
+__host__ __device__ inline size_t kept_cache_words(int cap) { return (size_t) kKeptHdr + (size_t) cap; }
+
+// Descriptor rows G[row] and the kept list of every row.
 hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *pos,
                              const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
-                             int nrows, const float4 *tab, float *G, hipStream_t stream);
+                             int nrows, const float4 *tab, float *G, uint32_t *kl,
+                             hipStream_t stream);
 
-// Force/virial: dEdG rows -> f (atomics, FP64, must be zeroed); per atom:
-// eatom[i] = scale * E[row] and watom[i][6] = sum over the atom's pairs of
-// -d (x) F (both written only for atoms that own a row); optional vatom
-// (FP64 atomics; must be zeroed).
-hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos,
-                        const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
-                        int nrows, const float4 *tab, const float *dEdG, const double *E,
-                        double scale, double *f, double *eatom, double *watom, double *vatom,
-                        hipStream_t stream);
+// Force/virial over the descriptor's kept lists: dEdG rows -> f (atomics,
+// FP64, must be zeroed); per atom: eatom[i] = scale * E[row] and
+// watom[i][6] = sum over the atom's pairs of -d (x) F (both written only for
+// atoms that own a row); optional vatom (FP64 atomics; must be zeroed).
+hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
+                        const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
+                        const float *dEdG, const double *E, double scale, double *f, double *eatom,
+                        double *watom, double *vatom, hipStream_t stream);
 
 // ---- MLP ----------------------------------------------------------------------
 // mode: 0 fp32, 1 bf16, 2 fp8 (e4m3 operands: weights with per-output-channel

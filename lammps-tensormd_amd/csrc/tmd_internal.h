@@ -40,11 +40,13 @@ struct DeviceBuffer {
 };
 
 // Buffers of one evaluation: positions, list, MLP activations, outputs.
-// ew holds eatom[n] followed by watom[n][6].
+// ew holds eatom[n] followed by watom[n][6]; kl holds the descriptor
+// kernel's kept list of every row, reused by the force kernel.
 struct Workspace {
   RowLayout rl{};
   DeviceBuffer pos, nbr, nn, f, vatom, ew;
   DeviceBuffer G, dEdG, E, row2atom, red, cells;
+  DeviceBuffer kl;
 };
 
 struct Server;
@@ -77,6 +79,9 @@ struct PhaseEvents {
 void set_err(tmd_ctx *ctx, const std::string &m);
 int hip_fail(tmd_ctx *ctx, hipError_t e, const char *what);
 int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows);
+// the kept-list cache for nrows rows of a list with `cap` entries per atom
+// (allocate before any graph capture of run_chain)
+int ensure_kept(tmd_ctx *ctx, Workspace &w, int nrows, int cap);
 // Inputs and outputs of one descriptor -> MLP -> force -> reduction chain.
 struct Chain {
   Geom geo{};

@@ -569,8 +569,10 @@ __device__ inline void gemm_t(const void *W, int lda, int t0, int ntw, const T *
 
 }  // namespace
 
+// at least 3 waves per SIMD (at most 168 VGPRs): more tiles in flight hide
+// the L2 latency of the weight operands
 template <int MODE, int ACT, int R>
-__global__ __launch_bounds__(256) void k_mlp_fast(RowLayout rl_cap, MlpParams mp, const float *Gd,
+__global__ __launch_bounds__(256, 3) void k_mlp_fast(RowLayout rl_cap, MlpParams mp, const float *Gd,
                                                   double *E, float *dEdG, const BatchDims *dyn)
 {
   const RowLayout rl = dyn ? dyn->rl : rl_cap;

@@ -130,6 +130,7 @@ size_t bytes_per_atom(const tmd_ctx *ctx, int listcap)
 {
   const size_t gw = (size_t) ctx->model.gwidth();
   return 8 * gw + 8 /*E*/ + 4 /*row2atom*/ + 4 * (size_t) listcap + 4 /*nn*/ + 32 /*pos*/ +
+      4 * kept_cache_words(listcap) /*kept lists*/ +
       24 /*f*/ + 56 /*eatom, watom*/ + 4 /*atom_struct*/ + 12 /*bins*/ + 20 /*atom bins*/ +
       32 /*x_build*/ + 48 /*staging, output*/;
 }
@@ -163,6 +164,8 @@ int alloc_list(Server *S, Slot &sl, int listcap)
 {
   hipError_t e = sl.w.nbr.ensure((size_t) S->full.atoms * listcap * 4);
   if (e != hipSuccess) return hip_fail(S->ctx, e, "serve: neighbor list alloc");
+  int rc = ensure_kept(S->ctx, sl.w, S->full.rows, listcap);
+  if (rc != TMD_OK) return rc;
   sl.listcap = listcap;
   drop_graphs(sl);
   return TMD_OK;

@@ -126,7 +126,7 @@ int alloc_list(tmd_system *s, int cap)
   hipError_t e;
   s->nb.cap = cap;
   if ((e = s->w.nbr.ensure((size_t) s->n * cap * 4)) != hipSuccess) return hip_fail(s->ctx, e, "neighbor list alloc");
-  return TMD_OK;
+  return ensure_kept(s->ctx, s->w, s->w.rl.nrows, cap);
 }
 
 // One force evaluation at the current positions (neighbor list maintained).

@@ -108,9 +108,20 @@ int upload_model(tmd_ctx *ctx)
   if (L > kMaxLayers) return (set_err(ctx, "more than 8 MLP layers"), TMD_ERR_MODEL);
   size_t total = 0;
   struct Off {
-    size_t W, Wt, Wb, Wtb, b, q8, qt8, swf, swb;
+    size_t W, Wt, Wb, Wtb, b, af, ab, swf, swb;
   };
   const bool fp8 = ctx->opt.mlp_precision == TMD_MLP_FP8;
+  const int mode = ctx->opt.mlp_precision;
+  // the fast kernel's A operands, in fragment order, for this mode only
+  bool fast;
+  {
+    MlpParams t;
+    std::memset(&t, 0, sizeof(t));
+    t.nl = L;
+    for (int l = 0; l <= L; l++) t.sizes[l] = n0.sizes[l];
+    fast = mlp_fast_ok(t, mode);
+  }
+  const size_t aesz = mode == TMD_MLP_FP32 ? 4 : (mode == TMD_MLP_BF16 ? 2 : 1);
   std::vector<Off> offs(L);
   auto al = [](size_t x) { return (x + 255) & ~(size_t) 255; };
   for (int l = 0; l < L; l++) {
@@ -125,11 +136,13 @@ int upload_model(tmd_ctx *ctx)
     total = al(total + per * m.nelt * 2);
     offs[l].b = total;
     total = al(total + (size_t) n0.sizes[l + 1] * m.nelt * 4);
-    if (fp8) {  // e4m3 copies and their per-output-channel scales
-      offs[l].q8 = total;
-      total = al(total + per * m.nelt);
-      offs[l].qt8 = total;
-      total = al(total + per * m.nelt);
+    if (fast && l < L - 1) {
+      offs[l].af = total;
+      total = al(total + per * m.nelt * aesz);
+      offs[l].ab = total;
+      total = al(total + per * m.nelt * aesz);
+    }
+    if (fp8) {  // per-output-channel scales of the e4m3 operands
       offs[l].swf = total;
       total = al(total + (size_t) n0.sizes[l + 1] * m.nelt * 4);
       offs[l].swb = total;
@@ -168,6 +181,31 @@ int upload_model(tmd_ctx *ctx)
         hasb = true;
         for (int j = 0; j < N; j++) b[el * N + j] = (float) net.B[l][j];
       }
+      if (fast && l < L - 1 && !fp8) {
+        // fragment order: [tile t][step s][lane][v]; lane = (group g) * 16 +
+        // (row in tile); forward outputs n = 16 t + row, inner k = s KS + g V
+        // + v (V = values per lane: 1 FP32, 8 BF16); backward swaps K and N
+        const bool f32 = mode == TMD_MLP_FP32;
+        const int KS = f32 ? 4 : 32, V = f32 ? 1 : 8;
+        const double *Wd = net.W[l].data();
+        auto frag = [&](unsigned char *dst, int out, int inner, bool fwd) {
+          const int nst = inner / KS;
+          for (int t = 0; t < out / 16; t++)
+            for (int st = 0; st < nst; st++)
+              for (int lane = 0; lane < 64; lane++)
+                for (int v = 0; v < V; v++) {
+                  const int o = 16 * t + lane % 16, in = st * KS + (lane / 16) * V + v;
+                  const double w = fwd ? Wd[(size_t) in * N + o] : Wd[(size_t) o * N + in];
+                  const size_t q = (((size_t) t * nst + st) * 64 + lane) * V + v;
+                  if (f32)
+                    reinterpret_cast<float *>(dst)[q] = (float) w;
+                  else
+                    reinterpret_cast<uint16_t *>(dst)[q] = bf16_bits((float) w);
+                }
+        };
+        frag(&host[offs[l].af] + el * per * aesz, N, K, true);
+        frag(&host[offs[l].ab] + el * per * aesz, K, N, false);
+      }
       if (fp8) {
         // scale = max |w| / 440 over the output channel of each GEMM (forward:
         // column n of W; backward: row k), so that w / scale fits e4m3
@@ -191,9 +229,11 @@ int upload_model(tmd_ctx *ctx)
     mp.Wb16[l] = (const uint16_t *) (base + offs[l].Wb);
     mp.Wtb[l] = (const uint16_t *) (base + offs[l].Wtb);
     mp.b[l] = hasb ? (const float *) (base + offs[l].b) : nullptr;
+    if (fast && l < L - 1) {
+      mp.Af[l] = base + offs[l].af;
+      mp.Ab[l] = base + offs[l].ab;
+    }
     if (fp8) {
-      mp.Wq8[l] = (const uint8_t *) (base + offs[l].q8);
-      mp.Wtq8[l] = (const uint8_t *) (base + offs[l].qt8);
       mp.swf[l] = (const float *) (base + offs[l].swf);
       mp.swb[l] = (const float *) (base + offs[l].swb);
     }
@@ -205,7 +245,6 @@ int upload_model(tmd_ctx *ctx)
   mp.bout = mp.b[L - 1];
   mp.b[L - 1] = nullptr;
   // the fused kernel keeps one row tile's activations in LDS
-  const int mode = ctx->opt.mlp_precision;
   if (mlp_lds_bytes(mp, mode, 16, nullptr) > 160 * 1024)
     return (set_err(ctx, "MLP too wide for the fused kernel's LDS budget"), TMD_ERR_MODEL);
   if (fp8 && !mlp_fast_ok(mp, mode))
@@ -215,7 +254,7 @@ int upload_model(tmd_ctx *ctx)
   if (e != hipSuccess) return hip_fail(ctx, e, "parameter upload");
   if (fp8) {
     for (int l = 0; l < L - 1; l++) {
-      e = launch_quant_fp8(mp.Wf32[l], mp.swf[l], mp.swb[l], (uint8_t *) mp.Wq8[l], (uint8_t *) mp.Wtq8[l],
+      e = launch_quant_fp8(mp.Wf32[l], mp.swf[l], mp.swb[l], (uint8_t *) mp.Af[l], (uint8_t *) mp.Ab[l],
                            n0.sizes[l], n0.sizes[l + 1], m.nelt, ctx->stream);
       if (e != hipSuccess) return hip_fail(ctx, e, "FP8 weight quantization");
     }

@@ -572,7 +572,7 @@ template <int K, int LP>
 __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const double4 &xi,
                                    const double4 *pos, const float4 *tab, const float *PS,
                                    const uint32_t *kept, const int *start, int base, int nk,
-                                   double scale, double *f, double *vatom, double fi[3], double wi[6])
+                                   double scale, double *f, double *vatom, double *facc)
 {
   using T = Tiling<K>;
   const int lane = threadIdx.x & 63;
@@ -598,15 +598,25 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
         const float4 *trow = tab + (size_t) ng.n * K + sub * KL;
         const float *S = PS + (size_t) b * T::SSTR + sub * KL * kND;
         constexpr int B = kTabBatch<K>;
+        // software pipeline over batches of B filters: interpolate the batch,
+        // which frees its table registers, then load the next batch's rows
+        // into them and run this batch's U/V products while those loads fly
+        float4 t[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) t[u] = trow[u];
 #pragma unroll 1
         for (int k0 = 0; k0 < KL; k0 += B) {
-          float4 t[B];
+          float h[B], dh[B];
 #pragma unroll
-          for (int u = 0; u < B; u++) t[u] = trow[k0 + u];
+          for (int u = 0; u < B; u++) kerley(t[u], ng.q, ng.pq, qp, g.dxf, h[u], dh[u]);
+          if (k0 + B < KL) {
+#pragma unroll
+            for (int u = 0; u < B; u++) t[u] = trow[k0 + B + u];
+          }
+          // U_k = sum_d S_kd M_d (M_0 = 1), V_d += S_kd H_k (V_0 is not
+          // needed: dM_0/du = 0), f1 += U_k dH_k
 #pragma unroll
           for (int u = 0; u < B; u++) {
-            float h, dh;
-            kerley(t[u], ng.q, ng.pq, qp, g.dxf, h, dh);
             const float4 *Sk = reinterpret_cast<const float4 *>(S + (k0 + u) * kND);
             float sv[kND];
 #pragma unroll
@@ -617,13 +627,12 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
               sv[4 * q4 + 2] = v4.z;
               sv[4 * q4 + 3] = v4.w;
             }
-            // U_k = sum_d S_kd M_d (M_0 = 1), V_d += S_kd H_k
             float U = sv[0];
 #pragma unroll
             for (int q = 1; q < kND; q++) U = fmaf(sv[q], M[q], U);
 #pragma unroll
-            for (int q = 0; q < kND; q++) V[q] = fmaf(sv[q], h, V[q]);
-            f1 = fmaf(U, dh, f1);
+            for (int q = 1; q < kND; q++) V[q] = fmaf(sv[q], h[u], V[q]);
+            f1 = fmaf(U, dh[u], f1);
           }
         }
       }
@@ -631,19 +640,22 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
     // combine the Lp partial sums into the neighbor's first lane
     if constexpr (LP >= 2) {
 #pragma unroll
-      for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor1>(V[q]);
+      for (int q = 1; q < kND; q++) V[q] += dpp_f<kDppXor1>(V[q]);
       f1 += dpp_f<kDppXor1>(f1);
     }
     if constexpr (LP >= 4) {
 #pragma unroll
-      for (int q = 0; q < kND; q++) V[q] += dpp_f<kDppXor2>(V[q]);
+      for (int q = 1; q < kND; q++) V[q] += dpp_f<kDppXor2>(V[q]);
       f1 += dpp_f<kDppXor2>(f1);
     }
     if constexpr (LP == 8) {
 #pragma unroll
-      for (int q = 0; q < kND; q++) V[q] += __shfl_xor(V[q], 4);
+      for (int q = 1; q < kND; q++) V[q] += __shfl_xor(V[q], 4);
       f1 += __shfl_xor(f1, 4);
     }
+    double cf[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) cf[q] = 0.0;
     if (live && sub == 0) {
       const float x = ng.ux, y = ng.uy, z = ng.uz;
       // g_beta = sum_d V_d dM_d/du_beta
@@ -660,20 +672,28 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
       double F1v = (double) (f1 * y + (gy - gu * y) * rinv) * scale;
       double F2v = (double) (f1 * z + (gz - gu * z) * rinv) * scale;
       const int j = ng.j;
-      fi[0] += F0;
-      fi[1] += F1v;
-      fi[2] += F2v;
       atomicAdd(&f[3 * (size_t) j + 0], -F0);
       atomicAdd(&f[3 * (size_t) j + 1], -F1v);
       atomicAdd(&f[3 * (size_t) j + 2], -F2v);
       const double dx = ng.d[0], dy = ng.d[1], dz = ng.d[2];
       double v[6] = {-dx * F0, -dy * F1v, -dz * F2v, -dx * F1v, -dx * F2v, -dy * F2v};
-#pragma unroll
-      for (int q = 0; q < 6; q++) wi[q] += v[q];
       if (vatom) {
 #pragma unroll
         for (int q = 0; q < 6; q++) atomicAdd(&vatom[6 * (size_t) j + q], 0.5 * v[q]);
       }
+      cf[0] = F0;
+      cf[1] = F1v;
+      cf[2] = F2v;
+#pragma unroll
+      for (int q = 0; q < 6; q++) cf[3 + q] = v[q];
+    }
+    // the chunk's own-force and virial sums, in a fixed lane order, into the
+    // atom's LDS accumulators (nothing FP64 is live through the filter loop)
+#pragma unroll
+    for (int q = 0; q < 9; q++) cf[q] = wave_sum_d(cf[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) facc[q] += cf[q];
     }
   }
 }
@@ -757,24 +777,27 @@ __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, 
 
   // Phase 3: the chain rule, chunk by chunk (1 lane per neighbor, or 2 / 4 / 8
   // for a short last chunk)
-  double fi[3] = {0, 0, 0};
-  double wi[6] = {0, 0, 0, 0, 0, 0};
+  // own force (3) and virial (6) of the atom, summed chunk by chunk
+  __shared__ double facc[9];
+  if (lane < 9) facc[lane] = 0.0;
   for (int base = 0; base < nk;) {
     const int Lp = chain_lanes(nk - base);
     if (Lp == 1)
-      force_chunk<K, 1>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, fi, wi);
+      force_chunk<K, 1>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, facc);
     else if (Lp == 2)
-      force_chunk<K, 2>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, fi, wi);
+      force_chunk<K, 2>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, facc);
     else if (Lp == 4)
-      force_chunk<K, 4>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, fi, wi);
+      force_chunk<K, 4>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, facc);
     else
-      force_chunk<K, 8>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, fi, wi);
+      force_chunk<K, 8>(g, cell, xi, pos, tab, PS, kept, start, base, nk, scale, f, vatom, facc);
     base += 64 / Lp;
   }
+  __syncthreads();
+  double fi[3], wi[6];
 #pragma unroll
-  for (int a = 0; a < 3; a++) fi[a] = wave_sum_d(fi[a]);
+  for (int a = 0; a < 3; a++) fi[a] = facc[a];
 #pragma unroll
-  for (int q = 0; q < 6; q++) wi[q] = wave_sum_d(wi[q]);
+  for (int q = 0; q < 6; q++) wi[q] = facc[3 + q];
 
   if (lane == 0) {
     atomicAdd(&f[3 * (size_t) i + 0], fi[0]);

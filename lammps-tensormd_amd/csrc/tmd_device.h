@@ -84,8 +84,15 @@ struct MlpParams {
   const float *Wt32[kMaxLayers];
   const uint16_t *Wtb[kMaxLayers];
   const uint16_t *Wb16[kMaxLayers];
-  const uint8_t *Wq8[kMaxLayers];   // FP8 e4m3 W [K][N] / per-row scale (backward A operand)
-  const uint8_t *Wtq8[kMaxLayers];  // FP8 e4m3 W^T [N][K] / per-row scale (forward A operand)
+  // fast path: the MFMA A operands of hidden layer l in the context's
+  // precision (FP32, BF16, or FP8 e4m3 divided by the per-output-channel
+  // scale), stored in fragment order: tile t of 16 output channels, step s
+  // of KS inner values, lane, then the lane's values, so that a wave loads
+  // one fragment as one contiguous block.  Af: forward (outputs N, inner K),
+  // Ab: backward (outputs K, inner N).  Element e at byte offset
+  // e * wstride * (4, 2 or 1).
+  const void *Af[kMaxLayers];
+  const void *Ab[kMaxLayers];
   const float *swf[kMaxLayers];     // forward per-output-channel scales [N]
   const float *swb[kMaxLayers];     // backward per-output-channel scales [K]
   const float *b[kMaxLayers];    // hidden biases (or null)
@@ -165,10 +172,11 @@ hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const 
 size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out);
 // the fast (aligned) MLP kernel applies: required by the FP8 mode
 bool mlp_fast_ok(const MlpParams &mp, int mode);
-// FP8 copies of one layer's weights for nelt elements: W [nelt][K][N] FP32,
-// per-output-channel scales swf [nelt][N] and swb [nelt][K]
-hipError_t launch_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Wq,
-                            uint8_t *Wtq, int K, int N, int nelt, hipStream_t stream);
+// FP8 A operands (MlpParams::Af/Ab layout) of one layer's weights for nelt
+// elements: W [nelt][K][N] FP32, per-output-channel scales swf [nelt][N]
+// (forward) and swb [nelt][K] (backward)
+hipError_t launch_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Af,
+                            uint8_t *Ab, int K, int N, int nelt, hipStream_t stream);
 
 // ---- batches of structures (serving) ---------------------------------------------
 // Capacities of one batch slot (grids of its captured graph).

@@ -401,21 +401,19 @@ __device__ inline long pack_fp8(const float *v)
   return (long) (((unsigned long long) (unsigned) hi << 32) | (unsigned) lo);
 }
 
-// A operand (weights).  BF16: row `orow + lane%16` of a row-major [rows][lda]
-// matrix, 8 consecutive inner indices; FP32: element (k, orow + lane%16) of
-// a [inner][lda] matrix.
+// A operand (weights) of output tile t, inner step s (of nst), stored in
+// fragment order (MlpParams::Af/Ab): one contiguous block per wave.
 template <int MODE>
-__device__ inline typename Frag<MODE>::A ld_w(const void *W, int lda, int orow, int k)
+__device__ inline typename Frag<MODE>::A ld_a(const void *A, int t, int s, int nst)
 {
   const int lane = threadIdx.x & 63;
+  const size_t idx = ((size_t) t * nst + s) * 64 + lane;
   if constexpr (MODE == 0)
-    return ((const float *) W)[(size_t) (k + (lane >> 4)) * lda + orow + (lane & 15)];
+    return ((const float *) A)[idx];
   else if constexpr (MODE == 1)
-    return *reinterpret_cast<const bf16x8 *>((const uint16_t *) W + (size_t) (orow + (lane & 15)) * lda +
-                                             k + (lane >> 4) * 8);
+    return reinterpret_cast<const bf16x8 *>(A)[idx];
   else
-    return *reinterpret_cast<const long *>((const uint8_t *) W + (size_t) (orow + (lane & 15)) * lda + k +
-                                           (lane >> 4) * 8);
+    return reinterpret_cast<const long *>(A)[idx];
 }
 
 // B operand (activations of rows r0..r0+15) from a row-major LDS tile;
@@ -440,25 +438,47 @@ __device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k,
   }
 }
 
-// B operand from the FP32 descriptor rows in HBM (layer 0)
+// B operand from the FP32 descriptor rows in HBM (layer 0), in two parts so
+// that a prefetch ring holds the raw values and converts them only at use:
+// ld_graw loads 8 (1 for FP32) values of row grow + lane%16, cvt_graw makes
+// the MFMA operand (BF16; FP8 after scaling by the row's inverse scale ri).
 template <int MODE>
-__device__ inline typename Frag<MODE>::A ld_g(const float *G, int gw, int grow, int k, float ri)
+struct GRaw {
+  float4 lo, hi;
+};
+template <>
+struct GRaw<0> {
+  float v;
+};
+
+template <int MODE>
+__device__ inline GRaw<MODE> ld_graw(const float *G, int gw, int grow, int k)
 {
   const int lane = threadIdx.x & 63;
   const float *p = G + (size_t) (grow + (lane & 15)) * gw + k;
+  GRaw<MODE> g;
   if constexpr (MODE == 0) {
-    return p[lane >> 4];
+    g.v = p[lane >> 4];
   } else {
     const float4 *q = reinterpret_cast<const float4 *>(p + (lane >> 4) * 8);
-    float4 x0 = q[0], x1 = q[1];
-    if constexpr (MODE == 1) {
-      f32x8 xv = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-      return __builtin_convertvector(xv, bf16x8);
-    } else {
-      float v[8] = {x0.x * ri, x0.y * ri, x0.z * ri, x0.w * ri,
-                    x1.x * ri, x1.y * ri, x1.z * ri, x1.w * ri};
-      return pack_fp8(v);
-    }
+    g.lo = q[0];
+    g.hi = q[1];
+  }
+  return g;
+}
+
+template <int MODE>
+__device__ inline typename Frag<MODE>::A cvt_graw(const GRaw<MODE> &g, float ri)
+{
+  if constexpr (MODE == 0) {
+    return g.v;
+  } else if constexpr (MODE == 1) {
+    f32x8 xv = {g.lo.x, g.lo.y, g.lo.z, g.lo.w, g.hi.x, g.hi.y, g.hi.z, g.hi.w};
+    return __builtin_convertvector(xv, bf16x8);
+  } else {
+    float v[8] = {g.lo.x * ri, g.lo.y * ri, g.lo.z * ri, g.lo.w * ri,
+                  g.hi.x * ri, g.hi.y * ri, g.hi.z * ri, g.hi.w * ri};
+    return pack_fp8(v);
   }
 }
 
@@ -523,13 +543,22 @@ __device__ inline void st4(uint16_t *p, const float v[4])
 }
 
 // acc[j][r] = 16 output columns (tile t0 + 4 j) x 16 rows (r * 16 ..) of
-// W X^T, summed over the inner dimension K, for the wave's ntw tiles.
-template <int MODE, int R, bool XG, typename T>
-__device__ inline void gemm_t(const void *W, int lda, int t0, int ntw, const T *X, int ldx,
-                              const float *G, int gw, int grow0, int K, const float *rinv,
-                              f32x4 (&acc)[4][R / 16])
+// W X^T, summed over the inner dimension K, for the wave's NTW tiles.
+//
+// Global operands (the weights, and the descriptor rows of layer 0) go
+// through a register ring D steps deep, D = (fragments budget) / NTW (2 for
+// layer 0, whose ring also holds its raw rows), so a load has D - 1 steps of
+// MFMA work to arrive in.  Every step issues the
+// same number of loads (past the last step they repeat it), which lets the
+// compiler wait for exactly the step being consumed.  Activations in LDS
+// are read at the step that uses them.
+template <int MODE, int R, bool XG, int NTW, typename T>
+__device__ inline void gemm_t(const void *W, int t0, const T *X, int ldx, const float *G, int gw,
+                              int grow0, int K, const float *rinv, f32x4 (&acc)[4][R / 16])
 {
   constexpr int KS = Traits<MODE>::KS, RT = R / 16;
+  constexpr int DW = (MODE == 0 ? 16 : 8) / NTW;
+  constexpr int D = XG ? 2 : (DW >= 2 ? DW : 2);  // layer 0 also rings its raw rows
   using F = typename Frag<MODE>::A;
 #pragma unroll
   for (int j = 0; j < 4; j++)
@@ -538,33 +567,57 @@ __device__ inline void gemm_t(const void *W, int lda, int t0, int ntw, const T *
   float ri[RT];
 #pragma unroll
   for (int r = 0; r < RT; r++) ri[r] = MODE == 2 ? rinv[r * 16 + (threadIdx.x & 15)] : 1.0f;
-  F a0[4], a1[4], b0[RT], b1[RT];
-  auto load = [&](F(&a)[4], F(&b)[RT], int k) {
+  const int nst = K / KS;
+  F a[D][NTW];
+  GRaw<MODE> g[XG ? D : 1][RT];
+  auto issue = [&](F(&as)[NTW], GRaw<MODE>(&gs)[RT], int s) {
+    const int sc = min(s, nst - 1), k = sc * KS;
 #pragma unroll
-    for (int j = 0; j < 4; j++)
-      if (j < ntw) a[j] = ld_w<MODE>(W, lda, (t0 + 4 * j) * 16, k);
+    for (int j = 0; j < NTW; j++) as[j] = ld_a<MODE>(W, t0 + 4 * j, sc, nst);
+    if constexpr (XG) {
 #pragma unroll
-    for (int r = 0; r < RT; r++) {
-      if constexpr (XG)
-        b[r] = ld_g<MODE>(G, gw, grow0 + r * 16, k, ri[r]);
-      else
-        b[r] = ld_x<MODE>(X, ldx, r * 16, k, ri[r]);
+      for (int r = 0; r < RT; r++) gs[r] = ld_graw<MODE>(G, gw, grow0 + r * 16, k);
     }
   };
-  auto step = [&](F(&a)[4], F(&b)[RT]) {
 #pragma unroll
-    for (int j = 0; j < 4; j++)
-      if (j < ntw)
+  for (int i = 0; i < D - 1; i++) issue(a[i], g[XG ? i : 0], i);
+  for (int s0 = 0; s0 < nst; s0 += D) {
 #pragma unroll
-        for (int r = 0; r < RT; r++) acc[j][r] = mma<MODE>(a[j], b[r], acc[j][r]);
-  };
-  load(a0, b0, 0);
-  for (int k = 0; k < K; k += 2 * KS) {
-    load(a1, b1, k + KS);
-    step(a0, b0);
-    if (k + 2 * KS < K) load(a0, b0, k + 2 * KS);
-    step(a1, b1);
+    for (int i = 0; i < D; i++) {
+      const int nxt = (i + D - 1) % D;
+      issue(a[nxt], g[XG ? nxt : 0], s0 + i + D - 1);
+      const int s = s0 + i;
+      if (s < nst) {
+        F b[RT];
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+          if constexpr (XG)
+            b[r] = cvt_graw<MODE>(g[i][r], ri[r]);
+          else
+            b[r] = ld_x<MODE>(X, ldx, r * 16, s * KS, ri[r]);
+        }
+#pragma unroll
+        for (int j = 0; j < NTW; j++)
+#pragma unroll
+          for (int r = 0; r < RT; r++) acc[j][r] = mma<MODE>(a[i][j], b[r], acc[j][r]);
+      }
+    }
   }
+}
+
+// gemm_t for a runtime tile count ntw in 1..4 (wave-uniform)
+template <int MODE, int R, bool XG, typename T>
+__device__ inline void gemm_n(const void *W, int t0, int ntw, const T *X, int ldx, const float *G,
+                              int gw, int grow0, int K, const float *rinv, f32x4 (&acc)[4][R / 16])
+{
+  if (ntw >= 4)
+    gemm_t<MODE, R, XG, 4, T>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
+  else if (ntw == 3)
+    gemm_t<MODE, R, XG, 3, T>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
+  else if (ntw == 2)
+    gemm_t<MODE, R, XG, 2, T>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
+  else
+    gemm_t<MODE, R, XG, 1, T>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
 }
 
 }  // namespace
@@ -583,7 +636,9 @@ __global__ __launch_bounds__(256, 3) void k_mlp_fast(RowLayout rl_cap, MlpParams
   const int row0 = blockIdx.x * R;
   if (row0 >= rl.nrows) return;
   const int e = tile_elem(rl, row0);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // wave index made wave-uniform for the compiler: tile bounds and guards
+  // derived from it are scalar branches
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int gw = mp.sizes[0];
   const int L = mp.nl;
   const int cq = 4 * (lane >> 4);  // first of the lane's 4 output columns in a tile
@@ -593,10 +648,8 @@ __global__ __launch_bounds__(256, 3) void k_mlp_fast(RowLayout rl_cap, MlpParams
   // ---------------- forward ----------------
   for (int l = 0; l < L - 1; l++) {
     const int K = mp.sizes[l], N = mp.sizes[l + 1];
-    const void *W = MODE == 0   ? (const void *) (mp.Wf32[l] + (size_t) e * mp.wstride[l])
-                    : MODE == 1 ? (const void *) (mp.Wtb[l] + (size_t) e * mp.wstride[l])
-                                : (const void *) (mp.Wtq8[l] + (size_t) e * mp.wstride[l]);
-    const int lda = MODE == 0 ? N : K;
+    constexpr size_t esz = MODE == 0 ? 4 : (MODE == 1 ? 2 : 1);
+    const void *W = (const unsigned char *) mp.Af[l] + (size_t) e * mp.wstride[l] * esz;
     const float *bias = mp.b[l] ? mp.b[l] + (size_t) e * mp.bstride[l] : nullptr;
     T *H = sm + mp.offH[l];
     T *D = sm + mp.offD[l];
@@ -618,9 +671,9 @@ __global__ __launch_bounds__(256, 3) void k_mlp_fast(RowLayout rl_cap, MlpParams
       if (ntw == 0) continue;
       f32x4 acc[4][RT];
       if (l == 0)
-        gemm_t<MODE, R, true, T>(W, lda, t0, ntw, nullptr, 0, Gd, gw, row0, K, rs + R, acc);
+        gemm_n<MODE, R, true, T>(W, t0, ntw, nullptr, 0, Gd, gw, row0, K, rs + R, acc);
       else
-        gemm_t<MODE, R, false, T>(W, lda, t0, ntw, Hin, ldi, nullptr, gw, row0, K, rs + R, acc);
+        gemm_n<MODE, R, false, T>(W, t0, ntw, Hin, ldi, nullptr, gw, row0, K, rs + R, acc);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= ntw) continue;
@@ -686,10 +739,8 @@ __global__ __launch_bounds__(256, 3) void k_mlp_fast(RowLayout rl_cap, MlpParams
   for (int l = L - 2; l >= 0; l--) {
     const int K = mp.sizes[l], N = mp.sizes[l + 1];
     // Gin^T [K x R] = W_l [K x N] G_l^T [N x R]
-    const void *W = MODE == 0   ? (const void *) (mp.Wt32[l] + (size_t) e * mp.wstride[l])
-                    : MODE == 1 ? (const void *) (mp.Wb16[l] + (size_t) e * mp.wstride[l])
-                                : (const void *) (mp.Wq8[l] + (size_t) e * mp.wstride[l]);
-    const int lda = MODE == 0 ? K : N;
+    constexpr size_t esz = MODE == 0 ? 4 : (MODE == 1 ? 2 : 1);
+    const void *W = (const unsigned char *) mp.Ab[l] + (size_t) e * mp.wstride[l] * esz;
     const T *Gl = sm + mp.offD[l];
     const int ldg = mp.ld[l + 1];
     T *gH = sm + mp.offGH;
@@ -704,7 +755,7 @@ __global__ __launch_bounds__(256, 3) void k_mlp_fast(RowLayout rl_cap, MlpParams
       const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
       if (ntw == 0) continue;
       f32x4 acc[4][RT];
-      gemm_t<MODE, R, false, T>(W, lda, t0, ntw, Gl, ldg, nullptr, gw, row0, N, rs + R, acc);
+      gemm_n<MODE, R, false, T>(W, t0, ntw, Gl, ldg, nullptr, gw, row0, N, rs + R, acc);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= ntw) continue;
@@ -793,8 +844,8 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
   }
 }
 
-__global__ void k_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Wq,
-                            uint8_t *Wtq, int K, int N, int nelt)
+__global__ void k_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Af,
+                            uint8_t *Ab, int K, int N, int nelt)
 {
   const size_t per = (size_t) K * N;
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -803,9 +854,14 @@ __global__ void k_quant_fp8(const float *W, const float *swf, const float *swb, 
   const size_t rem = i - (size_t) el * per;
   const int k = (int) (rem / N), n = (int) (rem - (size_t) k * N);
   const float w = W[i];
-  Wq[i] = (uint8_t) (__builtin_amdgcn_cvt_pk_fp8_f32(w / swb[(size_t) el * K + k], 0.0f, 0, false) & 0xff);
-  Wtq[(size_t) el * per + (size_t) n * K + k] =
+  // forward operand: output n (tile n/16, lane row n%16), inner k (step k/32,
+  // lane group (k%32)/8, byte k%8); backward: output k, inner n
+  const size_t f = (((size_t) (n / 16) * (K / 32) + k / 32) * 64 + ((k % 32) / 8) * 16 + n % 16) * 8 + k % 8;
+  const size_t b = (((size_t) (k / 16) * (N / 32) + n / 32) * 64 + ((n % 32) / 8) * 16 + k % 16) * 8 + n % 8;
+  Af[(size_t) el * per + f] =
       (uint8_t) (__builtin_amdgcn_cvt_pk_fp8_f32(w / swf[(size_t) el * N + n], 0.0f, 0, false) & 0xff);
+  Ab[(size_t) el * per + b] =
+      (uint8_t) (__builtin_amdgcn_cvt_pk_fp8_f32(w / swb[(size_t) el * K + k], 0.0f, 0, false) & 0xff);
 }
 }  // namespace
 
@@ -820,12 +876,13 @@ bool mlp_fast_ok(const MlpParams &mp, int mode)
   return true;
 }
 
-hipError_t launch_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Wq,
-                            uint8_t *Wtq, int K, int N, int nelt, hipStream_t stream)
+hipError_t launch_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Af,
+                            uint8_t *Ab, int K, int N, int nelt, hipStream_t stream)
 {
   size_t total = (size_t) K * N * nelt;
   if (total == 0) return hipSuccess;
-  k_quant_fp8<<<(unsigned) ((total + 255) / 256), 256, 0, stream>>>(W, swf, swb, Wq, Wtq, K, N, nelt);
+  if (K % 32 || N % 32) return hipErrorInvalidValue;
+  k_quant_fp8<<<(unsigned) ((total + 255) / 256), 256, 0, stream>>>(W, swf, swb, Af, Ab, K, N, nelt);
   return hipGetLastError();
 }
 

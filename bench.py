@@ -34,6 +34,8 @@ sys.path.insert(0, os.path.join(ROOT, "lammps-tensormd_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 PEAK = {"hbm_gbs": 8000.0, "fp32_tflops": 157.3, "bf16_tflops": 2500.0}
+KB_EV = 8.617343e-5          # Boltzmann constant, eV/K
+MVV2E = 1.0364269e-4         # (g/mol) (A/ps)^2 -> eV
 
 
 def per_atom_counts(c, *, d, k, b, m, gw, sizes):
@@ -64,6 +66,12 @@ def neighbors_within(system, x, cell, rc):
     i = np.repeat(np.arange(len(off) - 1), np.diff(off))
     d = x[j] + sh @ cell - x[i]
     return float((np.einsum("ij,ij->i", d, d) < rc * rc).sum() / (len(off) - 1))
+
+
+def kinetic_temperature(mass, v):
+    """Kinetic temperature (K) of velocities v (A/ps) with masses (g/mol)."""
+    mv2 = float((np.asarray(mass)[:, None] * np.asarray(v) ** 2).sum())
+    return mv2 * MVV2E / (3.0 * len(mass) * KB_EV)
 
 
 def _traffic(kernel):
@@ -213,7 +221,9 @@ def main():
 
     res = {}
     prof = {}
+    state = {}
     nbr_within_rc = None
+    head = "bf16" if "bf16" in args.precisions.split(",") else args.precisions.split(",")[0]
     for prec in args.precisions.split(","):
         m = TensorMD(model, symbols, prec, 0.001, device=0)
         s = m.system(x, t, cell, skin=1.0)
@@ -230,6 +240,13 @@ def main():
         res[prec] = dict(time=dt, rate=natoms * args.steps / dt, energy=out["energy"],
                          builds=s.nbuilds)
         prof[prec] = s.profile(args.profile_steps, 0.001)
+        if prec == head:
+            # the state the kernel times above were taken in: the random-init
+            # model does not hold the lattice, so the system heats and the
+            # neighbor count within rc grows during the run
+            st = s.get()
+            state = dict(neighbors_within_rc=neighbors_within(s, st["x"], cell, 6.5),
+                         temperature_K=kinetic_temperature(np.array(masses)[t - 1], st["v"]))
         print(f"bench: {prec}: {res[prec]['rate'] / 1e6:.2f} M atom-steps/s, "
               f"{1e3 * dt / args.steps:.3f} ms/step, builds={s.nbuilds}, phases(ms)={prof[prec]}",
               flush=True)
@@ -239,10 +256,9 @@ def main():
     # roofline of the dominant kernel of the headline (bf16) path
     from tensormd_amd import bench_copy
     copy_gbs = bench_copy(1 << 30, 20)
-    head = "bf16" if "bf16" in prof else list(prof)[0]
     p = prof[head]
     phases = {k: p[k] for k in ["neighbor", "descriptor", "mlp", "force", "reduce", "integrate"]}
-    c = nbr_within_rc
+    c = state.get("neighbors_within_rc", nbr_within_rc)
     per = per_atom_counts(c, d=20, k=24, b=4, m=3, gw=384, sizes=sizes)
     dom = max(["descriptor", "mlp", "force"], key=lambda k: phases[k])
     t_dom = p[dom] * 1e-3
@@ -269,6 +285,8 @@ def main():
     roof["measured_copy_gbs"] = copy_gbs
     roof["phases_ms"] = phases
     roof["neighbors_within_rc"] = c
+    roof["neighbors_within_rc_start"] = nbr_within_rc
+    roof["temperature_K_end"] = state.get("temperature_K")
 
     fp8 = None
     if args.c4_steps > 0:

@@ -156,6 +156,14 @@ __device__ inline float t_entry(int m, int d, int tsym)
   return t;
 }
 
+// True when monomials [d0, d0 + n) include one of degree m.
+__host__ __device__ constexpr bool group_has_degree(int d0, int n, int m)
+{
+  for (int d = d0; d < d0 + n && d < kND; d++)
+    if (kDeg[d] == m) return true;
+  return false;
+}
+
 // Cross-lane moves within a row of 16 lanes (DPP: no LDS traffic).
 // quad_perm [1,0,3,2] and [2,3,0,1] swap lanes 2i <-> 2i+1 and i <-> i^2;
 // row_ror:4 and row_ror:8 rotate a row of 16 lanes.
@@ -549,11 +557,15 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
 #pragma unroll
         for (int m = 1; m < kNM; m++) part[m] += tc[m][q] * p2;
       }
-      float tot[kNM] = {0, 0, 0, 0};
+      // sum over the NDG lanes of the filter: only the d groups that hold a
+      // monomial of degree m add to G_m (the others' partial sums are exact
+      // zeros; the T-symmetric extras are in the d group 0, the lane itself)
+      float tot[kNM] = {0, part[1], part[2], part[3]};
 #pragma unroll
-      for (int t = 0; t < T::NDG; t++)
+      for (int t = 1; t < T::NDG; t++)
 #pragma unroll
-        for (int m = 1; m < kNM; m++) tot[m] += __shfl(part[m], gbase + t);
+        for (int m = 1; m < kNM; m++)
+          if (group_has_degree(t * T::DT, T::DT, m)) tot[m] += __shfl(part[m], gbase + t);
       if (lane < T::LANES && dq == 0) {
         const int k = kq * T::KT + a;
         *reinterpret_cast<float4 *>(Grow + ((size_t) b * K + k) * kNM) =

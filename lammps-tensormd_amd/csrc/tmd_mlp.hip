@@ -622,10 +622,15 @@ __device__ inline void gemm_n(const void *W, int t0, int ntw, const T *X, int ld
 
 }  // namespace
 
-// at least 3 waves per SIMD (at most 168 VGPRs): more tiles in flight hide
-// the L2 latency of the weight operands
+// Waves per SIMD asked of the fast kernel (more tiles in flight hide the L2
+// latency of the weight operands): 4 for BF16 and 16-row tiles, 3 for FP32
+// and FP8 32-row tiles, 2 for 64-row tiles (their accumulators and rings
+// need up to 256 VGPRs).
+template <int MODE, int R>
+constexpr int kMlpWaves = R == 64 ? 2 : ((MODE == 1 || R == 16) ? 4 : 3);
+
 template <int MODE, int ACT, int R>
-__global__ __launch_bounds__(256, 3) void k_mlp_fast(RowLayout rl_cap, MlpParams mp, const float *Gd,
+__global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayout rl_cap, MlpParams mp, const float *Gd,
                                                   double *E, float *dEdG, const BatchDims *dyn)
 {
   const RowLayout rl = dyn ? dyn->rl : rl_cap;

@@ -184,9 +184,10 @@ int upload_model(tmd_ctx *ctx)
       if (fast && l < L - 1 && !fp8) {
         // fragment order: [tile t][step s][lane][v]; lane = (group g) * 16 +
         // (row in tile); forward outputs n = 16 t + row, inner k = s KS + g V
-        // + v (V = values per lane: 1 FP32, 8 BF16); backward swaps K and N
+        // + v (KS, V = 16, 4 for FP32, whose four MFMAs per step take the
+        // values v; 32, 8 for BF16); backward swaps K and N
         const bool f32 = mode == TMD_MLP_FP32;
-        const int KS = f32 ? 4 : 32, V = f32 ? 1 : 8;
+        const int KS = f32 ? 16 : 32, V = f32 ? 4 : 8;
         const double *Wd = net.W[l].data();
         auto frag = [&](unsigned char *dst, int out, int inner, bool fwd) {
           const int nst = inner / KS;

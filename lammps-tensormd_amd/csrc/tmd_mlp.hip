@@ -378,10 +378,16 @@ namespace {
 
 template <int MODE>
 struct Frag;
+// FP32: four consecutive inner values per lane, consumed by four 16x16x4
+// MFMAs (lane group g holds inner values 4g .. 4g+3 of a block of 16: the
+// MFMA q sums the values 4g + q, so the four cover the block)
 template <>
 struct Frag<0> {
-  using A = float;
+  using A = f32x4;
 };
+// inner values per fast-path step: 16 (FP32: four MFMAs) or 32
+template <int MODE>
+constexpr int kFastKS = MODE == 0 ? 16 : 32;
 template <>
 struct Frag<1> {
   using A = bf16x8;
@@ -409,7 +415,7 @@ __device__ inline typename Frag<MODE>::A ld_a(const void *A, int t, int s, int n
   const int lane = threadIdx.x & 63;
   const size_t idx = ((size_t) t * nst + s) * 64 + lane;
   if constexpr (MODE == 0)
-    return ((const float *) A)[idx];
+    return reinterpret_cast<const f32x4 *>(A)[idx];
   else if constexpr (MODE == 1)
     return reinterpret_cast<const bf16x8 *>(A)[idx];
   else
@@ -423,7 +429,7 @@ __device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k,
 {
   const int lane = threadIdx.x & 63;
   if constexpr (MODE == 0) {
-    return X[(r0 + (lane & 15)) * ld + k + (lane >> 4)];
+    return *reinterpret_cast<const f32x4 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 4);
   } else if constexpr (MODE == 1) {
     return *reinterpret_cast<const bf16x8 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8);
   } else {
@@ -440,7 +446,7 @@ __device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k,
 
 // B operand from the FP32 descriptor rows in HBM (layer 0), in two parts so
 // that a prefetch ring holds the raw values and converts them only at use:
-// ld_graw loads 8 (1 for FP32) values of row grow + lane%16, cvt_graw makes
+// ld_graw loads 8 (4 for FP32) values of row grow + lane%16, cvt_graw makes
 // the MFMA operand (BF16; FP8 after scaling by the row's inverse scale ri).
 template <int MODE>
 struct GRaw {
@@ -448,7 +454,7 @@ struct GRaw {
 };
 template <>
 struct GRaw<0> {
-  float v;
+  f32x4 v;
 };
 
 template <int MODE>
@@ -458,7 +464,7 @@ __device__ inline GRaw<MODE> ld_graw(const float *G, int gw, int grow, int k)
   const float *p = G + (size_t) (grow + (lane & 15)) * gw + k;
   GRaw<MODE> g;
   if constexpr (MODE == 0) {
-    g.v = p[lane >> 4];
+    g.v = *reinterpret_cast<const f32x4 *>(p + (lane >> 4) * 4);
   } else {
     const float4 *q = reinterpret_cast<const float4 *>(p + (lane >> 4) * 8);
     g.lo = q[0];
@@ -485,12 +491,16 @@ __device__ inline typename Frag<MODE>::A cvt_graw(const GRaw<MODE> &g, float ri)
 template <int MODE>
 __device__ inline f32x4 mma(typename Frag<MODE>::A a, typename Frag<MODE>::A b, f32x4 c)
 {
-  if constexpr (MODE == 0)
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  else if constexpr (MODE == 1)
+  if constexpr (MODE == 0) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], b[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], b[2], c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], b[3], c, 0, 0, 0);
+  } else if constexpr (MODE == 1) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-  else
+  } else {
     return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
+  }
 }
 
 // FP8: per-row scale of the next GEMM's B operand (the rows of X, or of the
@@ -556,8 +566,8 @@ template <int MODE, int R, bool XG, int NTW, typename T>
 __device__ inline void gemm_t(const void *W, int t0, const T *X, int ldx, const float *G, int gw,
                               int grow0, int K, const float *rinv, f32x4 (&acc)[4][R / 16])
 {
-  constexpr int KS = Traits<MODE>::KS, RT = R / 16;
-  constexpr int DW = (MODE == 0 ? 16 : 8) / NTW;
+  constexpr int KS = kFastKS<MODE>, RT = R / 16;
+  constexpr int DW = 8 / NTW;
   constexpr int D = XG ? 2 : (DW >= 2 ? DW : 2);  // layer 0 also rings its raw rows
   using F = typename Frag<MODE>::A;
 #pragma unroll
@@ -873,7 +883,7 @@ __global__ void k_quant_fp8(const float *W, const float *swf, const float *swb, 
 // every hidden width a multiple of 16 and every GEMM inner size a multiple of 2 KS
 bool mlp_fast_ok(const MlpParams &mp, int mode)
 {
-  const int ks2 = mode == 0 ? 8 : 64;
+  const int ks2 = mode == 0 ? 32 : 64;  // two fast-path steps
   for (int l = 0; l < mp.nl - 1; l++) {
     if (mp.sizes[l + 1] % 16 || mp.sizes[l] % 16) return false;
     if (mp.sizes[l] % ks2 || mp.sizes[l + 1] % ks2) return false;

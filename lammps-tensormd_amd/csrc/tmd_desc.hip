@@ -610,25 +610,20 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
         const float4 *trow = tab + (size_t) ng.n * K + sub * KL;
         const float *S = PS + (size_t) b * T::SSTR + sub * KL * kND;
         constexpr int B = kTabBatch<K>;
-        // software pipeline over batches of B filters: interpolate the batch,
-        // which frees its table registers, then load the next batch's rows
-        // into them and run this batch's U/V products while those loads fly
-        float4 t[B];
-#pragma unroll
-        for (int u = 0; u < B; u++) t[u] = trow[u];
+        // batches of B filters: load the table rows, interpolate H and dH/dr,
+        // then U_k = sum_d S_kd M_d (M_0 = 1), V_d += S_kd H_k (V_0 is not
+        // needed: dM_0/du = 0), f1 += U_k dH_k.  No software pipelining: four
+        // waves per SIMD hide the row latency, and a prefetch ring costs
+        // registers and copies (measured slower).
 #pragma unroll 1
         for (int k0 = 0; k0 < KL; k0 += B) {
-          float h[B], dh[B];
+          float4 t[B];
 #pragma unroll
-          for (int u = 0; u < B; u++) kerley(t[u], ng.q, ng.pq, qp, g.dxf, h[u], dh[u]);
-          if (k0 + B < KL) {
-#pragma unroll
-            for (int u = 0; u < B; u++) t[u] = trow[k0 + B + u];
-          }
-          // U_k = sum_d S_kd M_d (M_0 = 1), V_d += S_kd H_k (V_0 is not
-          // needed: dM_0/du = 0), f1 += U_k dH_k
+          for (int u = 0; u < B; u++) t[u] = trow[k0 + u];
 #pragma unroll
           for (int u = 0; u < B; u++) {
+            float h, dh;
+            kerley(t[u], ng.q, ng.pq, qp, g.dxf, h, dh);
             const float4 *Sk = reinterpret_cast<const float4 *>(S + (k0 + u) * kND);
             float sv[kND];
 #pragma unroll
@@ -643,8 +638,8 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
 #pragma unroll
             for (int q = 1; q < kND; q++) U = fmaf(sv[q], M[q], U);
 #pragma unroll
-            for (int q = 1; q < kND; q++) V[q] = fmaf(sv[q], h[u], V[q]);
-            f1 = fmaf(U, dh[u], f1);
+            for (int q = 1; q < kND; q++) V[q] = fmaf(sv[q], h, V[q]);
+            f1 = fmaf(U, dh, f1);
           }
         }
       }

@@ -609,12 +609,14 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
         const float qp = ng.q - ng.pq;
         const float4 *trow = tab + (size_t) ng.n * K + sub * KL;
         const float *S = PS + (size_t) b * T::SSTR + sub * KL * kND;
-        constexpr int B = kTabBatch<K>;
         // batches of B filters: load the table rows, interpolate H and dH/dr,
         // then U_k = sum_d S_kd M_d (M_0 = 1), V_d += S_kd H_k (V_0 is not
         // needed: dM_0/du = 0), f1 += U_k dH_k.  No software pipelining: four
         // waves per SIMD hide the row latency, and a prefetch ring costs
-        // registers and copies (measured slower).
+        // registers and copies (measured slower).  Six rows per batch where
+        // they divide the lane's filters (K = 24: 3 % faster than 3 or 8 rows;
+        // 12 rows spill), else the staging batch.
+        constexpr int B = KL % 6 == 0 ? 6 : kTabBatch<K>;
 #pragma unroll 1
         for (int k0 = 0; k0 < KL; k0 += B) {
           float4 t[B];

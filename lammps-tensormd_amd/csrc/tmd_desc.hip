@@ -421,12 +421,27 @@ __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &
     constexpr int B = kTabBatch<K>;
     if (table_cell_live(g, ng.n)) {
       const float4 *row = tab + (size_t) ng.n * K + part * KL;
-      for (int k0 = 0; k0 < KL; k0 += B) {
-        float4 t[B];
+      // six rows per wait where they divide the lane's filters: the staging
+      // is latency-bound, and each wait covering more loads cuts the
+      // descriptor and the P rebuild by 7-8 % at K = 24 (twelve rows raise
+      // the descriptor to 143 VGPRs and lose)
+      constexpr int B6 = (K / 2) % 6 == 0 ? 6 : B;
+      if (KL % B6 == 0) {
+        for (int k0 = 0; k0 < KL; k0 += B6) {
+          float4 t[B6];
 #pragma unroll
-        for (int u = 0; u < B; u++) t[u] = row[k0 + u];
+          for (int u = 0; u < B6; u++) t[u] = row[k0 + u];
 #pragma unroll
-        for (int u = 0; u < B; u++) Hc[k0 + u] = kerley_h(t[u], ng.q, ng.pq);
+          for (int u = 0; u < B6; u++) Hc[k0 + u] = kerley_h(t[u], ng.q, ng.pq);
+        }
+      } else {
+        for (int k0 = 0; k0 < KL; k0 += B) {
+          float4 t[B];
+#pragma unroll
+          for (int u = 0; u < B; u++) t[u] = row[k0 + u];
+#pragma unroll
+          for (int u = 0; u < B; u++) Hc[k0 + u] = kerley_h(t[u], ng.q, ng.pq);
+        }
       }
     } else {
       for (int k = 0; k < KL; k++) Hc[k] = 0.0f;

@@ -263,46 +263,60 @@ __device__ inline int kept_slot(const ModelGeo &g, const Box &cell, const double
                                     : -1;  // kernel.cu:1163
 }
 
+// Lists of at most 64 NCH entries: every entry's slot stays in registers.
+// All loads are unconditional, with indices clamped into the row (and to
+// atom 0 past its end), and issued before any is used: exec-masked loads
+// make the compiler wait for each one at its branch join.
+template <int NCH>
+__device__ inline int compact_short(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
+                                    const uint32_t *list, int nraw, const double4 *pos,
+                                    uint32_t *kept, int *start)
+{
+  const int lane = threadIdx.x & 63;
+  const int last = max(nraw - 1, 0);
+  uint32_t ent[NCH];
+  double4 xj[NCH];
+  int slot[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) {
+    const int idx = ch * 64 + lane;
+    const uint32_t e = list[min(idx, last)];
+    ent[ch] = idx < nraw ? e : 0u;
+  }
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) xj[ch] = pos[ent[ch] & kNbrJMask];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) {
+    const int sl = kept_slot(g, cell, xi, ei, ent[ch], xj[ch]);
+    slot[ch] = ch * 64 + lane < nraw ? sl : -1;
+  }
+  int s = 0;
+  for (int b = 0; b < g.nelt; b++) {
+    start[b] = s;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++) s += __popcll(__ballot(slot[ch] == b));
+  }
+  start[g.nelt] = s;
+  for (int b = 0; b < g.nelt; b++) {
+    int run = start[b];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++) {
+      unsigned long long m = __ballot(slot[ch] == b);
+      if (slot[ch] == b) kept[run + (int) lane_rank(m)] = ent[ch];
+      run += __popcll(m);
+    }
+  }
+  return s;
+}
+
 __device__ inline int compact(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
                               const uint32_t *list, int nraw, const double4 *pos, uint32_t *kept,
                               int *start)
 {
   const int lane = threadIdx.x & 63;
-  constexpr int NCH = 4;
-  if (nraw <= 64 * NCH) {
-    // all loads first, then the slot of every entry stays in registers
-    uint32_t ent[NCH];
-    double4 xj[NCH];
-    int slot[NCH];
-#pragma unroll
-    for (int ch = 0; ch < NCH; ch++) {
-      int idx = ch * 64 + lane;
-      ent[ch] = idx < nraw ? list[idx] : 0u;
-    }
-#pragma unroll
-    for (int ch = 0; ch < NCH; ch++)
-      if (ch * 64 + lane < nraw) xj[ch] = pos[ent[ch] & kNbrJMask];
-#pragma unroll
-    for (int ch = 0; ch < NCH; ch++)
-      slot[ch] = (ch * 64 + lane < nraw) ? kept_slot(g, cell, xi, ei, ent[ch], xj[ch]) : -1;
-    int s = 0;
-    for (int b = 0; b < g.nelt; b++) {
-      start[b] = s;
-#pragma unroll
-      for (int ch = 0; ch < NCH; ch++) s += __popcll(__ballot(slot[ch] == b));
-    }
-    start[g.nelt] = s;
-    for (int b = 0; b < g.nelt; b++) {
-      int run = start[b];
-#pragma unroll
-      for (int ch = 0; ch < NCH; ch++) {
-        unsigned long long m = __ballot(slot[ch] == b);
-        if (slot[ch] == b) kept[run + (int) lane_rank(m)] = ent[ch];
-        run += __popcll(m);
-      }
-    }
-    return s;
-  }
+  if (nraw <= 128) return compact_short<2>(g, cell, xi, ei, list, nraw, pos, kept, start);
+  if (nraw <= 192) return compact_short<3>(g, cell, xi, ei, list, nraw, pos, kept, start);
+  if (nraw <= 256) return compact_short<4>(g, cell, xi, ei, list, nraw, pos, kept, start);
   // long lists: two sweeps
   int cnt[kMaxElts];
   for (int b = 0; b < kMaxElts; b++) cnt[b] = 0;

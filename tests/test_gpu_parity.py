@@ -65,6 +65,29 @@ def test_system_c2_shape(synth_models):
     assert e["dV"] <= 1e-6
 
 
+@pytest.mark.parametrize("skin,lo,hi", [(2.0, 128, 192), (3.0, 192, 256), (4.5, 256, 400)])
+def test_compaction_paths(synth_models, skin, lo, hi):
+    """Raw lists of ~160, ~215 and ~333 entries per atom (rc + skin = 8.5, 9.5,
+    11 A): the descriptor's 3- and 4-batch compaction and its two-sweep path
+    for lists over 256 entries.  The kept set, and so the result, does not
+    depend on the skin."""
+    say(f"case: compaction path, C2 model, bcc 8x8x8, skin {skin}")
+    om = oracle.OracleModel(synth_models["c2"], 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (8, 8, 8), nelt=4, sigma=0.08, seed=28)
+    ref = om.compute_periodic(x, t - 1, cell)
+    m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)
+    s = m.system(x, t, cell, skin=skin)
+    got = s.compute()
+    n = np.diff(s.neighbors()[0])
+    say("raw list entries per atom", int(n.min()), int(n.max()))
+    assert lo < n.min() and n.max() <= hi
+    e = _errs(got, ref, len(x))
+    say("errors", e)
+    assert e["dF"] <= F_TOL
+    assert e["dE"] <= E_TOL
+    assert e["dV"] <= 1e-6
+
+
 def test_neighbor_set_bit_exact(synth_models):
     say("case: neighbor set, bcc 10x9x8 tilted cell")
     m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)

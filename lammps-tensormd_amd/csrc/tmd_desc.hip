@@ -6,7 +6,8 @@
 //     (gpu/cuda/kernel.cu:1163) and finds the element slot b
 //     (tensormd.cpp:165-176).  Kept entries are compacted into LDS sorted
 //     by slot (ballot prefix counts), so later passes run on full lanes.
-//  2. For each chunk of 64 kept neighbors: geometry u = d/r (FP32 from FP64),
+//  2. For each chunk of 32 kept neighbors (two lanes per neighbor; 4 or 8
+//     for a short last chunk): geometry u = d/r (FP32 from FP64),
 //     monomials M_d(u), the Kerley-interpolated radial values H_k(r)
 //     (gpu/cuda/ration.cu:13-71) with lanes over (neighbor, filter), then
 //     P[b][k][d] += H_k M_d accumulated in registers: each lane owns a

@@ -198,6 +198,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--c4-steps", type=int, default=200)
+    # every N-th timed step of the headline precision is launched kernel by
+    # kernel with HIP events around the kernels (0: off; the roofline then
+    # uses the kernel times of the profile steps after the run)
+    ap.add_argument("--kernel-timing-every", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -222,6 +226,8 @@ def main():
     res = {}
     prof = {}
     state = {}
+    ktime = {}
+    kt_every = args.kernel_timing_every
     nbr_within_rc = None
     head = "bf16" if "bf16" in args.precisions.split(",") else args.precisions.split(",")[0]
     for prec in args.precisions.split(","):
@@ -233,10 +239,18 @@ def main():
             nbr_within_rc = neighbors_within(s, s.get()["x"], cell, 6.5)
         s.md_run(args.warmup, 0.001, history=False)
         s.get()  # synchronise
+        if prec == head and kt_every > 0:
+            # kernel durations over the timed region: every kt_every-th step
+            # is launched kernel by kernel with HIP events on the library
+            # stream (the others replay the captured step graph)
+            s.set_kernel_timing(kt_every)
         t0 = time.perf_counter()
         s.md_run(args.steps, 0.001, history=False)
         out = s.get()  # synchronises the stream
         dt = time.perf_counter() - t0
+        if prec == head and kt_every > 0:
+            ktime[prec] = s.kernel_timing()
+            s.set_kernel_timing(0)
         res[prec] = dict(time=dt, rate=natoms * args.steps / dt, energy=out["energy"],
                          builds=s.nbuilds)
         prof[prec] = s.profile(args.profile_steps, 0.001)
@@ -258,10 +272,20 @@ def main():
     copy_gbs = bench_copy(1 << 30, 20)
     p = prof[head]
     phases = {k: p[k] for k in ["neighbor", "descriptor", "mlp", "force", "reduce", "integrate"]}
-    c = state.get("neighbors_within_rc", nbr_within_rc)
+    if head in ktime:
+        kt = ktime[head]
+        timing = (f"HIP events on the library stream around the kernels of every {kt_every}th "
+                  f"step of the timed run")
+    else:
+        kt = dict(descriptor=p["descriptor"], mlp=p["mlp"], force=p["force"],
+                  kept_per_atom=state.get("neighbors_within_rc", nbr_within_rc),
+                  samples=args.profile_steps)
+        timing = "HIP events around the kernels of the profile steps after the timed run"
+    # work per launch at the sampled steps: their mean kept-neighbor count
+    c = kt["kept_per_atom"]
     per = per_atom_counts(c, d=20, k=24, b=4, m=3, gw=384, sizes=sizes)
-    dom = max(["descriptor", "mlp", "force"], key=lambda k: phases[k])
-    t_dom = p[dom] * 1e-3
+    dom = max(["descriptor", "mlp", "force"], key=lambda k: kt[k])
+    t_dom = kt[dom] * 1e-3
     if dom == "mlp":
         roof = dict(kernel="k_mlp (fused per-element MLP fwd+bwd, BF16 MFMA)", bound="mfma",
                     achieved=per["mlp_flop"] * natoms / t_dom / 1e12, peak=PEAK["bf16_tflops"],
@@ -275,7 +299,10 @@ def main():
         roof["hbm_achieved_gbs"] = per[dom + "_bytes"] * natoms / t_dom / 1e9
         roof["hbm_peak_gbs"] = PEAK["hbm_gbs"]
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["avg_launch_ms"] = p[dom]
+    roof["avg_launch_ms"] = kt[dom]
+    roof["avg_launch_samples"] = kt["samples"]
+    roof["timing"] = timing
+    roof["kernel_ms_timed_run"] = {k: kt[k] for k in ("descriptor", "mlp", "force")}
     roof["algorithmic_per_atom"] = {k: v for k, v in per.items() if k.startswith(dom)}
     roof["traffic"] = None
     tr = _traffic(dom)
@@ -283,9 +310,10 @@ def main():
         roof["traffic"] = tr["bytes_per_launch"]
         roof["traffic_source"] = tr["source"]
     roof["measured_copy_gbs"] = copy_gbs
-    roof["phases_ms"] = phases
+    roof["phases_ms_end_state"] = phases
     roof["neighbors_within_rc"] = c
     roof["neighbors_within_rc_start"] = nbr_within_rc
+    roof["neighbors_within_rc_end"] = state.get("neighbors_within_rc")
     roof["temperature_K_end"] = state.get("temperature_K")
 
     fp8 = None

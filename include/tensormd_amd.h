@@ -128,6 +128,19 @@ int tmd_md_run(tmd_system *sys, int nsteps, double dt, double *epot, double *eki
  * reductions (with the force-buffer clears), integration, and the whole
  * step. */
 int tmd_md_profile(tmd_system *sys, int nsteps, double dt, double *ms);
+/* Sampled kernel timing inside tmd_md_run.  No counterpart in the reference
+ * (benchmark instrumentation: kernel durations measured over a timed run).
+ * every > 0: every every-th step of the following tmd_md_run calls is
+ * launched kernel by kernel with HIP events on the library stream around
+ * the descriptor, MLP and force kernels, the other steps replay the
+ * captured graph; every = 0 turns it off.  Clears the samples. */
+int tmd_system_set_kernel_timing(tmd_system *sys, int every);
+/* ms[0..2] = mean ms of the descriptor, MLP and force kernels over the
+ * sampled steps; *kept = mean kept neighbors (within rc) per atom at those
+ * steps (NULL allowed); *nsamples = number of sampled steps (NULL allowed).
+ * Steps of a run segment that is rerun after a list overflow are not
+ * counted. */
+int tmd_system_kernel_timing(const tmd_system *sys, double *ms, double *kept, long long *nsamples);
 /* Replay each MD step as one captured HIP graph (default 1) or launch its
  * kernels one by one (0); results are the same either way. */
 int tmd_system_set_graph(tmd_system *sys, int enable);

@@ -248,6 +248,10 @@ hipError_t launch_vv_second(int n, const double4 *pos, double *v, const double *
 hipError_t launch_ekin(int n, const double4 *pos, const double *v, const double *mass_by_elem,
                        double *out, hipStream_t stream);
 // per-step energy history through a device counter (safe to replay in a graph)
+// *acc += sum of the kept-neighbor counts of the real rows (row2atom >= 0)
+// in the kept-list cache kl (words per row)
+hipError_t launch_sum_kept(const uint32_t *kl, size_t words, const int *row2atom, int nrows,
+                           unsigned long long *acc, hipStream_t s);
 hipError_t launch_record_step(const double *epot, const double *ekin, double *hist,
                               unsigned long long *counter, int cap, hipStream_t stream);
 // copy e (per-step scalars) into history slots

@@ -34,6 +34,15 @@ struct tmd_system {
   hipGraphExec_t step_graph = nullptr;
   bool graph_hist = false;
   bool use_graph = true;
+  // sampled kernel timing inside tmd_md_run (tmd_system_set_kernel_timing):
+  // every time_every-th step is launched kernel by kernel with events around
+  // the descriptor, MLP and force kernels
+  int time_every = 0;
+  std::vector<PhaseEvents> tev;  // one set per sampled step of a run segment
+  DeviceBuffer tkept;            // kept-neighbor count summed over a segment's samples
+  double tsum[3] = {0, 0, 0};    // descriptor, MLP, force kernel ms summed over the samples
+  double tkept_sum = 0;          // kept neighbors summed over the samples (all atoms)
+  long long tsamples = 0;
 };
 
 namespace tmd {
@@ -144,12 +153,17 @@ Chain sys_chain(tmd_system *s)
   return c;
 }
 
-int sys_forces(tmd_system *s)
+int sys_forces(tmd_system *s, PhaseEvents *pe = nullptr)
 {
   hipStream_t st = s->ctx->stream;
   hipError_t e = nbr_step(s->nb, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p, (int *) s->w.nn.p, st);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "neighbor build");
-  return run_chain(s->ctx, s->w, sys_chain(s), st);
+  int rc = run_chain(s->ctx, s->w, sys_chain(s), st, pe);
+  if (rc != TMD_OK || !pe) return rc;
+  // the kept-neighbor counts of this evaluation, for the sampled work per launch
+  e = launch_sum_kept((const uint32_t *) s->w.kl.p, kept_cache_words(s->nb.cap),
+                      (const int *) s->w.row2atom.p, s->w.rl.nrows, (unsigned long long *) s->tkept.p, st);
+  return e == hipSuccess ? TMD_OK : hip_fail(s->ctx, e, "kept count");
 }
 
 // overflow check after the queued work; grows the list and asks for a rebuild
@@ -292,6 +306,8 @@ void tmd_system_destroy(tmd_system *s)
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   if (s->step_graph) hipGraphExecDestroy(s->step_graph);
+  for (auto &pe : s->tev)
+    for (auto &ev : pe.ev) hipEventDestroy(ev);
   delete s;
 }
 
@@ -406,12 +422,12 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
   const int kHist = 4096;
   double *ekin_out = hist + 2 * kHist;
   unsigned long long *counter = (unsigned long long *) (hist + 2 * kHist + 4096);
-  auto enqueue_step = [&]() -> int {
+  auto enqueue_step = [&](PhaseEvents *pe) -> int {
     hipError_t er;
     if ((er = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
                               (const double *) s->dtfm.p, dt, st)) != hipSuccess)
       return hip_fail(s->ctx, er, "vv first");
-    int rc = sys_forces(s);
+    int rc = sys_forces(s, pe);
     if (rc != TMD_OK) return rc;
     if ((er = launch_vv_second(s->n, (const double4 *) s->w.pos.p, (double *) s->vel.p,
                                (const double *) s->w.f.p, (const double *) s->dtfm.p, st)) != hipSuccess)
@@ -433,7 +449,7 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
     hipGraph_t graph;
     if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
       return hip_fail(s->ctx, e, "graph capture");
-    int rc = enqueue_step();
+    int rc = enqueue_step(nullptr);
     e = hipStreamEndCapture(st, &graph);
     if (rc != TMD_OK) return rc;
     if (e != hipSuccess) return hip_fail(s->ctx, e, "graph capture");
@@ -450,6 +466,18 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
   // list and reruns the segment from the snapshot, so no step ever uses a
   // truncated list.
   const int kSeg = 200;
+  const int every = s->time_every;
+  if (every > 0) {
+    const size_t need = (size_t) (kSeg + every - 1) / every;
+    while (s->tev.size() < need) {
+      PhaseEvents pe;
+      for (auto &ev : pe.ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return hip_fail(s->ctx, e, "timing events");
+      s->tev.push_back(pe);
+    }
+    if ((e = s->tkept.ensure(sizeof(unsigned long long))) != hipSuccess)
+      return hip_fail(s->ctx, e, "timing counter");
+  }
   const size_t xb = sizeof(double4) * (size_t) s->n, vb = 24 * (size_t) s->n;
   if ((e = s->snap.ensure(xb + 2 * vb)) != hipSuccess) return hip_fail(s->ctx, e, "snapshot alloc");
   unsigned char *snap = (unsigned char *) s->snap.p;
@@ -474,23 +502,50 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
         if ((e = hipMemcpyAsync(counter, &zero, sizeof(zero), hipMemcpyHostToDevice, st)) != hipSuccess)
           return hip_fail(s->ctx, e, "history reset");
       }
+      if (every > 0) {
+        unsigned long long zero = 0ull;
+        if ((e = hipMemcpyAsync(s->tkept.p, &zero, sizeof(zero), hipMemcpyHostToDevice, st)) != hipSuccess)
+          return hip_fail(s->ctx, e, "timing reset");
+      }
+      int nsamp = 0;
       for (int q = 0; q < nrun; q++) {
-        if (s->use_graph) {
+        if (every > 0 && (done + q) % every == 0) {  // a sampled step, kernel by kernel
+          int rc = enqueue_step(&s->tev[(size_t) nsamp++]);
+          if (rc != TMD_OK) return rc;
+        } else if (s->use_graph) {
           if ((e = hipGraphLaunch(s->step_graph, st)) != hipSuccess) return hip_fail(s->ctx, e, "graph launch");
         } else {
-          int rc = enqueue_step();
+          int rc = enqueue_step(nullptr);
           if (rc != TMD_OK) return rc;
         }
       }
       int rc = check_overflow(s, &over);  // grows the list and requests a rebuild
       if (rc != TMD_OK) return rc;
+      if (!over && nsamp > 0) {  // the segment stands: keep its samples (the stream is idle)
+        auto el = [&](hipEvent_t a, hipEvent_t b) {
+          float t = 0.f;
+          hipEventElapsedTime(&t, a, b);
+          return (double) t;
+        };
+        for (int j = 0; j < nsamp; j++) {
+          const PhaseEvents &pe = s->tev[(size_t) j];
+          s->tsum[0] += el(pe.ev[0], pe.ev[1]);
+          s->tsum[1] += el(pe.ev[1], pe.ev[2]);
+          s->tsum[2] += el(pe.ev[3], pe.ev[4]);
+        }
+        unsigned long long kept = 0;
+        if ((e = hipMemcpy(&kept, s->tkept.p, sizeof(kept), hipMemcpyDeviceToHost)) != hipSuccess)
+          return hip_fail(s->ctx, e, "timing counter copy");
+        s->tkept_sum += (double) kept;
+        s->tsamples += nsamp;
+      }
       if (over) {
         if ((e = copy3(false)) != hipSuccess) return hip_fail(s->ctx, e, "snapshot restore");
         if (s->use_graph && !s->step_graph) {  // the list grew: capture again
           hipGraph_t graph;
           if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
             return hip_fail(s->ctx, e, "graph capture");
-          rc = enqueue_step();
+          rc = enqueue_step(nullptr);
           e = hipStreamEndCapture(st, &graph);
           if (rc != TMD_OK) return rc;
           if (e != hipSuccess) return hip_fail(s->ctx, e, "graph capture");
@@ -515,6 +570,26 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
     }
     done += nrun;
   }
+  return TMD_OK;
+}
+
+int tmd_system_set_kernel_timing(tmd_system *s, int every)
+{
+  if (!s || every < 0) return TMD_ERR_ARG;
+  s->time_every = every;
+  s->tsum[0] = s->tsum[1] = s->tsum[2] = 0;
+  s->tkept_sum = 0;
+  s->tsamples = 0;
+  return TMD_OK;
+}
+
+int tmd_system_kernel_timing(const tmd_system *s, double *ms, double *kept, long long *nsamples)
+{
+  if (!s || !ms) return TMD_ERR_ARG;
+  const double n = s->tsamples > 0 ? (double) s->tsamples : 1.0;
+  for (int k = 0; k < 3; k++) ms[k] = s->tsum[k] / n;
+  if (kept) kept[0] = s->tkept_sum / (n * (double) s->n);
+  if (nsamples) nsamples[0] = s->tsamples;
   return TMD_OK;
 }
 

@@ -227,6 +227,27 @@ hipError_t launch_ekin(int n, const double4 *pos, const double *v, const double 
   return hipGetLastError();
 }
 
+// Sum of the kept-neighbor counts over the real rows (the first word of
+// each row of the descriptor's kept-list cache), added to *acc: one atomic
+// per wave.
+__global__ void k_sum_kept(const uint32_t *kl, size_t words, const int *row2atom, int nrows,
+                           unsigned long long *acc)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (r < nrows && row2atom[r] >= 0) v = kl[(size_t) r * words];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc, v);
+}
+
+hipError_t launch_sum_kept(const uint32_t *kl, size_t words, const int *row2atom, int nrows,
+                           unsigned long long *acc, hipStream_t s)
+{
+  if (nrows <= 0) return hipSuccess;
+  k_sum_kept<<<(nrows + 255) / 256, 256, 0, s>>>(kl, words, row2atom, nrows, acc);
+  return hipGetLastError();
+}
+
 hipError_t launch_record_step(const double *epot, const double *ekin, double *hist,
                               unsigned long long *counter, int cap, hipStream_t s)
 {

@@ -99,6 +99,8 @@ def lib():
     L.tmd_system_nbuilds.argtypes = [_vp]
     L.tmd_md_profile.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp]
     L.tmd_system_set_graph.argtypes = [_vp, ctypes.c_int]
+    L.tmd_system_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
+    L.tmd_system_kernel_timing.argtypes = [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.tmd_bench_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
     L.tmd_device_memory.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     L.tmd_system_neighbors.argtypes = [_vp, ctypes.c_long, _lp, _ip, _ip, _lp]
@@ -297,6 +299,23 @@ class System:
 
     def set_graph(self, enable):
         _check(lib().tmd_system_set_graph(self.h, 1 if enable else 0), self.model.h)
+
+    def set_kernel_timing(self, every):
+        """Sample every `every`-th step of later md_run calls kernel by kernel,
+        with HIP events around the descriptor, MLP and force kernels (0: off).
+        Clears the samples."""
+        _check(lib().tmd_system_set_kernel_timing(self.h, int(every)), self.model.h)
+
+    def kernel_timing(self):
+        """Mean kernel ms over the sampled steps, the mean kept neighbors per
+        atom at those steps, and the number of samples."""
+        ms = np.zeros(3)
+        kept = np.zeros(1)
+        n = ctypes.c_longlong()
+        _check(lib().tmd_system_kernel_timing(self.h, _p(ms, _dp), _p(kept, _dp), ctypes.byref(n)),
+               self.model.h)
+        return dict(descriptor=float(ms[0]), mlp=float(ms[1]), force=float(ms[2]),
+                    kept_per_atom=float(kept[0]), samples=int(n.value))
 
     def profile(self, nsteps, dt):
         """Mean ms per step of each phase, timed with HIP events on the library stream."""

@@ -88,6 +88,32 @@ def test_compaction_paths(synth_models, skin, lo, hi):
     assert e["dV"] <= 1e-6
 
 
+def test_kernel_timing_samples(synth_models):
+    """Sampled kernel timing inside md_run: every 5th of 20 steps launched
+    kernel by kernel with events; the trajectory is the one of the graph
+    replay, and the sampled kept-neighbor mean is the count within rc."""
+    say("case: sampled kernel timing in md_run, C2 model, bcc 6x6x6")
+    x, t, cell = W.crystal("bcc", 3.2, (6, 6, 6), nelt=4, sigma=0.08, seed=33)
+    m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)
+    v = W.maxwell_boltzmann(np.array([95.95, 92.906, 180.95, 183.84])[t - 1], 300.0, seed=34)
+    outs, kt = [], None
+    for every in (0, 5):
+        s = m.system(x, t, cell, skin=1.0)
+        s.set_velocities(v)
+        s.set_kernel_timing(every)
+        s.md_run(20, 0.001, history=False)
+        outs.append(s.get())
+        kt = s.kernel_timing()
+        s.close()
+    say("timing", kt)
+    assert kt["samples"] == 4
+    assert all(kt[k] > 0 for k in ("descriptor", "mlp", "force"))
+    ro, _, _ = oracle.neighbors(outs[1]["x"], cell, m.cutoff)
+    assert abs(kt["kept_per_atom"] - np.diff(ro).mean()) < 1.0
+    assert abs(outs[0]["x"] - outs[1]["x"]).max() < 1e-9
+    assert abs(outs[0]["f"] - outs[1]["f"]).max() < 1e-7
+
+
 def test_neighbor_set_bit_exact(synth_models):
     say("case: neighbor set, bcc 10x9x8 tilted cell")
     m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)

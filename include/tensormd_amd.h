@@ -85,7 +85,9 @@ int tmd_cutoff(const tmd_ctx *ctx, double *rc);
 int tmd_masses(const tmd_ctx *ctx, double *mass_by_type);
 /* map[t] = model element index for type t (0-based), -1 if unmapped */
 int tmd_type_map(const tmd_ctx *ctx, int *map);
-/* nelt, num_filters, max_moment, descriptor width, mlp layer count */
+/* nelt, num_filters, max_moment, descriptor width, mlp layer count, as in the
+ * model file (max_moment 0..3; a model with fewer than 4 moments is evaluated
+ * with zero first-layer weights on the moment channels it lacks) */
 int tmd_model_info(const tmd_ctx *ctx, int *info5);
 
 /* ---- Pair::compute drop-in ------------------------------------------------
@@ -151,7 +153,8 @@ int tmd_system_nbuilds(const tmd_system *sys);
  * entry count in *total.  Order within an atom is unspecified. */
 int tmd_system_neighbors(tmd_system *sys, long cap, long *off, int *j, int *shift, long *total);
 /* Descriptor rows G[natoms][width] and dE/dG of the last compute (FP32,
- * atom order), for kernel-level parity tests. */
+ * atom order, width and [b][k][m] layout of the model file), for
+ * kernel-level parity tests. */
 int tmd_system_descriptors(tmd_system *sys, float *G, float *dEdG);
 
 /* ---- serving: batched single-point requests --------------------------------

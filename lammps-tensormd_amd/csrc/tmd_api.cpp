@@ -478,8 +478,8 @@ int tmd_model_info(const tmd_ctx *ctx, int *info)
   if (!ctx || !info) return TMD_ERR_ARG;
   info[0] = ctx->model.nelt;
   info[1] = ctx->model.k;
-  info[2] = ctx->model.max_moment;
-  info[3] = ctx->model.gwidth();
+  info[2] = ctx->model.max_moment_file;
+  info[3] = ctx->model.gwidth_file();
   info[4] = ctx->model.mlp[0].nlayers;
   return TMD_OK;
 }

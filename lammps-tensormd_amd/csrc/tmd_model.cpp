@@ -95,6 +95,29 @@ void build_T(Model &m)
   }
 }
 
+// A max_moment < 3 model as the equivalent max_moment-3 model: the
+// descriptor channels G[b][k][m] with m > max_moment get zero first-layer
+// weights, so they add nothing to any activation and receive dE/dG = 0 (no
+// force either); the input index (b k_num + k)(max_moment + 1) + m of the
+// file (tensormd.cpp:193-196) becomes (b k_num + k) 4 + m.  T is then the
+// max_moment-3 table (its extra rows only meet the zero weights).
+void pad_moments(Model &m)
+{
+  const int nm0 = m.max_moment + 1, rows = m.nelt * m.k;
+  for (DenseNet &d : m.mlp) {
+    const int N = d.sizes[1];
+    std::vector<double> W((size_t) rows * 4 * N, 0.0);
+    for (int r = 0; r < rows; r++)
+      for (int mo = 0; mo < nm0; mo++)
+        for (int j = 0; j < N; j++)
+          W[((size_t) r * 4 + mo) * N + j] = d.W[0][((size_t) r * nm0 + mo) * N + j];
+    d.W[0] = std::move(W);
+    d.sizes[0] = rows * 4;
+  }
+  m.max_moment = 3;
+  build_T(m);
+}
+
 void build_table(Model &m, double delta)
 {
   std::vector<double> xs;
@@ -169,8 +192,9 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
     m.rc = getd("rmax").at(0);
     if (!(m.rc > 0)) throw std::string("rmax must be > 0");
     m.max_moment = geti("max_moment", 0, true);
-    if (m.max_moment != 3)
-      throw std::string("this build evaluates max_moment == 3 models (all shipped models)");
+    if (m.max_moment < 0 || m.max_moment > 3)
+      throw std::string("this build evaluates max_moment 0..3 models (the shipped models use 3)");
+    m.max_moment_file = m.max_moment;
     m.fctype = geti("fctype", 0, true);
     m.T_sym = geti("is_T_symmetric", 0, false) == 1;
     m.fp32_model = geti("precision", 64, false) == 32;
@@ -251,6 +275,7 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
       }
       m.mlp.push_back(std::move(d));
     }
+    if (m.max_moment < 3) pad_moments(m);
     if (!(dx > 0)) throw std::string("interp dx must be > 0");
     if (m.rc / dx > 4.0e6) throw std::string("interp dx too small");
     build_table(m, dx);

@@ -24,7 +24,8 @@ struct DenseNet {
 struct Model {
   int nelt = 0;
   double rc = 0;
-  int max_moment = 0;
+  int max_moment = 0;       // as evaluated (3: smaller ones are padded)
+  int max_moment_file = 0;  // as in the model file
   int nm = 0;  // max_moment + 1
   int nd = 0;  // monomials
   int k = 0;   // num_filters
@@ -42,6 +43,8 @@ struct Model {
   double dx = 0, xdx = 0;
   std::vector<double> tab;
   int gwidth() const { return nelt * k * nm; }
+  // descriptor width of the model file (max_moment_file + 1 moments)
+  int gwidth_file() const { return nelt * k * (max_moment_file + 1); }
 };
 
 // Reads `path`, validates the schema and builds T, slots and the table for

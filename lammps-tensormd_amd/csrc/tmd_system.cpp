@@ -703,8 +703,15 @@ int tmd_system_descriptors(tmd_system *s, float *G, float *dEdG)
   for (int r = 0; r < nr; r++) {
     int a = s->row2atom[r];
     if (a < 0) continue;
-    if (G) std::memcpy(G + (size_t) a * gw, g.data() + (size_t) r * gw, 4 * gw);
-    if (dEdG) std::memcpy(dEdG + (size_t) a * gw, dg.data() + (size_t) r * gw, 4 * gw);
+    // in the model file's layout [b][k][max_moment_file + 1] (a model with
+    // fewer moments is evaluated with its missing channels padded)
+    const int nmf = s->ctx->model.max_moment_file + 1, nm = s->ctx->model.nm;
+    const int gwf = s->ctx->model.gwidth_file();
+    for (int q = 0; q < gwf; q++) {
+      const size_t src = (size_t) r * gw + (size_t) (q / nmf) * nm + q % nmf;
+      if (G) G[(size_t) a * gwf + q] = g[src];
+      if (dEdG) dEdG[(size_t) a * gwf + q] = dg[src];
+    }
   }
   return TMD_OK;
 }

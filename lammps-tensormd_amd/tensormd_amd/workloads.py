@@ -26,7 +26,8 @@ def _symbol_codes(symbols):
 
 
 def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn="tanh",
-                fnn_hidden=None, fnn_actfn="softplus", max_moment=3, use_resnet_dt=False,
+                fnn_hidden=None, fnn_actfn="softplus", max_moment=3, t_symmetric=False,
+                use_resnet_dt=False,
                 apply_output_bias=True, fctype=0, seed=0, radial_scale=0.1, bias_sigma=0.05):
     """Write a random-init TensorMD model (Xavier-normal FP64 weights).
 
@@ -79,7 +80,7 @@ def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn=
     z["layer_sizes"] = i32(sizes)
     z["use_resnet_dt"] = i32(1 if use_resnet_dt else 0)
     z["apply_output_bias"] = i32(1 if apply_output_bias else 0)
-    z["is_T_symmetric"] = i32(0)
+    z["is_T_symmetric"] = i32(1 if t_symmetric else 0)
     for e in range(nelt):
         prev = gw
         for l, w in enumerate(sizes):

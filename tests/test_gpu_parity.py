@@ -114,6 +114,37 @@ def test_kernel_timing_samples(synth_models):
     assert abs(outs[0]["f"] - outs[1]["f"]).max() < 1e-7
 
 
+@pytest.mark.parametrize("mm,tsym", [(0, False), (1, False), (2, False), (2, True)])
+def test_lower_max_moment(tmp_path, mm, tsym):
+    """Models with max_moment 0-2 (the reference takes 0-5; the shipped models
+    use 3) run on the max_moment-3 kernels with the missing moment channels
+    given zero first-layer weights.  Forces, energies, virial and the
+    exported descriptors (in the file's [b][k][max_moment + 1] layout) match
+    the oracle, which evaluates the model's own max_moment."""
+    say(f"case: max_moment {mm}, T-symmetric {tsym}, 2 elements, bcc 6x6x6")
+    path = W.write_model(str(tmp_path / f"mm{mm}{int(tsym)}.npz"), symbols=["Mo", "W"],
+                         masses=[95.95, 183.84], rmax=6.0, num_filters=16, layer_sizes=[64, 32, 1],
+                         actfn="tanh", max_moment=mm, t_symmetric=tsym, seed=40 + mm,
+                         radial_scale=0.2)
+    om = oracle.OracleModel(path, 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (6, 6, 6), nelt=2, sigma=0.08, seed=41)
+    ref = om.compute_periodic(x, t - 1, cell, want_desc=True)
+    m = TensorMD(path, ["Mo", "W"], "fp32", 0.001)
+    info = m.info()
+    assert info["max_moment"] == mm and info["width"] == 2 * 16 * (mm + 1)
+    s = m.system(x, t, cell, skin=1.0)
+    got = s.compute()
+    e = _errs(got, ref, len(x))
+    say("errors", e)
+    assert e["dF"] <= F_TOL
+    assert e["dE"] <= E_TOL
+    assert e["dV"] <= 1e-6
+    G, dG = s.descriptors(info["width"])
+    scale = max(1.0, float(abs(ref["G"]).max()))
+    assert abs(G - ref["G"]).max() <= 1e-5 * scale
+    assert abs(dG - ref["dEdG"]).max() <= 1e-5 * max(1.0, float(abs(ref["dEdG"]).max()))
+
+
 def test_neighbor_set_bit_exact(synth_models):
     say("case: neighbor set, bcc 10x9x8 tilted cell")
     m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)

@@ -705,7 +705,8 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
       float gz = V[3] + x * V[6] + y * V[8] + 2.f * z * V[9] + x * x * V[12] + x * y * V[14] +
           2.f * x * z * V[15] + y * y * V[17] + 2.f * y * z * V[18] + 3.f * z * z * V[19];
       float gu = gx * x + gy * y + gz * z;
-      float rinv = (float) (1.0 / ng.r);
+      // 1-ulp hardware reciprocal (a division is a ten-instruction FP64 sequence)
+      float rinv = __builtin_amdgcn_rcpf((float) ng.r);
       // dE_i/dd = F1 + F2; dM/dd_a = sum_b dM/du_b (delta_ab - u_a u_b)/r
       double F0 = (double) (f1 * x + (gx - gu * x) * rinv) * scale;
       double F1v = (double) (f1 * y + (gy - gu * y) * rinv) * scale;
@@ -798,7 +799,7 @@ __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, 
     float *P = PS + (size_t) (e / K) * T::SSTR + (e % K) * kND;
     float4 gv = *reinterpret_cast<const float4 *>(grow + (size_t) e * kNM);
     float G0 = P[0];
-    float g0 = (fabsf(G0) < 1e-8f) ? 0.0f : gv.x / G0 * 0.5f;
+    float g0 = (fabsf(G0) < 1e-8f) ? 0.0f : gv.x * __builtin_amdgcn_rcpf(G0) * 0.5f;
     float w[kNM] = {2.0f * g0, 2.0f * gv.y, 2.0f * gv.z, 2.0f * gv.w};
     float w0 = w[0], w1 = w[1];
     if (g.tsym) {

@@ -34,13 +34,15 @@ typedef float f32x8 __attribute__((ext_vector_type(8)));
 __device__ inline float bf2f(uint16_t v) { return __uint_as_float(((uint32_t) v) << 16); }
 
 // activation value and derivative (nnp.cu:9-49, activation.hpp:158-261).
-// FAST (BF16 mode only, whose operands are rounded to 8 bits anyway): tanh
-// from one exp and one reciprocal, tanh z = 1 - 2 / (exp(2z) + 1).
+// FAST (BF16 and FP8 modes, whose operands are rounded to 8 significant bits
+// or fewer): tanh from one exp and one 1-ulp hardware reciprocal,
+// tanh z = 1 - 2 / (exp(2z) + 1) (a correctly rounded reciprocal is a
+// ten-instruction division sequence).
 template <int ACT, bool FAST = false>
 __device__ inline float act_f(float z, float &dz)
 {
   if (ACT == 2 && FAST) {
-    float h = 1.0f - 2.0f * __frcp_rn(__expf(2.0f * z) + 1.0f);
+    float h = 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * z) + 1.0f);
     dz = 1.0f - h * h;
     return h;
   }

@@ -145,6 +145,25 @@ def test_lower_max_moment(tmp_path, mm, tsym):
     assert abs(dG - ref["dEdG"]).max() <= 1e-5 * max(1.0, float(abs(ref["dEdG"]).max()))
 
 
+def test_polynomial_cutoff(tmp_path):
+    """fctype 1: the polynomial cutoff 1 + g Z^6 - (g + 1) Z^5 with g = 5
+    (tensormd.cpp:158-162, cutoff.cpp:47-51), folded into the radial table."""
+    say("case: polynomial cutoff (fctype 1), 2 elements, bcc 6x6x6")
+    path = W.write_model(str(tmp_path / "poly.npz"), symbols=["Mo", "W"], masses=[95.95, 183.84],
+                         rmax=6.0, num_filters=16, layer_sizes=[64, 32, 1], actfn="tanh", fctype=1,
+                         seed=47, radial_scale=0.2)
+    om = oracle.OracleModel(path, 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (6, 6, 6), nelt=2, sigma=0.08, seed=48)
+    ref = om.compute_periodic(x, t - 1, cell)
+    m = TensorMD(path, ["Mo", "W"], "fp32", 0.001)
+    got = m.system(x, t, cell, skin=1.0).compute()
+    e = _errs(got, ref, len(x))
+    say("errors", e)
+    assert e["dF"] <= F_TOL
+    assert e["dE"] <= E_TOL
+    assert e["dV"] <= 1e-6
+
+
 def test_neighbor_set_bit_exact(synth_models):
     say("case: neighbor set, bcc 10x9x8 tilted cell")
     m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)

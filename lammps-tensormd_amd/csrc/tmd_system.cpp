@@ -498,13 +498,12 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
     bool over = true;
     for (int attempt = 0; attempt < 4 && over; attempt++) {
       if (want_hist) {
-        unsigned long long zero = 0ull;
-        if ((e = hipMemcpyAsync(counter, &zero, sizeof(zero), hipMemcpyHostToDevice, st)) != hipSuccess)
+        // stream-ordered clear (this loop is never captured into a graph)
+        if ((e = hipMemsetAsync(counter, 0, sizeof(unsigned long long), st)) != hipSuccess)
           return hip_fail(s->ctx, e, "history reset");
       }
       if (every > 0) {
-        unsigned long long zero = 0ull;
-        if ((e = hipMemcpyAsync(s->tkept.p, &zero, sizeof(zero), hipMemcpyHostToDevice, st)) != hipSuccess)
+        if ((e = hipMemsetAsync(s->tkept.p, 0, sizeof(unsigned long long), st)) != hipSuccess)
           return hip_fail(s->ctx, e, "timing reset");
       }
       int nsamp = 0;

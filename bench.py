@@ -202,6 +202,9 @@ def main():
     # kernel with HIP events around the kernels (0: off; the roofline then
     # uses the kernel times of the profile steps after the run)
     ap.add_argument("--kernel-timing-every", type=int, default=10)
+    # NVE total-energy drift of each precision over this many steps, from the
+    # same initial state, in a separate untimed run with per-step energies
+    ap.add_argument("--drift-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -227,6 +230,7 @@ def main():
     prof = {}
     state = {}
     ktime = {}
+    drift = {}
     kt_every = args.kernel_timing_every
     nbr_within_rc = None
     head = "bf16" if "bf16" in args.precisions.split(",") else args.precisions.split(",")[0]
@@ -261,6 +265,18 @@ def main():
             st = s.get()
             state = dict(neighbors_within_rc=neighbors_within(s, st["x"], cell, 6.5),
                          temperature_K=kinetic_temperature(np.array(masses)[t - 1], st["v"]))
+        if args.drift_steps > 0:
+            sd = m.system(x, t, cell, skin=1.0)
+            sd.set_velocities(v)
+            sd.compute()
+            ep, ek = sd.md_run(args.drift_steps, 0.001, history=True)
+            et = (ep + ek) / natoms
+            drift[prec] = dict(steps=args.drift_steps, etot_per_atom_start=float(et[0]),
+                               drift_per_atom=float(et[-1] - et[0]),
+                               max_dev_per_atom=float(abs(et - et[0]).max()),
+                               epot_per_atom_start=float(ep[0] / natoms),
+                               epot_per_atom_end=float(ep[-1] / natoms))
+            sd.close()
         print(f"bench: {prec}: {res[prec]['rate'] / 1e6:.2f} M atom-steps/s, "
               f"{1e3 * dt / args.steps:.3f} ms/step, builds={s.nbuilds}, phases(ms)={prof[prec]}",
               flush=True)
@@ -371,6 +387,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "fp8_experiment": fp8,
+        "nve_drift": drift or None,
     }
     print(json.dumps(line), flush=True)
 

@@ -167,50 +167,73 @@ __global__ void k_nb_sortbins(NbrBuild nb, int nbins)
   }
 }
 
-// One thread per atom walks the stencil in a fixed order.
-__global__ void k_nb_list(NbrBuild nb, const double4 *pos, uint32_t *nbr, int *nn)
+// One wavefront per atom, one lane per stencil bin (rounds of 64 bins in
+// the stencil order o0, o1, o2): each lane counts the kept candidates of its
+// bin, a prefix sum over the lanes gives each bin its output offset, and the
+// lanes write their entries there.  The list order is the serial walk's
+// (stencil order, then ascending atom index within a bin), with ~2 rounds of
+// a few candidates per lane instead of one thread walking ~400 candidates.
+__global__ __launch_bounds__(256) void k_nb_list(NbrBuild nb, const double4 *pos, uint32_t *nbr, int *nn)
 {
   if (!rebuild_due(nb.ctl, nb.trig2)) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (i >= live_atoms(nb)) return;
   const CellInfo &c = cell_of(nb, i);
-  double4 xi = pos[i];
-  int bc0 = nb.atom_bc[3 * i], bc1 = nb.atom_bc[3 * i + 1], bc2 = nb.atom_bc[3 * i + 2];
-  int cnt = 0;
-  uint32_t *out = nbr + (size_t) i * nb.cap;
+  const double4 xi = pos[i];
+  const int bc0 = nb.atom_bc[3 * i], bc1 = nb.atom_bc[3 * i + 1], bc2 = nb.atom_bc[3 * i + 2];
   const int n0 = c.nb[0], n1 = c.nb[1], n2 = c.nb[2];
-  for (int o0 = -c.reach[0]; o0 <= c.reach[0]; o0++) {
-    int q0 = bc0 + o0;
-    int s0 = (q0 >= 0) ? q0 / n0 : -((-q0 + n0 - 1) / n0);
-    int t0 = q0 - s0 * n0;
-    for (int o1 = -c.reach[1]; o1 <= c.reach[1]; o1++) {
-      int q1 = bc1 + o1;
-      int s1 = (q1 >= 0) ? q1 / n1 : -((-q1 + n1 - 1) / n1);
-      int t1 = q1 - s1 * n1;
-      for (int o2 = -c.reach[2]; o2 <= c.reach[2]; o2++) {
-        int q2 = bc2 + o2;
-        int s2 = (q2 >= 0) ? q2 / n2 : -((-q2 + n2 - 1) / n2);
-        int t2 = q2 - s2 * n2;
-        int b = c.bin_base + (t0 * n1 + t1) * n2 + t2;
-        int lo = nb.bin_start[b], hi = nb.bin_start[b + 1];
-        for (int p = lo; p < hi; p++) {
-          int j = nb.bin_atoms[p];
-          if (j == i && s0 == 0 && s1 == 0 && s2 == 0) continue;
-          double d[3];
-          displacement(xi, pos[j], s0, s1, s2, c, d);
-          if (dist2(d) <= nb.cut2) {
-            if (cnt < nb.cap) {
-              uint32_t code = (uint32_t) ((s0 + 3) * 49 + (s1 + 3) * 7 + (s2 + 3));
-              out[cnt] = (uint32_t) j | (code << kNbrJBits);
-            }
-            cnt++;
-          }
-        }
+  const int w1 = 2 * c.reach[1] + 1, w2 = 2 * c.reach[2] + 1;
+  const int nst = (2 * c.reach[0] + 1) * w1 * w2;
+  uint32_t *out = nbr + (size_t) i * nb.cap;
+  int base = 0;  // entries of the earlier rounds (wave-uniform)
+  for (int r0 = 0; r0 < nst; r0 += 64) {
+    const int sidx = r0 + lane;
+    int lo = 0, hi = 0, s0 = 0, s1 = 0, s2 = 0;
+    if (sidx < nst) {
+      const int o0 = sidx / (w1 * w2) - c.reach[0];
+      const int o1 = (sidx / w2) % w1 - c.reach[1];
+      const int o2 = sidx % w2 - c.reach[2];
+      const int q0 = bc0 + o0, q1 = bc1 + o1, q2 = bc2 + o2;
+      s0 = (q0 >= 0) ? q0 / n0 : -((-q0 + n0 - 1) / n0);
+      s1 = (q1 >= 0) ? q1 / n1 : -((-q1 + n1 - 1) / n1);
+      s2 = (q2 >= 0) ? q2 / n2 : -((-q2 + n2 - 1) / n2);
+      const int t0 = q0 - s0 * n0, t1 = q1 - s1 * n1, t2 = q2 - s2 * n2;
+      const int b = c.bin_base + (t0 * n1 + t1) * n2 + t2;
+      lo = nb.bin_start[b];
+      hi = nb.bin_start[b + 1];
+    }
+    const bool self_image = s0 == 0 && s1 == 0 && s2 == 0;
+    auto kept = [&](int j) {
+      if (j == i && self_image) return false;
+      double d[3];
+      displacement(xi, pos[j], s0, s1, s2, c, d);
+      return dist2(d) <= nb.cut2;
+    };
+    int cnt = 0;
+    for (int p = lo; p < hi; p++) cnt += kept(nb.bin_atoms[p]) ? 1 : 0;
+    // exclusive prefix sum of the lane counts
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    int k = base + incl - cnt;
+    const uint32_t code = (uint32_t) ((s0 + 3) * 49 + (s1 + 3) * 7 + (s2 + 3)) << kNbrJBits;
+    for (int p = lo; p < hi; p++) {
+      const int j = nb.bin_atoms[p];
+      if (kept(j)) {
+        if (k < nb.cap) out[k] = (uint32_t) j | code;
+        k++;
       }
     }
+    base += __shfl(incl, 63);
   }
-  nn[i] = min(cnt, nb.cap);
-  if (cnt > nb.cap) atomicMax(&nb.ctl[2], (unsigned long long) cnt);
+  if (lane == 0) {
+    nn[i] = min(base, nb.cap);
+    if (base > nb.cap) atomicMax(&nb.ctl[2], (unsigned long long) base);
+  }
 }
 
 __global__ void k_nb_done(NbrBuild nb, int nbins)
@@ -253,7 +276,7 @@ hipError_t nbr_step(const NbrBuild &nb, double4 *pos, uint32_t *nbr, int *nn, hi
   k_nb_scan<<<1, 1024, 0, stream>>>(nb, nbins);
   k_nb_fill<<<blocks, 256, 0, stream>>>(nb);
   k_nb_sortbins<<<(nbins + 255) / 256, 256, 0, stream>>>(nb, nbins);
-  k_nb_list<<<(nb.n + 127) / 128, 128, 0, stream>>>(nb, pos, nbr, nn);
+  k_nb_list<<<(nb.n + 3) / 4, 256, 0, stream>>>(nb, pos, nbr, nn);
   k_nb_done<<<1, 1024, 0, stream>>>(nb, nbins);
   return hipGetLastError();
 }

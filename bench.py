@@ -9,7 +9,12 @@ rebuilds (skin 1 A).  A step = velocity Verlet + neighbor check/rebuild +
 descriptor + MLP forward/backward + forces/virial + FP64 energy/virial sums.
 
 `value` is the BF16-MLP path's atom-steps/s; the plain FP32 path is timed in
-the same run (`fp32_atom_steps_per_s`).
+the same run (`fp32_atom_steps_per_s`).  `roofline` prices the dominant
+kernel's algorithmic FLOPs at its mean duration over the timed run (every
+10th step launched kernel by kernel with HIP events on the library stream)
+and at the kept-neighbor count of those steps.  `nve_drift` is each path's
+total-energy drift over a separate untimed 1000-step run from the same
+initial state.
 
 Serving (the `serving` object): 4096 single-point requests, periodic BCC/FCC
 supercells with log-uniform atom counts in [16, 4096] and random

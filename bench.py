@@ -201,7 +201,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--precisions", default="fp32,bf16")
     ap.add_argument("--serve-requests", type=int, default=4096)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--c4-steps", type=int, default=200)
     # every N-th timed step of the headline precision is launched kernel by
     # kernel with HIP events around the kernels (0: off; the roofline then
@@ -348,9 +348,19 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        # threads pinned to consecutive cores (set before the oracle's OpenMP
+        # runtime starts)
+        os.environ.setdefault("OMP_PROC_BIND", "close")
         import oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu_model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as fh:
+                cpu_model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         om = oracle.OracleModel(model, 0.001)
+        oracle.set_threads(threads)
         nsteps = args.cpu_steps
         t0 = time.perf_counter()
         om.nve(x, v, t - 1, cell, 0.001, nsteps)
@@ -361,7 +371,23 @@ def main():
                    sample=f"FP64 C++/OpenMP restatement of the TensorMD semantics (not the "
                           f"reference binary) running velocity-Verlet on the same {natoms}-atom "
                           f"system: {nsteps} steps = {nsteps + 1} force evaluations with a "
-                          f"neighbor build each, {dtc:.1f} s, OMP threads={threads}")
+                          f"neighbor build each, {dtc:.1f} s, OMP threads={threads} "
+                          f"(OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}), CPU: {cpu_model}")
+        # the single-element FCC C1 system (2048 atoms), 1 thread and all threads
+        c1 = W.write_model(os.path.join(tmp, "c1.npz"), symbols=["Cu"], masses=[63.546], rmax=6.0,
+                           num_filters=16, layer_sizes=[128, 64, 1], actfn="tanh", seed=11,
+                           radial_scale=0.2)
+        x1, t1, cell1 = W.crystal("fcc", 3.615, (8, 8, 8), sigma=0.05, seed=3)
+        v1 = W.maxwell_boltzmann(np.full(len(x1), 63.546), 300.0, seed=4)
+        o1 = oracle.OracleModel(c1, 0.001)
+        c1_rates = {}
+        for nth in (1, threads):
+            oracle.set_threads(nth)
+            t0 = time.perf_counter()
+            o1.nve(x1, v1, t1 - 1, cell1, 0.001, args.cpu_steps)
+            c1_rates[f"threads_{nth}"] = len(x1) * (args.cpu_steps + 1) / (time.perf_counter() - t0)
+        oracle.set_threads(threads)
+        cpu["c1_2048_atoms_atom_steps_per_s"] = c1_rates
         print(f"bench: cpu baseline {cpu}", flush=True)
 
     if "bf16" not in res or "fp32" not in res:

@@ -62,6 +62,7 @@ def lib():
         L.tmdo_nve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _ip, _dp, _dp,
                                ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
         L.tmdo_num_threads.restype = ctypes.c_int
+        L.tmdo_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -272,3 +273,7 @@ def neighbors(x, cell, cut, method="binned", cap_per_atom=400):
 
 def num_threads():
     return int(lib().tmdo_num_threads())
+
+
+def set_threads(n):
+    lib().tmdo_set_threads(int(n))

@@ -1109,4 +1109,15 @@ int tmdo_num_threads()
 #endif
 }
 
+// threads of the following parallel regions (benchmark baselines: 1 thread
+// and all cores)
+void tmdo_set_threads(int n)
+{
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void) n;
+#endif
+}
+
 }  // extern "C"

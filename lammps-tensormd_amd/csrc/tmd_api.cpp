@@ -326,9 +326,7 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
                    watom, c.vatom, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "force launch");
   if (pe) hipEventRecord(pe->ev[4], s);
-  if (c.skip_reduce)
-    e = hipSuccess;
-  else if (c.struct_off)
+  if (c.struct_off)
     e = launch_reduce_struct(eatom, watom, c.struct_off, c.nstruct, c.struct_out, s, c.dyn);
   else
     e = launch_reduce_ev(eatom, watom, c.natoms, (double *) w.red.p, s);

@@ -101,7 +101,6 @@ struct Chain {
   double *struct_out = nullptr;     // (null struct_off: one sum into w.red[0..6])
   int rtile = 0;
   const BatchDims *dyn = nullptr;   // device extents of a batch (capture-once graphs)
-  bool skip_reduce = false;         // leave the energy/virial sums to the caller
 };
 int ensure_atoms(tmd_ctx *ctx, Workspace &w, int natoms);
 int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEvents *pe = nullptr);

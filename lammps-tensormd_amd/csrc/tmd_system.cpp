@@ -153,16 +153,12 @@ Chain sys_chain(tmd_system *s)
   return c;
 }
 
-// skip_reduce: the energy/virial sums are left to the caller (an MD run
-// without per-step energies sums them once, after its last step)
-int sys_forces(tmd_system *s, PhaseEvents *pe = nullptr, bool skip_reduce = false)
+int sys_forces(tmd_system *s, PhaseEvents *pe = nullptr)
 {
   hipStream_t st = s->ctx->stream;
   hipError_t e = nbr_step(s->nb, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p, (int *) s->w.nn.p, st);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "neighbor build");
-  Chain c = sys_chain(s);
-  c.skip_reduce = skip_reduce;
-  int rc = run_chain(s->ctx, s->w, c, st, pe);
+  int rc = run_chain(s->ctx, s->w, sys_chain(s), st, pe);
   if (rc != TMD_OK || !pe) return rc;
   // the kept-neighbor counts of this evaluation, for the sampled work per launch
   e = launch_sum_kept((const uint32_t *) s->w.kl.p, kept_cache_words(s->nb.cap),
@@ -431,7 +427,7 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
     if ((er = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
                               (const double *) s->dtfm.p, dt, st)) != hipSuccess)
       return hip_fail(s->ctx, er, "vv first");
-    int rc = sys_forces(s, pe, !want_hist);  // per-step sums only for the history
+    int rc = sys_forces(s, pe);
     if (rc != TMD_OK) return rc;
     if ((er = launch_vv_second(s->n, (const double4 *) s->w.pos.p, (double *) s->vel.p,
                                (const double *) s->w.f.p, (const double *) s->dtfm.p, st)) != hipSuccess)
@@ -572,11 +568,6 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
       }
     }
     done += nrun;
-  }
-  if (!want_hist) {  // the steps skipped the energy/virial sums: sum the last state's
-    double *eatom = (double *) s->w.ew.p;
-    if ((e = launch_reduce_ev(eatom, eatom + s->n, s->n, (double *) s->w.red.p, st)) != hipSuccess)
-      return hip_fail(s->ctx, e, "energy sums");
   }
   return TMD_OK;
 }

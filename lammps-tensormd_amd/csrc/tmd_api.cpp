@@ -551,6 +551,7 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   if (w.rl.nrows == 0) {
     std::memset(f, 0, sizeof(double) * 3 * (size_t) nall);
     if (vatom) std::memset(vatom, 0, sizeof(double) * 6 * (size_t) nall);
+    if (eatom) std::memset(eatom, 0, sizeof(double) * (size_t) nlocal);
     *energy = 0.0;
     if (virial) std::memset(virial, 0, 6 * sizeof(double));
     return TMD_OK;

@@ -49,6 +49,83 @@ def test_list_mode_fcc_c1(synth_models):
     assert e["dV"] <= 1e-6
 
 
+def _sub_list(ilist, numneigh, flat, sub):
+    """CSR neighbor data of the centers `sub` (in that order) from a full list."""
+    off = np.concatenate([[0], np.cumsum(numneigh[ilist])])
+    start = {int(i): int(off[k]) for k, i in enumerate(ilist)}
+    nn = np.zeros_like(numneigh)
+    nn[sub] = numneigh[sub]
+    sflat = np.concatenate([flat[start[int(i)]:start[int(i)] + numneigh[i]] for i in sub])
+    return np.asarray(sub, dtype=np.int32), nn, sflat
+
+
+def test_list_mode_edge_cases(synth_models):
+    """Pair::compute inputs beyond a plain full list: the scale factor, a
+    partial shuffled ilist (eatom of the other atoms is zero), an isolated
+    atom with no neighbor inside rc, and an empty ilist that must still
+    overwrite every output buffer."""
+    import ctypes
+    from tensormd_amd import lib, _p, _dp, _ip
+    say("case: list mode edge cases, C2 model, bcc 3x3x3 4 elements")
+    om = oracle.OracleModel(synth_models["c2"], 0.001)
+    m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (3, 3, 3), nelt=4, sigma=0.08, seed=32)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, om.rc + 1.0)
+    tall = t[owner]
+    n = len(x)
+    say("case: scale 0.37")
+    ref = om.compute_list(n, xall, tall - 1, ilist, numneigh, flat, scale=0.37)
+    got = m.compute_list(n, xall, tall, ilist, numneigh, flat, scale=0.37)
+    e = _errs(got, ref, n)
+    say("errors", e)
+    assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
+    assert abs(got["eatom"] - ref["eatom"]).max() <= 1e-6
+    assert abs(got["vatom"] - ref["vatom"]).max() <= 1e-5
+    say("case: partial shuffled ilist")
+    sub = np.random.default_rng(6).permutation(n)[: n // 3]
+    sl, snn, sflat = _sub_list(ilist, numneigh, flat, sub)
+    ref = om.compute_list(n, xall, tall - 1, sl, snn, sflat)
+    got = m.compute_list(n, xall, tall, sl, snn, sflat)
+    e = _errs(got, ref, len(sub))
+    say("errors", e)
+    assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
+    rest = np.setdiff1d(np.arange(n), sub)
+    assert np.array_equal(got["eatom"][rest], np.zeros(len(rest)))
+    assert abs(got["eatom"][sub] - ref["eatom"][sub]).max() <= 1e-6
+    say("case: isolated atom")
+    xi = np.array([[0.0, 0.0, 0.0], [2.6, 0.3, -0.2], [30.0, 30.0, 30.0]])
+    ti = np.array([1, 4, 2], dtype=np.int32)
+    il = np.arange(3, dtype=np.int32)
+    nn = np.array([1, 1, 0], dtype=np.int32)
+    fl = np.array([1, 0], dtype=np.int32)
+    ref = om.compute_list(3, xi, ti - 1, il, nn, fl)
+    got = m.compute_list(3, xi, ti, il, nn, fl)
+    e = _errs(got, ref, 3)
+    say("errors", e)
+    assert e["dF"] <= F_TOL and e["dE"] <= E_TOL
+    assert np.array_equal(got["f"][2], np.zeros(3))
+    assert abs(got["eatom"] - ref["eatom"]).max() <= 1e-6
+    say("case: empty ilist, outputs pre-filled")
+    xa, ta = _f64_c(xall), np.ascontiguousarray(tall, dtype=np.int32)
+    f = np.full((len(xa), 3), 7.0)
+    ea = np.full(n, 7.0)
+    va = np.full((len(xa), 6), 7.0)
+    vir = np.full(6, 7.0)
+    en = ctypes.c_double(7.0)
+    empty = np.zeros(1, dtype=np.int32)
+    rc = lib().tmd_compute_list(m.h, n, len(xa), _p(xa, _dp), _p(ta, _ip), 0, _p(empty, _ip),
+                                _p(empty, _ip), None, 1.0, _p(f, _dp), _p(ea, _dp), _p(va, _dp),
+                                ctypes.byref(en), _p(vir, _dp))
+    assert rc == 0
+    assert en.value == 0.0
+    for a in (f, ea, va, vir):
+        assert np.array_equal(a, np.zeros_like(a))
+
+
+def _f64_c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
 def test_system_c2_shape(synth_models):
     say("case: periodic system, C2 model, bcc 8x8x8 4 elements")
     om = oracle.OracleModel(synth_models["c2"], 0.001)

@@ -224,3 +224,34 @@ def test_wrapper_refuses_mismatched_arrays(c2):
         c2.compute_periodic(x, 0, cell)
     with pytest.raises(ValueError):
         c2.nve(x, np.zeros_like(x), np.zeros(5, dtype=np.int32), cell, 0.001, 1)
+
+
+def test_list_mode_scale_and_partial_ilist(c2):
+    """The pair style's scale factor multiplies E, eatom, f and vatom
+    (src/ML-TENSORMD/tensormd.cpp:494-614), and a partial ilist computes only
+    those centers: two disjoint ilists sum to the full list, in any order,
+    with eatom zero for atoms outside the list."""
+    x, t, cell = W.crystal("bcc", 3.2, (3, 3, 3), nelt=4, sigma=0.08, seed=31)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, c2.rc + 1.0)
+    elem = (t - 1)[owner]
+    n = len(x)
+    full = c2.compute_list(n, xall, elem, ilist, numneigh, flat)
+    sc = c2.compute_list(n, xall, elem, ilist, numneigh, flat, scale=0.37)
+    assert abs(sc["energy"] - 0.37 * full["energy"]) <= 1e-12 * abs(full["energy"])
+    for k in ("f", "eatom", "vatom", "virial"):
+        assert np.allclose(sc[k], 0.37 * full[k], rtol=1e-12, atol=1e-14), k
+    perm = np.random.default_rng(5).permutation(n)
+    A, B = np.sort(perm[: n // 3])[::-1], perm[n // 3:]
+    parts = []
+    for sub in (A, B):
+        nn = np.zeros_like(numneigh)
+        nn[sub] = numneigh[sub]
+        off = np.concatenate([[0], np.cumsum(numneigh[ilist])])
+        start = {int(i): int(off[k]) for k, i in enumerate(ilist)}
+        sflat = np.concatenate([flat[start[int(i)]:start[int(i)] + numneigh[i]] for i in sub])
+        parts.append(c2.compute_list(n, xall, elem, sub.astype(np.int32), nn, sflat))
+    assert abs(parts[0]["energy"] + parts[1]["energy"] - full["energy"]) <= 1e-10
+    for k in ("f", "vatom", "virial"):
+        assert np.allclose(parts[0][k] + parts[1][k], full[k], atol=1e-11), k
+    assert np.array_equal(parts[0]["eatom"][B], np.zeros(len(B)))
+    assert np.allclose(parts[0]["eatom"][A], full["eatom"][A], atol=1e-13)

@@ -302,15 +302,15 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
   if (w.kl.bytes < (size_t) w.rl.nrows * kept_cache_words(c.cap) * 4)
     return (set_err(ctx, "internal: kept-list cache not allocated for this list"), TMD_ERR_STATE);
   uint32_t *kl = (uint32_t *) w.kl.p;
-  if (pe) hipEventRecord(pe->ev[0], s);
+  if (pe && (e = hipEventRecord(pe->ev[0], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
   e = launch_descriptor(ctx->geo, c.geo, c.pos, c.nbr, c.nn, c.cap, row2atom, w.rl.nrows,
                         (const float4 *) ctx->tab.p, (float *) w.G.p, kl, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
-  if (pe) hipEventRecord(pe->ev[1], s);
+  if (pe && (e = hipEventRecord(pe->ev[1], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
   e = launch_mlp(mode, w.rl, ctx->mlp, (const float *) w.G.p, (double *) w.E.p, (float *) w.dEdG.p, s,
                  c.rtile, c.dyn);
   if (e != hipSuccess) return hip_fail(ctx, e, "mlp launch");
-  if (pe) hipEventRecord(pe->ev[2], s);
+  if (pe && (e = hipEventRecord(pe->ev[2], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
   if (c.dyn)
     e = launch_zero_atoms(c.f, 3, (int) (c.fbytes / (3 * sizeof(double))), c.dyn, s);
   else
@@ -320,18 +320,18 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
     return hip_fail(ctx, e, "zero vatom");
   if (c.zero_ew && (e = launch_zero(eatom, (size_t) c.natoms * 7 * sizeof(double), s)) != hipSuccess)
     return hip_fail(ctx, e, "zero eatom");
-  if (pe) hipEventRecord(pe->ev[3], s);
+  if (pe && (e = hipEventRecord(pe->ev[3], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
   e = launch_force(ctx->geo, c.geo, c.pos, c.cap, row2atom, w.rl.nrows, (const float4 *) ctx->tab.p,
                    kl, (const float *) w.dEdG.p, (const double *) w.E.p, c.scale, c.f, eatom,
                    watom, c.vatom, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "force launch");
-  if (pe) hipEventRecord(pe->ev[4], s);
+  if (pe && (e = hipEventRecord(pe->ev[4], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
   if (c.struct_off)
     e = launch_reduce_struct(eatom, watom, c.struct_off, c.nstruct, c.struct_out, s, c.dyn);
   else
     e = launch_reduce_ev(eatom, watom, c.natoms, (double *) w.red.p, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "reduce launch");
-  if (pe) hipEventRecord(pe->ev[5], s);
+  if (pe && (e = hipEventRecord(pe->ev[5], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
   return TMD_OK;
 }
 
@@ -502,7 +502,8 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
     set_err(ctx, "tmd_compute_list: nall exceeds the 2^23-1 atom limit of the neighbor encoding");
     return TMD_ERR_CAPACITY;
   }
-  hipSetDevice(ctx->device);
+  if (hipError_t de = hipSetDevice(ctx->device); de != hipSuccess)
+    return hip_fail(ctx, de, "hipSetDevice");
   Workspace &w = ctx->list_ws;
   hipStream_t s = ctx->stream;
   // element per position (type map), packed positions
@@ -566,15 +567,19 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   if ((e = w.f.ensure(sizeof(double) * 3 * (size_t) nall)) != hipSuccess) return hip_fail(ctx, e, "f alloc");
   if (vatom && (e = w.vatom.ensure(sizeof(double) * 6 * (size_t) nall)) != hipSuccess)
     return hip_fail(ctx, e, "vatom alloc");
-  hipMemcpyAsync(w.pos.p, pos.data(), sizeof(double4) * pos.size(), hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(w.nbr.p, list.data(), 4 * list.size(), hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(w.nn.p, nn.data(), 4 * nn.size(), hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(w.row2atom.p, row2atom.data(), 4 * row2atom.size(), hipMemcpyHostToDevice, s);
+  if ((e = hipMemcpyAsync(w.pos.p, pos.data(), sizeof(double4) * pos.size(), hipMemcpyHostToDevice, s)) !=
+          hipSuccess ||
+      (e = hipMemcpyAsync(w.nbr.p, list.data(), 4 * list.size(), hipMemcpyHostToDevice, s)) != hipSuccess ||
+      (e = hipMemcpyAsync(w.nn.p, nn.data(), 4 * nn.size(), hipMemcpyHostToDevice, s)) != hipSuccess ||
+      (e = hipMemcpyAsync(w.row2atom.p, row2atom.data(), 4 * row2atom.size(), hipMemcpyHostToDevice, s)) !=
+          hipSuccess)
+    return hip_fail(ctx, e, "list upload");
   // list mode: positions are already images (ghosts), the one cell is zero
   CellInfo zero_cell;
   std::memset(&zero_cell, 0, sizeof(zero_cell));
   if ((e = w.cells.ensure(sizeof(CellInfo))) != hipSuccess) return hip_fail(ctx, e, "cell alloc");
-  hipMemcpyAsync(w.cells.p, &zero_cell, sizeof(CellInfo), hipMemcpyHostToDevice, s);
+  if ((e = hipMemcpyAsync(w.cells.p, &zero_cell, sizeof(CellInfo), hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(ctx, e, "cell upload");
   if ((rc = ensure_atoms(ctx, w, nlocal)) != TMD_OK) return rc;
   Chain c;
   c.geo.cells = (const CellInfo *) w.cells.p;
@@ -592,10 +597,13 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   rc = run_chain(ctx, w, c, s);
   if (rc != TMD_OK) return rc;
   double red[7];
-  hipMemcpyAsync(f, w.f.p, sizeof(double) * 3 * (size_t) nall, hipMemcpyDeviceToHost, s);
-  if (vatom) hipMemcpyAsync(vatom, w.vatom.p, sizeof(double) * 6 * (size_t) nall, hipMemcpyDeviceToHost, s);
-  hipMemcpyAsync(red, w.red.p, sizeof(red), hipMemcpyDeviceToHost, s);
-  if (eatom) hipMemcpyAsync(eatom, w.ew.p, sizeof(double) * (size_t) nlocal, hipMemcpyDeviceToHost, s);
+  if ((e = hipMemcpyAsync(f, w.f.p, sizeof(double) * 3 * (size_t) nall, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (vatom && (e = hipMemcpyAsync(vatom, w.vatom.p, sizeof(double) * 6 * (size_t) nall,
+                                    hipMemcpyDeviceToHost, s)) != hipSuccess) ||
+      (e = hipMemcpyAsync(red, w.red.p, sizeof(red), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+      (eatom && (e = hipMemcpyAsync(eatom, w.ew.p, sizeof(double) * (size_t) nlocal, hipMemcpyDeviceToHost,
+                                    s)) != hipSuccess))
+    return hip_fail(ctx, e, "result download");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(ctx, e, "tmd_compute_list");
   *energy = red[0];
   if (virial)

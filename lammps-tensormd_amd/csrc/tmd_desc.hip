@@ -860,7 +860,10 @@ hipError_t desc_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, con
   size_t offs[3];
   size_t lds = lds_layout<K>(cap, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
-  hipFuncSetAttribute((const void *) k_descriptor<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+  if (hipError_t ae = hipFuncSetAttribute((const void *) k_descriptor<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int) lds);
+      ae != hipSuccess)
+    return ae;
   k_descriptor<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl);
   return hipGetLastError();
 }
@@ -874,7 +877,10 @@ hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, in
   size_t offs[3];
   const size_t lds = force_lds_layout<K>(g.nelt, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
-  hipFuncSetAttribute((const void *) k_force<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
+  if (hipError_t ae = hipFuncSetAttribute((const void *) k_force<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int) lds);
+      ae != hipSuccess)
+    return ae;
   k_force<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale,
                                         f, eatom, watom, vatom);
   return hipGetLastError();

@@ -823,8 +823,10 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
     if (!fast) return hipErrorInvalidConfiguration;
 #define TMD_MLP8_CASE(A)                                                                         \
   case A:                                                                                        \
-    hipFuncSetAttribute((const void *) k_mlp_fast<2, A, R>,                                      \
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                  \
+    if (hipError_t ae = hipFuncSetAttribute((const void *) k_mlp_fast<2, A, R>,                  \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);              \
+        ae != hipSuccess)                                                                        \
+      return ae;                                                                                 \
     k_mlp_fast<2, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                         \
     break;
     switch (mp.act) {
@@ -840,12 +842,16 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
 #define TMD_MLP_CASE(A)                                                                          \
   case A:                                                                                        \
     if (fast) {                                                                                  \
-      hipFuncSetAttribute((const void *) k_mlp_fast<MODE, A, R>,                                 \
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                \
+      if (hipError_t ae = hipFuncSetAttribute((const void *) k_mlp_fast<MODE, A, R>,             \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);            \
+          ae != hipSuccess)                                                                      \
+        return ae;                                                                               \
       k_mlp_fast<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                    \
     } else {                                                                                     \
-      hipFuncSetAttribute((const void *) k_mlp<MODE, A, R>,                                      \
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);                \
+      if (hipError_t ae = hipFuncSetAttribute((const void *) k_mlp<MODE, A, R>,                  \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);            \
+          ae != hipSuccess)                                                                      \
+        return ae;                                                                               \
       k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                         \
     }                                                                                            \
     break;

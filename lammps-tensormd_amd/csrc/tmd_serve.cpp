@@ -451,8 +451,8 @@ bool finish_batch(Server *S, Slot &sl, bool *rerun)
 {
   *rerun = false;
   const double *out = sl.out;
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, sl.ev_start, sl.ev_done);
+  float ms = 0.f;  // the batch's device time feeds the statistics only
+  if (hipEventElapsedTime(&ms, sl.ev_start, sl.ev_done) != hipSuccess) ms = 0.f;
   const long overflow = (long) out[0];
   const int err = (int) out[3];
   if (overflow > 0 && !err) {
@@ -657,7 +657,8 @@ int tmd_serve_start(tmd_ctx *ctx, const tmd_serve_options *opt)
   if (o.slots < 1 || o.slots > 8 || o.max_batch_atoms < 16 || o.max_batch_atoms > kMaxAtoms ||
       o.max_batch_structs < 1 || !(o.hbm_fraction > 0 && o.hbm_fraction <= 0.9) || o.max_queue < 1)
     return serve_fail(ctx, TMD_ERR_ARG, "tmd_serve_start: invalid options");
-  hipSetDevice(ctx->device);
+  if (hipError_t de = hipSetDevice(ctx->device); de != hipSuccess)
+    return hip_fail(ctx, de, "hipSetDevice");
   std::unique_ptr<Server> S(new Server());
   S->ctx = ctx;
   S->opt = o;

@@ -185,7 +185,9 @@ int check_overflow(tmd_system *s, bool *overflow)
     int rc = alloc_list(s, cap);
     if (rc != TMD_OK) return rc;
     unsigned long long reset[3] = {0ull, 1ull, 0ull};
-    hipMemcpy(s->ctl.p, reset, sizeof(reset), hipMemcpyHostToDevice);
+    if (hipError_t ce = hipMemcpy(s->ctl.p, reset, sizeof(reset), hipMemcpyHostToDevice);
+        ce != hipSuccess)
+      return hip_fail(s->ctx, ce, "list control reset");
   }
   return TMD_OK;
 }
@@ -206,7 +208,8 @@ int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type
     set_err(ctx, "tmd_system_create: more atoms than the 2^23-1 limit of the neighbor encoding");
     return TMD_ERR_CAPACITY;
   }
-  hipSetDevice(ctx->device);
+  if (hipError_t de = hipSetDevice(ctx->device); de != hipSuccess)
+    return hip_fail(ctx, de, "hipSetDevice");
   std::unique_ptr<tmd_system> s(new tmd_system());
   s->ctx = ctx;
   s->n = natoms;
@@ -314,7 +317,8 @@ void tmd_system_destroy(tmd_system *s)
 int tmd_system_set_positions(tmd_system *s, const double *x)
 {
   if (!s || !x) return TMD_ERR_ARG;
-  hipSetDevice(s->ctx->device);
+  if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
+    return hip_fail(s->ctx, de, "hipSetDevice");
   std::vector<double4> pos(s->n);
   hipStream_t st = s->ctx->stream;
   hipError_t e = hipMemcpyAsync(pos.data(), s->w.pos.p, sizeof(double4) * s->n, hipMemcpyDeviceToHost, st);
@@ -337,7 +341,8 @@ int tmd_system_set_positions(tmd_system *s, const double *x)
 int tmd_system_set_velocities(tmd_system *s, const double *v)
 {
   if (!s || !v) return TMD_ERR_ARG;
-  hipSetDevice(s->ctx->device);
+  if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
+    return hip_fail(s->ctx, de, "hipSetDevice");
   hipError_t e = hipMemcpy(s->vel.p, v, sizeof(double) * 3 * (size_t) s->n, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "set velocities");
   return TMD_OK;
@@ -346,7 +351,8 @@ int tmd_system_set_velocities(tmd_system *s, const double *v)
 int tmd_system_compute(tmd_system *s)
 {
   if (!s) return TMD_ERR_ARG;
-  hipSetDevice(s->ctx->device);
+  if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
+    return hip_fail(s->ctx, de, "hipSetDevice");
   for (int attempt = 0; attempt < 4; attempt++) {
     int rc = sys_forces(s);
     if (rc != TMD_OK) return rc;
@@ -365,7 +371,8 @@ int tmd_system_get(tmd_system *s, double *x, double *v, double *f, double *eatom
                    double *virial)
 {
   if (!s) return TMD_ERR_ARG;
-  hipSetDevice(s->ctx->device);
+  if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
+    return hip_fail(s->ctx, de, "hipSetDevice");
   hipStream_t st = s->ctx->stream;
   std::vector<double4> pos;
   double red[7];
@@ -396,7 +403,8 @@ int tmd_system_get(tmd_system *s, double *x, double *v, double *f, double *eatom
 int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
 {
   if (!s || nsteps < 0 || !(dt > 0)) return TMD_ERR_ARG;
-  hipSetDevice(s->ctx->device);
+  if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
+    return hip_fail(s->ctx, de, "hipSetDevice");
   hipStream_t st = s->ctx->stream;
   const Model &m = s->ctx->model;
   hipError_t e;

@@ -126,6 +126,45 @@ def _f64_c(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def test_list_mode_rejections(synth_models):
+    """Malformed Pair::compute inputs are refused with TMD_ERR_ARG before any
+    device work: an unmapped type, a neighbor index past nall, nall < nlocal,
+    and an ilist entry past nlocal."""
+    import ctypes
+    from tensormd_amd import TmdError, lib, _p, _dp, _ip
+    say("case: list mode argument errors")
+    m = TensorMD(synth_models["c1"], ["Cu"], "fp32", 0.001)
+    x, t, cell = W.crystal("fcc", 3.615, (2, 2, 2), sigma=0.05, seed=23)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, 7.0)
+    tall = t[owner].astype(np.int32)
+    n = len(x)
+    bad_t = tall.copy()
+    bad_t[-1] = 2  # one type past the single mapped type
+    bad_flat = flat.copy()
+    bad_flat[0] = len(xall)
+    for what, args in (("unmapped type", (n, xall, bad_t, ilist, numneigh, flat)),
+                       ("neighbor past nall", (n, xall, tall, ilist, numneigh, bad_flat)),
+                       ("nall < nlocal", (len(xall) + 1, xall, tall, ilist,
+                                          np.concatenate([numneigh, np.zeros(len(xall) + 1 - n,
+                                                                             dtype=numneigh.dtype)]),
+                                          flat))):
+        say("case:", what)
+        with pytest.raises(TmdError) as e:
+            m.compute_list(*args)
+        assert e.value.code == -1, what
+    say("case: ilist past nlocal")
+    xa, ta = _f64_c(xall), np.ascontiguousarray(tall)
+    il = np.array([0, n], dtype=np.int32)
+    nn = np.zeros(n + 1, dtype=np.int32)
+    ptrs = (_ip * (n + 1))()
+    f = np.zeros((len(xa), 3))
+    en = ctypes.c_double()
+    rc = lib().tmd_compute_list(m.h, n, len(xa), _p(xa, _dp), _p(ta, _ip), 2, _p(il, _ip),
+                                _p(nn, _ip), ptrs, 1.0, _p(f, _dp), None, None,
+                                ctypes.byref(en), None)
+    assert rc == -1
+
+
 def test_system_c2_shape(synth_models):
     say("case: periodic system, C2 model, bcc 8x8x8 4 elements")
     om = oracle.OracleModel(synth_models["c2"], 0.001)

@@ -19,6 +19,8 @@
  *                       TensorMD::calc_tensor_density/run (tensormd.cpp:368-433,
  *                       :450-615) and Pair::virial_fdotr_compute
  *                       (src/pair.cpp:1735-1783)
+ *   tmd_descriptors_list ComputeTensorMDDensity::compute_peratom
+ *                       (src/ML-TENSORMD/compute_tensormd_density.cpp:129-205)
  *   tmd_system_*,       the LAMMPS step loop around the pair style:
  *   tmd_md_run          Verlet::run (src/verlet.cpp:246-350), NPairFullBin::build
  *                       (src/npair_full_bin.cpp:68-126), Neighbor::decide
@@ -103,6 +105,14 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
                      int inum, const int *ilist, const int *numneigh,
                      const int *const *firstneigh, double scale, double *f, double *eatom,
                      double *vatom, double *energy, double *virial);
+
+/* `compute tensormd/density`: the descriptor row of every atom in ilist,
+ * G[nlocal][width] in the model file's width and [b][k][m] layout (rows of
+ * atoms not in ilist are zero), from the same inputs as tmd_compute_list.
+ * Computed in FP32 on the device, returned as double. */
+int tmd_descriptors_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type,
+                         int inum, const int *ilist, const int *numneigh,
+                         const int *const *firstneigh, double *G);
 
 /* ---- device-resident periodic system ----------------------------------------
  * cell = rows a, b, c (Cartesian, 9 doubles); any triclinic cell.  Positions

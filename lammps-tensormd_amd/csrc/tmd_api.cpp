@@ -484,16 +484,18 @@ int tmd_model_info(const tmd_ctx *ctx, int *info)
   return TMD_OK;
 }
 
-int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type,
-                     int inum, const int *ilist, const int *numneigh,
-                     const int *const *firstneigh, double scale, double *f, double *eatom,
-                     double *vatom, double *energy, double *virial)
+// Pair::compute over a LAMMPS full list; with Gout set, only the descriptor
+// rows of the listed atoms ([nlocal][gwidth_file], other rows zero)
+static int list_compute(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type, int inum,
+                        const int *ilist, const int *numneigh, const int *const *firstneigh, double scale,
+                        double *f, double *eatom, double *vatom, double *energy, double *virial,
+                        double *Gout)
 {
   if (!ctx) {
     set_err(nullptr, "tmd_compute_list: ctx is NULL");
     return TMD_ERR_ARG;
   }
-  if (nall < nlocal || inum < 0 || inum > nlocal || !x || !type || !f || !energy ||
+  if (nall < nlocal || inum < 0 || inum > nlocal || !x || !type || (!Gout && (!f || !energy)) ||
       (inum > 0 && (!ilist || !numneigh || !firstneigh))) {
     set_err(ctx, "tmd_compute_list: invalid arguments");
     return TMD_ERR_ARG;
@@ -549,7 +551,10 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   }
   std::vector<int> row2atom;
   make_rows(ctx->model.nelt, atoms, elem, w.rl, row2atom);
+  const int gwf = ctx->model.gwidth_file();
+  if (Gout) std::memset(Gout, 0, sizeof(double) * (size_t) nlocal * gwf);
   if (w.rl.nrows == 0) {
+    if (Gout) return TMD_OK;
     std::memset(f, 0, sizeof(double) * 3 * (size_t) nall);
     if (vatom) std::memset(vatom, 0, sizeof(double) * 6 * (size_t) nall);
     if (eatom) std::memset(eatom, 0, sizeof(double) * (size_t) nlocal);
@@ -581,6 +586,28 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   if ((e = hipMemcpyAsync(w.cells.p, &zero_cell, sizeof(CellInfo), hipMemcpyHostToDevice, s)) != hipSuccess)
     return hip_fail(ctx, e, "cell upload");
   if ((rc = ensure_atoms(ctx, w, nlocal)) != TMD_OK) return rc;
+  if (Gout) {
+    Geom geo;
+    std::memset(&geo, 0, sizeof(geo));
+    geo.cells = (const CellInfo *) w.cells.p;
+    e = launch_descriptor(ctx->geo, geo, (const double4 *) w.pos.p, (const uint32_t *) w.nbr.p,
+                          (const int *) w.nn.p, cap, (const int *) w.row2atom.p, w.rl.nrows,
+                          (const float4 *) ctx->tab.p, (float *) w.G.p, (uint32_t *) w.kl.p, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
+    const int gw = ctx->model.gwidth(), nm = ctx->model.nm, nmf = ctx->model.max_moment_file + 1;
+    std::vector<float> g((size_t) w.rl.nrows * gw);
+    if ((e = hipMemcpyAsync(g.data(), w.G.p, 4 * g.size(), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return hip_fail(ctx, e, "descriptor download");
+    // the model file's layout [b][k][max_moment_file + 1]
+    for (int r = 0; r < w.rl.nrows; r++) {
+      const int a = row2atom[r];
+      if (a < 0) continue;
+      for (int q = 0; q < gwf; q++)
+        Gout[(size_t) a * gwf + q] = g[(size_t) r * gw + (size_t) (q / nmf) * nm + q % nmf];
+    }
+    return TMD_OK;
+  }
   Chain c;
   c.geo.cells = (const CellInfo *) w.cells.p;
   c.pos = (const double4 *) w.pos.p;
@@ -609,6 +636,26 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
   if (virial)
     for (int q = 0; q < 6; q++) virial[q] = red[1 + q];
   return TMD_OK;
+}
+
+int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type,
+                     int inum, const int *ilist, const int *numneigh,
+                     const int *const *firstneigh, double scale, double *f, double *eatom,
+                     double *vatom, double *energy, double *virial)
+{
+  return list_compute(ctx, nlocal, nall, x, type, inum, ilist, numneigh, firstneigh, scale, f, eatom, vatom,
+                      energy, virial, nullptr);
+}
+
+int tmd_descriptors_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type, int inum,
+                         const int *ilist, const int *numneigh, const int *const *firstneigh, double *G)
+{
+  if (!G) {
+    set_err(ctx, "tmd_descriptors_list: G is NULL");
+    return TMD_ERR_ARG;
+  }
+  return list_compute(ctx, nlocal, nall, x, type, inum, ilist, numneigh, firstneigh, 1.0, nullptr, nullptr,
+                      nullptr, nullptr, nullptr, G);
 }
 
 int tmd_device_memory(size_t *free_bytes, size_t *total_bytes)

@@ -41,6 +41,7 @@ class PairTensorMDAMD : public Pair {
   double memory_usage() override;
 
   double etemp;  // accepted for input compatibility; TDNP models are rejected
+  tmd_ctx *context() const { return ctx; }  // for compute tensormd/density/amd
 
  protected:
   tmd_ctx *ctx;

@@ -1,10 +1,12 @@
-/* Registers pair_style tensormd/amd with a LAMMPS binary built with
-   PKG_PLUGIN (src/lammpsplugin.h:17-39, src/PLUGIN/plugin.cpp:98-122 of the
-   reference tree): `plugin load tensormd_amd_plugin.so`. */
+/* Registers pair_style tensormd/amd and compute tensormd/density/amd with a
+   LAMMPS binary built with PKG_PLUGIN (src/lammpsplugin.h:17-39,
+   src/PLUGIN/plugin.cpp:98-122 of the reference tree):
+   `plugin load tensormd_amd_plugin.so`. */
 
 #include "lammpsplugin.h"
 #include "version.h"
 
+#include "compute_tensormd_density_amd.h"
 #include "pair_tensormd_amd.h"
 
 using namespace LAMMPS_NS;
@@ -12,6 +14,11 @@ using namespace LAMMPS_NS;
 static Pair *tensormd_amd_creator(LAMMPS *lmp)
 {
   return new PairTensorMDAMD(lmp);
+}
+
+static Compute *tensormd_density_amd_creator(LAMMPS *lmp, int narg, char **arg)
+{
+  return new ComputeTensorMDDensityAMD(lmp, narg, arg);
 }
 
 extern "C" void lammpsplugin_init(void *lmp, void *handle, void *regfunc)
@@ -25,5 +32,11 @@ extern "C" void lammpsplugin_init(void *lmp, void *handle, void *regfunc)
   plugin.author = "tensormd-mi355x";
   plugin.creator.v1 = (lammpsplugin_factory1 *) &tensormd_amd_creator;
   plugin.handle = handle;
+  (*reg)(&plugin, lmp);
+
+  plugin.style = "compute";
+  plugin.name = "tensormd/density/amd";
+  plugin.info = "TensorMD per-atom descriptors on MI355X (libtensormd_amd)";
+  plugin.creator.v2 = (lammpsplugin_factory2 *) &tensormd_density_amd_creator;
   (*reg)(&plugin, lmp);
 }

@@ -88,6 +88,8 @@ def lib():
     L.tmd_compute_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int, _ip,
                                    _ip, ctypes.POINTER(_ip), ctypes.c_double, _dp, _dp, _dp, _dp,
                                    _dp]
+    L.tmd_descriptors_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int, _ip,
+                                       _ip, ctypes.POINTER(_ip), _dp]
     L.tmd_system_create.argtypes = [_vp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_double,
                                     ctypes.POINTER(_vp)]
     L.tmd_system_destroy.argtypes = [_vp]
@@ -233,6 +235,27 @@ class TensorMD:
                                     ctypes.byref(e), _p(vir, _dp))
         _check(rc, self.h)
         return dict(f=f, eatom=ea, vatom=va, energy=e.value, virial=vir)
+
+    def descriptors_list(self, nlocal, x, types, ilist, numneigh, neigh_flat):
+        """compute tensormd/density: descriptor rows [nlocal, width] (zero for
+        atoms not in ilist) from a LAMMPS-style full neighbor list."""
+        x = _f64(x).reshape(-1, 3)
+        types = _i32(types)
+        ilist = _i32(ilist)
+        numneigh = _i32(numneigh)
+        neigh_flat = _i32(neigh_flat)
+        ptrs = (_ip * max(nlocal, 1))()
+        base = neigh_flat.ctypes.data
+        off = 0
+        for i in ilist:
+            ptrs[int(i)] = ctypes.cast(base + 4 * off, _ip)
+            off += int(numneigh[i])
+        G = np.zeros((nlocal, self.info()["width"]))
+        rc = lib().tmd_descriptors_list(self.h, int(nlocal), x.shape[0], _p(x, _dp), _p(types, _ip),
+                                        ilist.size, _p(ilist, _ip), _p(numneigh, _ip), ptrs,
+                                        _p(G, _dp))
+        _check(rc, self.h)
+        return G
 
     def system(self, x, types, cell, skin=1.0):
         return System(self, x, types, cell, skin)

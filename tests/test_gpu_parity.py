@@ -126,6 +126,35 @@ def _f64_c(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+@pytest.mark.parametrize("mm", [3, 1])
+def test_descriptors_list(tmp_path, mm):
+    """compute tensormd/density over a LAMMPS full list: the descriptor row of
+    each listed atom in the model file's layout, zero rows for the others,
+    against the oracle's G (FP32 descriptor: 1e-5 of the largest |G|)."""
+    say(f"case: list-mode descriptors, max_moment {mm}, 4 elements, bcc 3x3x3")
+    path = W.write_model(str(tmp_path / f"d{mm}.npz"), symbols=["Mo", "Nb", "Ta", "W"],
+                         masses=[95.95, 92.906, 180.95, 183.84], rmax=6.5, num_filters=24,
+                         layer_sizes=[64, 32, 1], actfn="tanh", max_moment=mm, seed=50 + mm,
+                         radial_scale=0.2)
+    om = oracle.OracleModel(path, 0.001)
+    m = TensorMD(path, ["Mo", "Nb", "Ta", "W"], "fp32", 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (3, 3, 3), nelt=4, sigma=0.08, seed=51)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, om.rc + 1.0)
+    tall = t[owner]
+    n = len(x)
+    sub = np.random.default_rng(7).permutation(n)[: n // 2]
+    for il, nn, fl in ((ilist, numneigh, flat), _sub_list(ilist, numneigh, flat, sub)):
+        ref = om.compute_list(n, xall, tall - 1, il, nn, fl, want_desc=True)["G"]
+        got = m.descriptors_list(n, xall, tall, il, nn, fl)
+        assert got.shape == (n, 4 * 24 * (mm + 1))
+        scale = max(1.0, float(abs(ref).max()))
+        err = float(abs(got[il] - ref).max())
+        say("rows", len(il), "max |dG|", err, "max |G|", scale)
+        assert err <= 1e-5 * scale
+        rest = np.setdiff1d(np.arange(n), il)
+        assert np.array_equal(got[rest], np.zeros((len(rest), got.shape[1])))
+
+
 def test_list_mode_rejections(synth_models):
     """Malformed Pair::compute inputs are refused with TMD_ERR_ARG before any
     device work: an unmapped type, a neighbor index past nall, nall < nlocal,

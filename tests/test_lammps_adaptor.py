@@ -15,7 +15,8 @@ LMP_SRC = "/root/reference/src"
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(LMP_SRC, "pair.h")) or
                     shutil.which("g++") is None, reason="LAMMPS headers not available")
-@pytest.mark.parametrize("src", ["pair_tensormd_amd.cpp", "tensormd_amd_plugin.cpp"])
+@pytest.mark.parametrize("src", ["pair_tensormd_amd.cpp", "compute_tensormd_density_amd.cpp",
+                                 "tensormd_amd_plugin.cpp"])
 def test_adaptor_compiles(src):
     here = os.path.join(ROOT, "lammps-tensormd_amd", "lammps")
     cmd = ["g++", "-std=c++17", "-fsyntax-only", "-I" + LMP_SRC, "-I" + os.path.join(LMP_SRC, "STUBS"),

@@ -529,9 +529,10 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
       int rc = check_overflow(s, &over);  // grows the list and requests a rebuild
       if (rc != TMD_OK) return rc;
       if (!over && nsamp > 0) {  // the segment stands: keep its samples (the stream is idle)
+        hipError_t te = hipSuccess;
         auto el = [&](hipEvent_t a, hipEvent_t b) {
           float t = 0.f;
-          hipEventElapsedTime(&t, a, b);
+          if (te == hipSuccess) te = hipEventElapsedTime(&t, a, b);
           return (double) t;
         };
         for (int j = 0; j < nsamp; j++) {
@@ -540,6 +541,7 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
           s->tsum[1] += el(pe.ev[1], pe.ev[2]);
           s->tsum[2] += el(pe.ev[3], pe.ev[4]);
         }
+        if (te != hipSuccess) return hip_fail(s->ctx, te, "kernel timing");
         unsigned long long kept = 0;
         if ((e = hipMemcpy(&kept, s->tkept.p, sizeof(kept), hipMemcpyDeviceToHost)) != hipSuccess)
           return hip_fail(s->ctx, e, "timing counter copy");
@@ -622,13 +624,14 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
     hipEvent_t *E = &ev[(size_t) q * NE];
     PhaseEvents pe;
     for (int k = 0; k < 6; k++) pe.ev[k] = E[3 + k];
-    hipEventRecord(E[0], st);
-    e = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
-                        (const double *) s->dtfm.p, dt, st);
-    hipEventRecord(E[1], st);
+    e = hipEventRecord(E[0], st);
+    if (e == hipSuccess)
+      e = launch_vv_first(s->n, (double4 *) s->w.pos.p, (double *) s->vel.p, (const double *) s->w.f.p,
+                          (const double *) s->dtfm.p, dt, st);
+    if (e == hipSuccess) e = hipEventRecord(E[1], st);
     if (e == hipSuccess)
       e = nbr_step(s->nb, (double4 *) s->w.pos.p, (uint32_t *) s->w.nbr.p, (int *) s->w.nn.p, st);
-    hipEventRecord(E[2], st);
+    if (e == hipSuccess) e = hipEventRecord(E[2], st);
     if (e == hipSuccess) {
       rc = run_chain(s->ctx, s->w, sys_chain(s), st, &pe);
       if (rc != TMD_OK) return rc;
@@ -636,7 +639,7 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
     if (e == hipSuccess)
       e = launch_vv_second(s->n, (const double4 *) s->w.pos.p, (double *) s->vel.p,
                            (const double *) s->w.f.p, (const double *) s->dtfm.p, st);
-    hipEventRecord(E[9], st);
+    if (e == hipSuccess) e = hipEventRecord(E[9], st);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "profile");
@@ -645,7 +648,7 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   auto el = [&](hipEvent_t a, hipEvent_t b) {
     float t = 0.f;
-    hipEventElapsedTime(&t, a, b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
     return (double) t;
   };
   for (int q = 0; q < nsteps; q++) {
@@ -659,6 +662,7 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
     acc[6] += el(E[0], E[9]);
   }
   for (auto &x : ev) hipEventDestroy(x);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "profile timing");
   for (int k = 0; k < 7; k++) ms[k] = acc[k] / nsteps;
   bool over = false;
   rc = check_overflow(s, &over);
@@ -672,7 +676,8 @@ int tmd_system_nbuilds(const tmd_system *s) { return s ? s->nbuilds_seen : 0; }
 int tmd_system_neighbors(tmd_system *s, long cap, long *off, int *j, int *shift, long *total)
 {
   if (!s || !off || !total) return TMD_ERR_ARG;
-  hipSetDevice(s->ctx->device);
+  if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
+    return hip_fail(s->ctx, de, "hipSetDevice");
   std::vector<int> nn(s->n);
   std::vector<uint32_t> list((size_t) s->n * s->nb.cap);
   hipError_t e = hipMemcpy(nn.data(), s->w.nn.p, 4 * (size_t) s->n, hipMemcpyDeviceToHost);
@@ -700,7 +705,8 @@ int tmd_system_neighbors(tmd_system *s, long cap, long *off, int *j, int *shift,
 int tmd_system_descriptors(tmd_system *s, float *G, float *dEdG)
 {
   if (!s) return TMD_ERR_ARG;
-  hipSetDevice(s->ctx->device);
+  if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
+    return hip_fail(s->ctx, de, "hipSetDevice");
   int gw = s->ctx->model.gwidth();
   int nr = s->w.rl.nrows;
   std::vector<float> g((size_t) nr * gw), dg((size_t) nr * gw);

@@ -126,6 +126,39 @@ def _f64_c(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def test_model_file_variants(synth_models, tmp_path):
+    """The same model stored deflate-compressed gives the same results (to
+    the FP64 rounding of the atomically summed f and vatom, whose order of
+    summation varies from run to run; eatom exactly); stored with float32
+    arrays and `precision` 32 (the reference's float models) it matches the
+    oracle reading that same file."""
+    say("case: model file variants (deflate, float32 arrays), C1 model, fcc 4x4x4")
+    src = np.load(synth_models["c1"])  # allow_pickle stays False
+    arrays = {k: src[k] for k in src.files}
+    comp = str(tmp_path / "c1_deflate.npz")
+    np.savez_compressed(comp, **arrays)
+    f32 = str(tmp_path / "c1_f32.npz")
+    a32 = {k: (v.astype(np.float32) if v.dtype == np.float64 else v) for k, v in arrays.items()}
+    a32["precision"] = np.int32(32)
+    np.savez(f32, **a32)
+    x, t, cell = W.crystal("fcc", 3.615, (4, 4, 4), sigma=0.05, seed=24)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, 7.0)
+    tall = t[owner]
+    base = TensorMD(synth_models["c1"], ["Cu"], "fp32", 0.001).compute_list(len(x), xall, tall, ilist,
+                                                                            numneigh, flat)
+    got = TensorMD(comp, ["Cu"], "fp32", 0.001).compute_list(len(x), xall, tall, ilist, numneigh, flat)
+    assert np.array_equal(got["eatom"], base["eatom"])
+    for k in ("f", "vatom", "virial"):
+        assert np.allclose(got[k], base[k], rtol=0, atol=1e-12), k
+    assert abs(got["energy"] - base["energy"]) <= 1e-12 * abs(base["energy"])
+    om = oracle.OracleModel(f32, 0.001)
+    ref = om.compute_list(len(x), xall, tall - 1, ilist, numneigh, flat)
+    got = TensorMD(f32, ["Cu"], "fp32", 0.001).compute_list(len(x), xall, tall, ilist, numneigh, flat)
+    e = _errs(got, ref, len(x))
+    say("float32 file errors", e)
+    assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
+
+
 @pytest.mark.parametrize("mm", [3, 1])
 def test_descriptors_list(tmp_path, mm):
     """compute tensormd/density over a LAMMPS full list: the descriptor row of

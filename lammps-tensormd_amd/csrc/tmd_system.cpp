@@ -616,7 +616,15 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
   if (rc != TMD_OK) return rc;
   hipStream_t st = s->ctx->stream;
   const int NE = 10;
-  std::vector<hipEvent_t> ev((size_t) nsteps * NE);
+  std::vector<hipEvent_t> ev((size_t) nsteps * NE, nullptr);
+  struct EventsGuard {  // destroyed on every return path
+    std::vector<hipEvent_t> &v;
+    ~EventsGuard()
+    {
+      for (auto &x : v)
+        if (x) hipEventDestroy(x);
+    }
+  } guard{ev};
   for (auto &e : ev)
     if (hipEventCreate(&e) != hipSuccess) return sys_fail(s, TMD_ERR_DEVICE, "hipEventCreate");
   hipError_t e = hipSuccess;
@@ -661,7 +669,6 @@ int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
     acc[5] += el(E[0], E[1]) + el(E[8], E[9]);
     acc[6] += el(E[0], E[9]);
   }
-  for (auto &x : ev) hipEventDestroy(x);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "profile timing");
   for (int k = 0; k < 7; k++) ms[k] = acc[k] / nsteps;
   bool over = false;

@@ -225,6 +225,10 @@ def test_list_mode_rejections(synth_models):
                                 _p(nn, _ip), ptrs, 1.0, _p(f, _dp), None, None,
                                 ctypes.byref(en), None)
     assert rc == -1
+    say("case: descriptor rows without an output buffer")
+    rc = lib().tmd_descriptors_list(m.h, n, len(xa), _p(xa, _dp), _p(ta, _ip), 0, _p(il, _ip),
+                                    _p(nn, _ip), None, None)
+    assert rc == -1
 
 
 def test_system_c2_shape(synth_models):

@@ -34,6 +34,8 @@
 #ifndef TENSORMD_AMD_H
 #define TENSORMD_AMD_H
 
+#include <stddef.h> /* size_t */
+
 #ifdef __cplusplus
 extern "C" {
 #endif

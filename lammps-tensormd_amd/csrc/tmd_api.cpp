@@ -19,6 +19,10 @@ using namespace tmd;
 
 namespace {
 thread_local std::string g_thread_err;
+// set on library-owned threads (the serving worker): their errors stay in
+// the thread's own message and never touch a context's message, which the
+// caller's thread reads and writes without a lock
+thread_local bool g_library_thread = false;
 }
 
 namespace tmd {
@@ -26,8 +30,12 @@ namespace tmd {
 void set_err(tmd_ctx *ctx, const std::string &m)
 {
   g_thread_err = m;
-  if (ctx) ctx->err = m;
+  if (ctx && !g_library_thread) ctx->err = m;
 }
+
+void mark_library_thread() { g_library_thread = true; }
+
+const std::string &thread_error() { return g_thread_err; }
 
 int hip_fail(tmd_ctx *ctx, hipError_t e, const char *what)
 {

@@ -227,17 +227,24 @@ __host__ __device__ inline size_t lds_layout(int cap, size_t *offs)
   return o;
 }
 
-// force LDS: the staging area (monomials, radial functions), later P and
-// then S of every slot [nelt][SSTR] over it; the kept list stays in HBM
+// force LDS: the staging area (monomials, radial functions), then P and S
+// of every slot [nelt][SSTR] at offs[0]: over the staging area when P is
+// built in one pass, after it when it takes several (the staging area is
+// still in use while the first passes' P is stored).  The kept list stays
+// in HBM.
 template <int K>
-__host__ __device__ inline size_t force_lds_layout(int nelt, size_t *offs)
+__host__ __device__ inline size_t force_lds_layout(int nelt, bool multipass, size_t *offs)
 {
   const size_t mreg = (size_t) kChunk * kMS * 4;
   const size_t hreg = (size_t) 2 * Tiling<K>::HALF * 4;
   const size_t preg = (size_t) nelt * Tiling<K>::SSTR * 4;
-  offs[0] = 0;
   offs[1] = 0;
   offs[2] = mreg;
+  if (multipass) {
+    offs[0] = (mreg + hreg + 15) & ~(size_t) 15;
+    return offs[0] + preg;
+  }
+  offs[0] = 0;
   return mreg + hreg > preg ? mreg + hreg : preg;
 }
 
@@ -465,9 +472,11 @@ __device__ inline void stage(const ModelGeo &g, const Box &cell, const double4 &
   __syncthreads();
 }
 
-// Phase 2 accumulation over the staged chunk into the lane's register block
+// Phase 2 accumulation over the staged chunk into the lane's register block.
+// The pass covers nsl slots whose kept entries start at start[0..nsl]; slot
+// b of the pass accumulates into acc[b].
 template <int K, int NS>
-__device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int *start, int base,
+__device__ inline void accumulate(const LDSMap &L, const int *start, int nsl, int base,
                                   float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
 {
   using T = Tiling<K>;
@@ -478,7 +487,7 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
   const float *Mb = L.M + dq * T::DT;
 #pragma unroll
   for (int b = 0; b < NS; b++) {
-    if (b >= g.nelt) break;
+    if (b >= nsl) break;
     int c0 = max(start[b] - base, 0), c1 = min(start[b + 1] - base, kChunk);
 #pragma unroll 4
     for (int c = c0; c < c1; c++) {
@@ -495,12 +504,13 @@ __device__ inline void accumulate(const ModelGeo &g, const LDSMap &L, const int 
   }
 }
 
-// Phase 2: P of every slot in the lane's register block over the kept list
-// L.kept[0, nk) with slot starts start[].
+// Phase 2: P of the nsl slots of one pass in the lane's register block, over
+// the kept entries L.kept[start[0], start[nsl]) (the list is slot-sorted, so
+// a pass stages only its own slots' neighbors).
 template <int K, int NS>
 __device__ inline void accumulate_P(const ModelGeo &g, const Box &cell, const double4 &xi,
                                     const double4 *pos, const float4 *tab, LDSMap &L,
-                                    const int *start, int nk,
+                                    const int *start, int nsl,
                                     float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
 {
   using T = Tiling<K>;
@@ -510,28 +520,66 @@ __device__ inline void accumulate_P(const ModelGeo &g, const Box &cell, const do
     for (int a = 0; a < T::KT; a++)
 #pragma unroll
       for (int q = 0; q < T::DT; q++) acc[b][a][q] = 0.0f;
-  for (int base = 0; base < nk;) {
-    const int Lp = lanes_per_nbr(nk - base);
-    stage<K>(g, cell, xi, pos, tab, base, nk, Lp, L);
-    accumulate<K, NS>(g, L, start, base, acc);
+  const int end = start[nsl];
+  for (int base = start[0]; base < end;) {
+    const int Lp = lanes_per_nbr(end - base);
+    stage<K>(g, cell, xi, pos, tab, base, end, Lp, L);
+    accumulate<K, NS>(L, start, nsl, base, acc);
     __syncthreads();
     base += 64 / Lp;
   }
 }
 
-// Phases 1-2: compaction, then P.  Returns nk.
+// G[b0 + b][k][m] = sum_d T[m][d] P^2 (m >= 1) for the nsl slots of a pass:
+// partial sums over the lane's d block, then a sum over the NDG lanes that
+// share k; G[b][k][0] = P0 (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
 template <int K, int NS>
-__device__ inline int build_P(const ModelGeo &g, const Box &cell, const double4 &xi, int ei,
-                              const uint32_t *list, int nraw, const double4 *pos, const float4 *tab,
-                              LDSMap &L, int *start, float (*acc)[Tiling<K>::KT][Tiling<K>::DT])
+__device__ inline void write_G(const ModelGeo &g, float (*acc)[Tiling<K>::KT][Tiling<K>::DT],
+                               float *Grow, int b0, int nsl)
 {
-  int nk = compact(g, cell, xi, ei, list, nraw, pos, L.kept, start);
-  __syncthreads();
-  accumulate_P<K, NS>(g, cell, xi, pos, tab, L, start, nk, acc);
-  return nk;
+  using T = Tiling<K>;
+  const int lane = threadIdx.x & 63;
+  const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
+  const int gbase = kq * T::NDG;
+  float tc[kNM][T::DT];
+#pragma unroll
+  for (int m = 1; m < kNM; m++)
+#pragma unroll
+    for (int q = 0; q < T::DT; q++) tc[m][q] = t_entry(m, dq * T::DT + q, g.tsym);
+#pragma unroll
+  for (int b = 0; b < NS; b++) {
+    if (b >= nsl) break;
+#pragma unroll
+    for (int a = 0; a < T::KT; a++) {
+      float part[kNM] = {0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < T::DT; q++) {
+        float p2 = acc[b][a][q] * acc[b][a][q];
+#pragma unroll
+        for (int m = 1; m < kNM; m++) part[m] += tc[m][q] * p2;
+      }
+      // sum over the NDG lanes of the filter: only the d groups that hold a
+      // monomial of degree m add to G_m (the others' partial sums are exact
+      // zeros; the T-symmetric extras are in the d group 0, the lane itself)
+      float tot[kNM] = {0, part[1], part[2], part[3]};
+#pragma unroll
+      for (int t = 1; t < T::NDG; t++)
+#pragma unroll
+        for (int m = 1; m < kNM; m++)
+          if (group_has_degree(t * T::DT, T::DT, m)) tot[m] += __shfl(part[m], gbase + t);
+      if (lane < T::LANES && dq == 0) {
+        const int k = kq * T::KT + a;
+        *reinterpret_cast<float4 *>(Grow + ((size_t) (b0 + b) * K + k) * kNM) =
+            make_float4(acc[b][a][0], tot[1], tot[2], tot[3]);
+      }
+    }
+  }
 }
 
-template <int K, int NS>
+// NS: slots accumulated per register pass.  MP: the model has more slots
+// than NS, and the kernel runs ceil(nelt / NS) passes over the slot-sorted
+// kept list (each neighbor is still staged once).
+template <int K, int NS, bool MP>
 __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const double4 *pos,
                                                    const uint32_t *nbr, const int *nn, int cap,
                                                    const int *row2atom, int nrows,
@@ -553,54 +601,34 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
   int start[kMaxElts + 1];
   const double4 xi = pos[i];
   const Box cell = load_box(geo, i);
-  float acc[NS][T::KT][T::DT];
-  const int nk =
-      build_P<K, NS>(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, tab, L, start, acc);
+  // phase 1: compaction
+  const int nk = compact(g, cell, xi, (int) xi.w, nbr + (size_t) i * cap, nn[i], pos, L.kept, start);
+  __syncthreads();
   // the row's kept list with its slot starts, for the force kernel
-  {
+  auto write_kept = [&]() {
     uint32_t *kr = kl + (size_t) row * kept_cache_words(cap);
     if (lane == 0) {
       kr[0] = (uint32_t) nk;
       for (int b = 0; b <= g.nelt; b++) kr[1 + b] = (uint32_t) start[b];
     }
     for (int e = lane; e < nk; e += 64) kr[kKeptHdr + e] = L.kept[e];
-  }
-  // G[b][k][m] = sum_d T[m][d] P^2 (m >= 1): partial sums over the lane's d
-  // block, then a sum over the NDG lanes that share k; G[b][k][0] = P0
-  // (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
-  const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
-  const int gbase = kq * T::NDG;
-  float tc[kNM][T::DT];
-#pragma unroll
-  for (int m = 1; m < kNM; m++)
-#pragma unroll
-    for (int q = 0; q < T::DT; q++) tc[m][q] = t_entry(m, dq * T::DT + q, g.tsym);
-#pragma unroll
-  for (int b = 0; b < NS; b++) {
-    if (b >= g.nelt) break;
-#pragma unroll
-    for (int a = 0; a < T::KT; a++) {
-      float part[kNM] = {0, 0, 0, 0};
-#pragma unroll
-      for (int q = 0; q < T::DT; q++) {
-        float p2 = acc[b][a][q] * acc[b][a][q];
-#pragma unroll
-        for (int m = 1; m < kNM; m++) part[m] += tc[m][q] * p2;
-      }
-      // sum over the NDG lanes of the filter: only the d groups that hold a
-      // monomial of degree m add to G_m (the others' partial sums are exact
-      // zeros; the T-symmetric extras are in the d group 0, the lane itself)
-      float tot[kNM] = {0, part[1], part[2], part[3]};
-#pragma unroll
-      for (int t = 1; t < T::NDG; t++)
-#pragma unroll
-        for (int m = 1; m < kNM; m++)
-          if (group_has_degree(t * T::DT, T::DT, m)) tot[m] += __shfl(part[m], gbase + t);
-      if (lane < T::LANES && dq == 0) {
-        const int k = kq * T::KT + a;
-        *reinterpret_cast<float4 *>(Grow + ((size_t) b * K + k) * kNM) =
-            make_float4(acc[b][a][0], tot[1], tot[2], tot[3]);
-      }
+  };
+  if constexpr (!MP) {
+    float acc[NS][T::KT][T::DT];
+    accumulate_P<K, NS>(g, cell, xi, pos, tab, L, start, g.nelt, acc);
+    write_kept();
+    write_G<K, NS>(g, acc, Grow, 0, g.nelt);
+  } else {
+    __shared__ int s_start[kMaxElts + 1];
+    if (lane == 0)
+      for (int b = 0; b <= g.nelt; b++) s_start[b] = start[b];
+    write_kept();
+    __syncthreads();
+    for (int b0 = 0; b0 < g.nelt; b0 += NS) {
+      const int nsl = min(NS, g.nelt - b0);
+      float acc[NS][T::KT][T::DT];
+      accumulate_P<K, NS>(g, cell, xi, pos, tab, L, s_start + b0, nsl, acc);
+      write_G<K, NS>(g, acc, Grow, b0, nsl);
     }
   }
 }
@@ -746,7 +774,28 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
 template <int K, int NS>
 constexpr int kForceWaves = (K <= 24 && NS * Tiling<K>::KT * Tiling<K>::DT <= 40) ? 4 : 1;
 
+// P of the nsl slots of a pass -> LDS [b0 + b][k][d]
 template <int K, int NS>
+__device__ inline void store_P(float (*acc)[Tiling<K>::KT][Tiling<K>::DT], float *PS, int b0, int nsl)
+{
+  using T = Tiling<K>;
+  const int lane = threadIdx.x & 63;
+  if (lane < T::LANES) {
+    const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
+#pragma unroll
+    for (int b = 0; b < NS; b++) {
+      if (b >= nsl) break;
+#pragma unroll
+      for (int a = 0; a < T::KT; a++)
+#pragma unroll
+        for (int q = 0; q < T::DT; q++)
+          PS[(size_t) (b0 + b) * T::SSTR + (kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
+    }
+  }
+}
+
+// NS, MP: as for k_descriptor
+template <int K, int NS, bool MP>
 __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, Geom geo, const double4 *pos, int cap,
                                               const int *row2atom, int nrows, const float4 *tab,
                                               const uint32_t *kl, const float *dEdG,
@@ -761,11 +810,11 @@ __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, 
   const int i = row2atom[row];
   if (i < 0) return;  // padding row
   size_t offs[3];
-  force_lds_layout<K>(g.nelt, offs);
+  force_lds_layout<K>(g.nelt, MP, offs);
   LDSMap L;
   L.M = reinterpret_cast<float *>(smem + offs[1]);
   L.H = reinterpret_cast<float *>(smem + offs[2]);
-  float *PS = reinterpret_cast<float *>(smem);  // over the staging area once P is built
+  float *PS = reinterpret_cast<float *>(smem + offs[0]);
   const double4 xi = pos[i];
   const Box cell = load_box(geo, i);
   // the descriptor kernel's kept list of this row (slot-sorted, within rc)
@@ -775,19 +824,21 @@ __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, 
   for (int b = 0; b <= g.nelt; b++) start[b] = (int) kr[1 + b];
   const uint32_t *kept = kr + kKeptHdr;
   L.kept = const_cast<uint32_t *>(kept);
-  float acc[NS][T::KT][T::DT];
-  accumulate_P<K, NS>(g, cell, xi, pos, tab, L, start, nk, acc);
-  // P -> LDS [b][k][d] (over the staging area; accumulate_P ends with a barrier)
-  if (lane < T::LANES) {
-    const int kq = lane / T::NDG, dq = lane - (lane / T::NDG) * T::NDG;
-#pragma unroll
-    for (int b = 0; b < NS; b++) {
-      if (b >= g.nelt) break;
-#pragma unroll
-      for (int a = 0; a < T::KT; a++)
-#pragma unroll
-        for (int q = 0; q < T::DT; q++)
-          PS[(size_t) b * T::SSTR + (kq * T::KT + a) * kND + dq * T::DT + q] = acc[b][a][q];
+  if constexpr (!MP) {
+    float acc[NS][T::KT][T::DT];
+    accumulate_P<K, NS>(g, cell, xi, pos, tab, L, start, g.nelt, acc);
+    // P -> LDS over the staging area (accumulate_P ends with a barrier)
+    store_P<K, NS>(acc, PS, 0, g.nelt);
+  } else {
+    __shared__ int s_start[kMaxElts + 1];
+    if (lane == 0)
+      for (int b = 0; b <= g.nelt; b++) s_start[b] = start[b];
+    __syncthreads();
+    for (int b0 = 0; b0 < g.nelt; b0 += NS) {
+      const int nsl = min(NS, g.nelt - b0);
+      float acc[NS][T::KT][T::DT];
+      accumulate_P<K, NS>(g, cell, xi, pos, tab, L, s_start + b0, nsl, acc);
+      store_P<K, NS>(acc, PS, b0, nsl);
     }
   }
   __syncthreads();
@@ -852,7 +903,7 @@ __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, 
   if (lane == 6) eatom[i] = scale * E[row];
 }
 
-template <int K, int NS>
+template <int K, int NS, bool MP>
 hipError_t desc_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
                      const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
                      float *G, uint32_t *kl, hipStream_t s)
@@ -860,34 +911,35 @@ hipError_t desc_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, con
   size_t offs[3];
   size_t lds = lds_layout<K>(cap, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
-  if (hipError_t ae = hipFuncSetAttribute((const void *) k_descriptor<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int) lds);
+  if (hipError_t ae = hipFuncSetAttribute((const void *) k_descriptor<K, NS, MP>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
       ae != hipSuccess)
     return ae;
-  k_descriptor<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl);
+  k_descriptor<K, NS, MP><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl);
   return hipGetLastError();
 }
 
-template <int K, int NS>
+template <int K, int NS, bool MP>
 hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
                       const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
                       const float *dEdG, const double *E, double scale, double *f, double *eatom,
                       double *watom, double *vatom, hipStream_t s)
 {
   size_t offs[3];
-  const size_t lds = force_lds_layout<K>(g.nelt, offs);
+  const size_t lds = force_lds_layout<K>(g.nelt, MP, offs);
   if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
-  if (hipError_t ae = hipFuncSetAttribute((const void *) k_force<K, NS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int) lds);
+  if (hipError_t ae = hipFuncSetAttribute((const void *) k_force<K, NS, MP>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
       ae != hipSuccess)
     return ae;
-  k_force<K, NS><<<nrows, 64, lds, s>>>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale,
-                                        f, eatom, watom, vatom);
+  k_force<K, NS, MP><<<nrows, 64, lds, s>>>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E,
+                                            scale, f, eatom, watom, vatom);
   return hipGetLastError();
 }
 
-// slot capacity: the smallest of 1, 2, 4, 8 that holds nelt, within a
-// register budget of 128 accumulators per lane
+// Slots per register pass: the smallest of 1, 2, 4, 8 that holds nelt
+// within a budget of 128 accumulators per lane; models with more slots than
+// the largest such count (k = 64: 4, k = 128: 2) take several passes.
 template <int K, bool FORCE>
 hipError_t dispatch(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
                     const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
@@ -895,21 +947,25 @@ hipError_t dispatch(const ModelGeo &g, const Geom &geo, const double4 *pos, cons
                     double *f, double *eatom, double *watom, double *vatom, hipStream_t s)
 {
   constexpr int per = Tiling<K>::KT * Tiling<K>::DT;
-#define TMD_NS_CASE(NS)                                                                             \
-  if (g.nelt <= NS) {                                                                               \
-    if (NS * per > 128) return hipErrorInvalidConfiguration;                                        \
-    if (FORCE)                                                                                      \
-      return force_impl<K, (NS * per > 128 ? 1 : NS)>(g, geo, pos, cap, row2atom, nrows, tab, kl,   \
-                                                       dEdG, E, scale, f, eatom, watom, vatom, s);  \
-    return desc_impl<K, (NS * per > 128 ? 1 : NS)>(g, geo, pos, nbr, nn, cap, row2atom, nrows,      \
-                                                   tab, G, kl, s);                                   \
+  constexpr int kMaxNS = 8 * per <= 128 ? 8 : (4 * per <= 128 ? 4 : (2 * per <= 128 ? 2 : 1));
+#define TMD_LAUNCH(NS, MP)                                                                          \
+  return FORCE ? force_impl<K, NS, MP>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale, \
+                                       f, eatom, watom, vatom, s)                                   \
+               : desc_impl<K, NS, MP>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, s);
+  static_assert(kMaxNS >= 2, "every filter count holds two slots per pass");
+  if (g.nelt < 1 || g.nelt > kMaxElts) return hipErrorInvalidValue;
+  if (g.nelt <= 1) TMD_LAUNCH(1, false)
+  if (g.nelt <= 2) TMD_LAUNCH(2, false)
+  if constexpr (kMaxNS >= 8) {
+    if (g.nelt <= 4) TMD_LAUNCH(4, false)
+    TMD_LAUNCH(8, false)
+  } else if constexpr (kMaxNS == 4) {
+    if (g.nelt <= 4) TMD_LAUNCH(4, false)
+    TMD_LAUNCH(4, true)
+  } else {
+    TMD_LAUNCH(2, true)
   }
-  TMD_NS_CASE(1)
-  TMD_NS_CASE(2)
-  TMD_NS_CASE(4)
-  TMD_NS_CASE(8)
-#undef TMD_NS_CASE
-  return hipErrorInvalidConfiguration;
+#undef TMD_LAUNCH
 }
 
 }  // namespace

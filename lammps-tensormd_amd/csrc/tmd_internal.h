@@ -76,7 +76,14 @@ namespace tmd {
 struct PhaseEvents {
   hipEvent_t ev[6];
 };
+// Records an error message for the calling thread and, on a caller's
+// thread, in ctx->err (tmd_last_error).
 void set_err(tmd_ctx *ctx, const std::string &m);
+// Called first on a thread the library starts itself: set_err then leaves
+// every ctx->err alone on that thread.
+void mark_library_thread();
+// The calling thread's last error message.
+const std::string &thread_error();
 int hip_fail(tmd_ctx *ctx, hipError_t e, const char *what);
 int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows);
 // the kept-list cache for nrows rows of a list with `cap` entries per atom

@@ -188,7 +188,7 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
   };
   try {
     m.nelt = geti("nelt", 0, true);
-    if (m.nelt < 1 || m.nelt > 8) throw std::string("nelt must be in 1..8");
+    if (m.nelt < 1 || m.nelt > kMaxModelElts) throw std::string("nelt must be in 1..8");
     m.rc = getd("rmax").at(0);
     if (!(m.rc > 0)) throw std::string("rmax must be > 0");
     m.max_moment = geti("max_moment", 0, true);
@@ -241,8 +241,9 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
     }
     m.k = geti("fnn::num_filters", 0, true);
     if (f.sizes.back() != m.k) throw std::string("fnn output width != fnn::num_filters");
-    if (m.k % 4 != 0 || m.k > 128)
-      throw std::string("num_filters must be a multiple of 4 and <= 128");
+    if (!filters_supported(m.k))
+      throw std::string("fnn::num_filters " + std::to_string(m.k) +
+                        " is not supported: the descriptor kernels take 16, 24, 32, 64 or 128");
     // main networks
     int nl = geti("nlayers", 0, true);
     std::vector<int> ls = z.at("layer_sizes").as_ints();

@@ -10,6 +10,13 @@ namespace tmd {
 
 enum Act { ACT_RELU = 0, ACT_SOFTPLUS = 1, ACT_TANH = 2, ACT_SQUAREPLUS = 3 };
 
+// Radial filter counts the descriptor and force kernels are instantiated
+// for.  Every element count up to kMaxModelElts runs with each of them:
+// models whose slots do not fit one register pass (k = 64 with more than 4
+// elements, k = 128 with more than 2) accumulate the slots in several passes.
+constexpr bool filters_supported(int k) { return k == 16 || k == 24 || k == 32 || k == 64 || k == 128; }
+constexpr int kMaxModelElts = 8;
+
 struct DenseNet {
   int nlayers = 0;
   std::vector<int> sizes;  // sizes[0] = input width, sizes[l + 1] = output of layer l

@@ -534,6 +534,7 @@ void take_batch(Server *S, Slot &sl)
 
 void worker_main(Server *S)
 {
+  mark_library_thread();  // errors go to S->fatal, never to ctx->err
   hipSetDevice(S->ctx->device);
   for (;;) {
     bool progressed = false, any_busy = false;
@@ -589,7 +590,7 @@ void worker_main(Server *S)
       stage_batch(S, sl);
       if (launch_batch(S, sl) != TMD_OK) {
         std::lock_guard<std::mutex> lk(S->mu);
-        S->fatal = "serve: launch failed: " + S->ctx->err;
+        S->fatal = "serve: launch failed: " + thread_error();
       }
       if (!sl.busy) fail_batch(S, sl, TMD_ERR_DEVICE);
       progressed = true;

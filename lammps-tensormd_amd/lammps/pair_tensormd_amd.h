@@ -41,7 +41,10 @@ class PairTensorMDAMD : public Pair {
   double memory_usage() override;
 
   double etemp;  // accepted for input compatibility; TDNP models are rejected
-  tmd_ctx *context() const { return ctx; }  // for compute tensormd/density/amd
+  tmd_ctx *context() const { return ctx; }
+  // the pair_coeff arguments after "* *" (model file, symbols, keywords),
+  // for compute tensormd/density/amd, which loads the model itself
+  const std::vector<std::string> &coeff_args() const { return kwargs; }
 
  protected:
   tmd_ctx *ctx;

@@ -103,3 +103,44 @@ def test_valid_model_reaches_device_init(synth_models, tmp_path):
         assert any(i.compress_type == zipfile.ZIP_DEFLATED for i in z.infolist())
     rc, msg = _create_rc(str(p), ["Cu"])
     assert rc == -3, msg
+
+
+@pytest.mark.parametrize("lang,std", [("c", "c99"), ("c", "c11"), ("c++", "c++11"), ("c++", "c++17")])
+def test_header_is_standalone(tmp_path, lang, std):
+    """include/tensormd_amd.h compiles on its own as C and as C++."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc" if lang == "c" else "g++")
+    if cc is None:
+        pytest.skip("no host compiler")
+    src = tmp_path / ("use." + ("c" if lang == "c" else "cpp"))
+    src.write_text('#include "tensormd_amd.h"\n'
+                   "int probe(size_t n) { tmd_options o; tmd_options_default(&o); return (int) n + o.device; }\n")
+    r = subprocess.run([cc, "-x", lang, f"-std={std}", "-Wall", "-Werror", "-pedantic", "-fsyntax-only",
+                        "-I" + os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("k", [12, 48, 96])
+def test_unsupported_filter_count_is_model_error(tmp_path, k):
+    """Radial widths the descriptor kernels are not built for are refused at
+    load (TMD_ERR_MODEL), before any device work, not at the first compute."""
+    from tensormd_amd import workloads as W
+    p = W.write_model(str(tmp_path / f"k{k}.npz"), symbols=["Mo", "W"], masses=[95.95, 183.84], rmax=5.0,
+                      num_filters=k, layer_sizes=[32, 1], seed=3)
+    rc, msg = _create_rc(p, ["Mo", "W"])
+    assert rc == -2 and "num_filters" in msg, msg
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device path")
+@pytest.mark.parametrize("k,nelt", [(64, 5), (128, 3), (128, 5), (24, 8)])
+def test_wide_slot_models_load(tmp_path, k, nelt):
+    """Models whose slots need several register passes (k = 64 with 5
+    elements, k = 128 with 3 or more) and 8-element models pass the model
+    checks: without a GPU the call fails at device initialisation."""
+    from tensormd_amd import workloads as W
+    sym = ["Mo", "Nb", "Ta", "W", "V", "Cr", "Ti", "Zr"][:nelt]
+    p = W.write_model(str(tmp_path / "m.npz"), symbols=sym, masses=[50.0 + 10 * i for i in range(nelt)],
+                      rmax=5.0, num_filters=k, layer_sizes=[32, 1], seed=4)
+    rc, msg = _create_rc(p, sym)
+    assert rc == -3, msg

@@ -426,6 +426,49 @@ def test_k128_model_vs_oracle():
     assert e["dE"] <= e_tol
 
 
+@pytest.mark.parametrize("k,nelt", [(64, 5), (128, 3), (128, 5), (32, 8)])
+def test_wide_slot_models(tmp_path, k, nelt):
+    """Models whose slots take several register passes in the descriptor and
+    force kernels (k = 64 with 5 elements, the reference's element limit,
+    tensormd.h:27; k = 128 with 3 and 5 elements), and an 8-element model
+    in one pass: FP32 path vs the FP64 oracle, energies, forces, virial and
+    the descriptor rows themselves."""
+    sym = ["Mo", "Nb", "Ta", "W", "V", "Cr", "Ti", "Zr"][:nelt]
+    say(f"case: k={k}, {nelt} elements, bcc 4x4x4 jittered")
+    p = W.write_model(str(tmp_path / "wide.npz"), symbols=sym, masses=[50.0 + 10 * i for i in range(nelt)],
+                      rmax=5.5, num_filters=k, layer_sizes=[64, 32, 1], actfn="tanh", seed=50 + k + nelt,
+                      radial_scale=0.2)
+    x, t, cell = W.crystal("bcc", 3.2, (4, 4, 4), nelt=nelt, sigma=0.08, seed=47)
+    om = oracle.OracleModel(p, 0.001)
+    ref = om.compute_periodic(x, t - 1, cell, want_desc=True)
+    m = TensorMD(p, sym, "fp32", 0.001)
+    s = m.system(x, t, cell, skin=1.0)
+    got = s.compute()
+    e = _errs(got, ref, len(x))
+    G, dG = s.descriptors(m.info()["width"])
+    rG = float(abs(G - ref["G"]).max() / max(1.0, abs(ref["G"]).max()))
+    rdG = float(abs(dG - ref["dEdG"]).max() / max(1.0, abs(ref["dEdG"]).max()))
+    say("errors", e, "rel dG", rG, "rel d(dE/dG)", rdG)
+    assert rG <= 1e-5 and rdG <= 1e-5
+    assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
+
+
+@pytest.mark.parametrize("resnet", [False, True])
+def test_relu_model(tmp_path, resnet):
+    """ReLU main network (actfn 0), with and without resnet skips: FP32 path
+    vs the FP64 oracle (whose ReLU network is pinned by torch autograd)."""
+    say(f"case: relu model resnet={resnet}, 2 elements, fcc 4x4x4 jittered")
+    p = W.write_model(str(tmp_path / "relu.npz"), symbols=["Al", "Ni"], masses=[26.98, 58.69], rmax=5.0,
+                      num_filters=16, layer_sizes=[64, 64, 32, 1], actfn="relu", use_resnet_dt=resnet,
+                      seed=22, radial_scale=0.2, bias_sigma=0.3)
+    x, t, cell = W.crystal("fcc", 3.8, (4, 4, 4), nelt=2, sigma=0.06, seed=48)
+    ref = oracle.OracleModel(p, 0.001).compute_periodic(x, t - 1, cell)
+    got = TensorMD(p, ["Al", "Ni"], "fp32", 0.001).system(x, t, cell, skin=1.0).compute()
+    e = _errs(got, ref, len(x))
+    say("errors", e)
+    assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
+
+
 def test_bf16_mlp_error(synth_models):
     say("case: bf16 MLP, C2 model, bcc 8x8x8")
     om = oracle.OracleModel(synth_models["c2"], 0.001)

@@ -135,6 +135,37 @@ def c2_path(m):
     return m._path
 
 
+@pytest.mark.parametrize("resnet", [False, True])
+def test_relu_mlp_gradient_against_torch(tmp_path, resnet):
+    """ReLU (actfn 0): the oracle's energy and dE/dG equal torch autograd of
+    the textbook network, with and without the resnet skip of equal-width
+    hidden layers (nnp.cpp:179-181)."""
+    torch = pytest.importorskip("torch")
+    sizes = [32, 32, 16, 1]
+    p = W.write_model(str(tmp_path / "relu.npz"), symbols=["Al", "Ni"], masses=[26.98, 58.69], rmax=5.0,
+                      num_filters=16, layer_sizes=sizes, actfn="relu", use_resnet_dt=resnet, seed=21,
+                      bias_sigma=0.3)
+    m = oracle.OracleModel(p, 0.001)
+    rng = np.random.default_rng(2)
+    G = rng.normal(size=(64, m.gwidth)) * 0.5
+    E, dG = m.mlp(1, G)
+    with np.load(p, allow_pickle=False) as z:
+        Ws = [torch.tensor(z[f"weights_1_{l}"]).reshape(-1, sizes[l]) for l in range(4)]
+        bs = [torch.tensor(z[f"biases_1_{l}"]).reshape(-1) for l in range(4)]
+    g = torch.tensor(G, requires_grad=True)
+    h = g
+    for l in range(3):
+        a = torch.relu(h @ Ws[l] + bs[l])
+        h = a + h if (resnet and l > 0 and Ws[l].shape[0] == Ws[l].shape[1]) else a
+    e = (h @ Ws[3] + bs[3]).reshape(-1)
+    e.sum().backward()
+    # some units are off and some on, so the test exercises both branches
+    z0 = (g @ Ws[0] + bs[0]).detach().numpy()
+    assert (z0 > 0).mean() > 0.2 and (z0 < 0).mean() > 0.2
+    assert np.allclose(E, e.detach().numpy(), rtol=1e-13, atol=1e-13)
+    assert np.allclose(dG, g.grad.numpy(), rtol=1e-11, atol=1e-13)
+
+
 def test_forces_are_minus_energy_gradient(synth_models):
     for key, sig in [("c2", 0.1), ("sq2", 0.08), ("sp3", 0.08)]:
         m = oracle.OracleModel(synth_models[key], 0.001)

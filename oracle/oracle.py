@@ -63,6 +63,7 @@ def lib():
                                ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
         L.tmdo_num_threads.restype = ctypes.c_int
         L.tmdo_set_threads.argtypes = [ctypes.c_int]
+        L.tmdo_set_emulation.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -121,6 +122,16 @@ class OracleModel:
         if getattr(self, "h", None) and _lib is not None:
             _lib.tmdo_free(self.h)
             self.h = None
+
+    # Stages of the chain evaluated in FP32 by the reduced-precision
+    # emulation (tmd_oracle.cpp atom_chain_emul): it prices the rounding
+    # floor of an FP32 evaluation; it is not the reference.
+    EMUL_DESC, EMUL_MLP, EMUL_FORCE, EMUL_FORCE_TEMPLATE = 1, 2, 4, 8
+
+    def set_emulation(self, flags):
+        """0 = the FP64 chain; otherwise a bitmask of EMUL_* stages in FP32."""
+        lib().tmdo_set_emulation(self.h, int(flags))
+        return self
 
     # ---- pieces -------------------------------------------------------
     def T(self):

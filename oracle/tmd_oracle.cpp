@@ -66,6 +66,10 @@ struct Model {
   std::vector<double> tab;
   bool ready = false;
   std::string err;
+  // Reduced-precision emulation (tmdo_set_emulation): bit 0 descriptor,
+  // bit 1 MLP, bit 2 force stage in FP32; bit 3 runs the emulated chain
+  // even when no stage is FP32 (it must then agree with the FP64 chain).
+  int emul = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -367,8 +371,8 @@ struct AtomOut {
   std::vector<double> G, dEdG;
 };
 
-void atom_chain(const Model &m, int ei, const std::vector<Nbr> &nb, double scale, AtomOut &out,
-                bool keep_desc)
+void atom_chain_fp64(const Model &m, int ei, const std::vector<Nbr> &nb, double scale, AtomOut &out,
+                     bool keep_desc)
 {
   const int k = m.k, nd = m.nd, nm = m.nm, nb_slots = m.nelt;
   const double rc2 = m.rc * m.rc;
@@ -471,6 +475,265 @@ void atom_chain(const Model &m, int ei, const std::vector<Nbr> &nb, double scale
   }
 }
 
+// ---------------------------------------------------------------------------
+// Reduced-precision emulation of the device chain (test infrastructure: it
+// prices the rounding floor of an FP32 evaluation, it is not a reference).
+// The algorithm is atom_chain_fp64's; the arithmetic type of each stage is a
+// template parameter:
+//   RD  descriptor: unit vector, monomials, table interpolation (table values
+//       rounded to RD, the device's expression order), P = sum H M, G;
+//   RM  MLP hidden layers forward and backward (output layer: RM products
+//       summed in FP64 with the bias rounded to RM, as the device does);
+//   RF  force chain: S, U, V, F1, F2 per neighbor.
+// Displacements, r and the table cell stay FP64 in every stage, and with
+// every type double the result equals atom_chain_fp64 to rounding.
+template <class R>
+inline R fmar(R a, R b, R c)
+{
+  return std::fma(a, b, c);
+}
+
+template <class R>
+void radial_emul(const Model &m, double r, R *H, R *dH)
+{
+  int k = m.k;
+  int i = std::min(m.tab_n - 2, (int) (r * m.xdx));
+  if (i <= 0 || i == m.tab_n - 2) {
+    for (int j = 0; j < k; j++) H[j] = dH[j] = R(0);
+    return;
+  }
+  double qd = r - (double) i * m.dx;
+  const R q = (R) qd, p = (R) (m.dx - qd), qp = q - p, dx = (R) m.dx;
+  const double *Y = &m.tab[((size_t) i * 4 + 0) * k];
+  const double *S = &m.tab[((size_t) i * 4 + 1) * k];
+  const double *C1 = &m.tab[((size_t) i * 4 + 2) * k];
+  const double *C2 = &m.tab[((size_t) i * 4 + 3) * k];
+  for (int j = 0; j < k; j++) {
+    const R y = (R) Y[j], s = (R) S[j], c1 = (R) C1[j], c2 = (R) C2[j];
+    R a = std::fabs(c2) * std::fabs(p);
+    R den = fmar(std::fabs(c1), std::fabs(q), a);
+    R w = den > R(0) ? a / den : R(0);
+    R dc = c1 - c2;
+    R c = fmar(w, dc, c2);
+    H[j] = fmar(q, fmar(-p, c, s), y);
+    dH[j] = fmar(dx * dc, fmar(-w, w, w), fmar(qp, c, s));
+  }
+}
+
+template <class R>
+inline R act_emul(int act, R z, R *dz)
+{
+  switch (act) {
+    case SOFTPLUS: {
+      // the device's FP32 threshold is -(log(eps_f32) + 2); FP64 keeps eps_f64
+      const R thr = sizeof(R) == 4 ? R(13.942385) : R(-(std::log(2.22045e-16) + 2.0));
+      R e = std::exp(z);
+      R h = z < thr ? std::log1p(e) : z;
+      *dz = R(1) / (R(1) + R(1) / e);
+      return h;
+    }
+    case SQUAREPLUS: {
+      R s = std::sqrt(z * z + R(4));
+      R h = z >= R(0) ? R(0.5) * (z + s) : R(2) / (s - z);
+      *dz = h / s;
+      return h;
+    }
+    case TANH: {
+      R h = std::tanh(z);
+      *dz = R(1) - h * h;
+      return h;
+    }
+    default:
+      *dz = z >= R(0) ? R(1) : R(0);
+      return z >= R(0) ? z : R(0);
+  }
+}
+
+template <class RM>
+double mlp_emul(const Dense &net, const std::vector<double> &G, std::vector<double> &gx)
+{
+  const int L = net.nlayers;
+  std::vector<std::vector<RM>> h(L), dz(L);
+  h[0].resize(net.sizes[0]);
+  for (int i = 0; i < net.sizes[0]; i++) h[0][i] = (RM) G[i];
+  double E = 0.0;
+  for (int l = 0; l < L; l++) {
+    const int K = net.sizes[l], N = net.sizes[l + 1];
+    const double *W = net.W[l].data();
+    if (l == L - 1) {
+      for (int i = 0; i < K; i++) E += (double) (h[l][i] * (RM) W[i]);
+      if (!net.B[l].empty()) E += (double) (RM) net.B[l][0];
+      break;
+    }
+    std::vector<RM> z(N, RM(0));
+    for (int i = 0; i < K; i++)
+      for (int j = 0; j < N; j++) z[j] = fmar(h[l][i], (RM) W[(size_t) i * N + j], z[j]);
+    const bool res = net.resnet && l > 0 && K == N;
+    h[l + 1].resize(N);
+    dz[l].resize(N);
+    for (int j = 0; j < N; j++) {
+      RM zz = net.B[l].empty() ? z[j] : z[j] + (RM) net.B[l][j];
+      RM v = act_emul<RM>(net.act, zz, &dz[l][j]);
+      h[l + 1][j] = res ? v + h[l][j] : v;
+    }
+  }
+  // backward, seed 1 at the output
+  std::vector<RM> g(net.sizes[L - 1]);
+  for (int j = 0; j < net.sizes[L - 1]; j++) g[j] = (RM) net.W[L - 1][j];
+  for (int l = L - 2; l >= 0; l--) {
+    const int K = net.sizes[l], N = net.sizes[l + 1];
+    const double *W = net.W[l].data();
+    std::vector<RM> gz(N), gin(K, RM(0));
+    for (int j = 0; j < N; j++) gz[j] = g[j] * dz[l][j];
+    for (int i = 0; i < K; i++) {
+      RM s = RM(0);
+      for (int j = 0; j < N; j++) s = fmar((RM) W[(size_t) i * N + j], gz[j], s);
+      gin[i] = s;
+    }
+    if (net.resnet && l > 0 && K == N)
+      for (int i = 0; i < K; i++) gin[i] += g[i];
+    g.swap(gin);
+  }
+  gx.resize(net.sizes[0]);
+  for (int i = 0; i < net.sizes[0]; i++) gx[i] = (double) g[i];
+  return E;
+}
+
+template <class R>
+void monomials_emul(const Model &m, const R u[3], R *M, R *dMdu)
+{
+  auto pw = [](R x, int e) {
+    R r = R(1);
+    for (int i = 0; i < e; i++) r *= x;
+    return r;
+  };
+  for (int d = 0; d < m.nd; d++) {
+    int a = m.mono[3 * d], b = m.mono[3 * d + 1], c = m.mono[3 * d + 2];
+    R px = pw(u[0], a), py = pw(u[1], b), pz = pw(u[2], c);
+    M[d] = px * py * pz;
+    if (dMdu) {
+      dMdu[0 * m.nd + d] = a ? R(a) * pw(u[0], a - 1) * py * pz : R(0);
+      dMdu[1 * m.nd + d] = b ? R(b) * px * pw(u[1], b - 1) * pz : R(0);
+      dMdu[2 * m.nd + d] = c ? R(c) * px * py * pw(u[2], c - 1) : R(0);
+    }
+  }
+}
+
+template <class RD, class RM, class RF>
+void atom_chain_emul(const Model &m, int ei, const std::vector<Nbr> &nb, double scale,
+                     AtomOut &out, bool keep_desc)
+{
+  const int k = m.k, nd = m.nd, nm = m.nm, nslot = m.nelt;
+  const double rc2 = m.rc * m.rc;
+  const size_t n = nb.size();
+  std::vector<int> keep(n, 0), slot(n, 0);
+  std::vector<double> r(n);
+  std::vector<RD> u(3 * n), M(n * nd), H(n * k), dH(n * k);
+  std::vector<RD> P((size_t) nslot * k * nd, RD(0));
+  // slot-sorted kept order (the device's compaction), list order within a slot
+  std::vector<size_t> order;
+  for (int b = 0; b < nslot; b++)
+    for (size_t c = 0; c < n; c++) {
+      const double *dl = nb[c].d;
+      double r2 = dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2];
+      if (!(r2 < rc2 && r2 > 1e-10)) continue;
+      if (m.eltij[ei][nb[c].ej] != b) continue;
+      order.push_back(c);
+      keep[c] = 1;
+      slot[c] = b;
+      r[c] = std::sqrt(r2);
+    }
+  for (size_t c : order) {
+    const double rinv = 1.0 / r[c];
+    for (int a = 0; a < 3; a++) u[3 * c + a] = (RD) (nb[c].d[a] * rinv);
+    monomials_emul<RD>(m, &u[3 * c], &M[c * nd], nullptr);
+    radial_emul<RD>(m, r[c], &H[c * k], &dH[c * k]);
+    RD *Pb = &P[(size_t) slot[c] * k * nd];
+    for (int j = 0; j < k; j++)
+      for (int d = 0; d < nd; d++) Pb[j * nd + d] = fmar(H[c * k + j], M[c * nd + d], Pb[j * nd + d]);
+  }
+  const int gw = nslot * k * nm;
+  std::vector<double> G(gw, 0.0);
+  for (int b = 0; b < nslot; b++)
+    for (int j = 0; j < k; j++) {
+      const RD *Pp = &P[((size_t) b * k + j) * nd];
+      for (int mm = 0; mm < nm; mm++) {
+        RD s = RD(0);
+        for (int d = 0; d < nd; d++) s = fmar((RD) m.T[mm * nd + d] * Pp[d], Pp[d], s);
+        G[(b * k + j) * nm + mm] = (double) s;
+      }
+      G[(b * k + j) * nm + 0] = (double) Pp[0];  // sign(P0) sqrt(P0^2)
+    }
+  std::vector<double> g;
+  out.E = mlp_emul<RM>(m.mlp[ei], G, g);
+  if (keep_desc) {
+    out.G = G;
+    out.dEdG = g;
+  }
+  std::vector<RF> S((size_t) nslot * k * nd);
+  for (int b = 0; b < nslot; b++)
+    for (int j = 0; j < k; j++) {
+      RF gp[8];
+      for (int mm = 0; mm < nm; mm++) gp[mm] = (RF) g[(b * k + j) * nm + mm];
+      const RF G0 = (RF) G[(b * k + j) * nm + 0];
+      gp[0] = std::fabs(G0) < RF(1e-8) ? RF(0) : gp[0] / G0 * RF(0.5);
+      const RD *Pp = &P[((size_t) b * k + j) * nd];
+      for (int d = 0; d < nd; d++) {
+        RF s = RF(0);
+        for (int mm = 0; mm < nm; mm++) s = fmar(gp[mm], RF(2) * (RF) m.T[mm * nd + d], s);
+        S[((size_t) b * k + j) * nd + d] = s * (RF) Pp[d];
+      }
+    }
+  out.F.assign(3 * n, 0.0);
+  std::vector<RF> dMdu(3 * nd), Mc(nd), V(nd);
+  for (size_t c : order) {
+    const RF *Sb = &S[(size_t) slot[c] * k * nd];
+    RF uc[3] = {(RF) u[3 * c], (RF) u[3 * c + 1], (RF) u[3 * c + 2]};
+    monomials_emul<RF>(m, uc, Mc.data(), dMdu.data());
+    RF f1 = RF(0);
+    std::fill(V.begin(), V.end(), RF(0));
+    for (int j = 0; j < k; j++) {
+      RF Uk = RF(0);
+      for (int d = 0; d < nd; d++) Uk = fmar(Sb[j * nd + d], Mc[d], Uk);
+      f1 = fmar(Uk, (RF) dH[c * k + j], f1);
+      const RF h = (RF) H[c * k + j];
+      for (int d = 0; d < nd; d++) V[d] = fmar(Sb[j * nd + d], h, V[d]);
+    }
+    const RF rinv = (RF) (1.0 / r[c]);
+    for (int a = 0; a < 3; a++) {
+      RF f2 = RF(0);
+      for (int d = 0; d < nd; d++) {
+        RF dm = RF(0);
+        for (int b = 0; b < 3; b++) {
+          RF J = ((a == b ? RF(1) : RF(0)) - uc[a] * uc[b]) * rinv;
+          dm = fmar(dMdu[b * nd + d], J, dm);
+        }
+        f2 = fmar(V[d], dm, f2);
+      }
+      out.F[3 * c + a] = (double) fmar(f1, uc[a], f2) * scale;
+    }
+  }
+}
+
+void atom_chain(const Model &m, int ei, const std::vector<Nbr> &nb, double scale, AtomOut &out,
+                bool keep_desc)
+{
+  if (m.emul == 0) {
+    atom_chain_fp64(m, ei, nb, scale, out, keep_desc);
+    return;
+  }
+  switch (m.emul & 7) {
+    case 0: atom_chain_emul<double, double, double>(m, ei, nb, scale, out, keep_desc); break;
+    case 1: atom_chain_emul<float, double, double>(m, ei, nb, scale, out, keep_desc); break;
+    case 2: atom_chain_emul<double, float, double>(m, ei, nb, scale, out, keep_desc); break;
+    case 3: atom_chain_emul<float, float, double>(m, ei, nb, scale, out, keep_desc); break;
+    case 4: atom_chain_emul<double, double, float>(m, ei, nb, scale, out, keep_desc); break;
+    case 5: atom_chain_emul<float, double, float>(m, ei, nb, scale, out, keep_desc); break;
+    case 6: atom_chain_emul<double, float, float>(m, ei, nb, scale, out, keep_desc); break;
+    default: atom_chain_emul<float, float, float>(m, ei, nb, scale, out, keep_desc); break;
+  }
+}
+
 thread_local std::string g_err;
 
 }  // namespace
@@ -485,6 +748,14 @@ const char *tmdo_last_error() { return g_err.c_str(); }
 void *tmdo_new() { return new Model(); }
 
 void tmdo_free(void *h) { delete static_cast<Model *>(h); }
+
+// Selects the reduced-precision emulation (flags as Model::emul; 0 = the
+// FP64 chain).
+int tmdo_set_emulation(void *h, int flags)
+{
+  static_cast<Model *>(h)->emul = flags;
+  return 0;
+}
 
 int tmdo_set_ints(void *h, const char *key, const int *v, int n)
 {

@@ -19,6 +19,9 @@
  *                       TensorMD::calc_tensor_density/run (tensormd.cpp:368-433,
  *                       :450-615) and Pair::virial_fdotr_compute
  *                       (src/pair.cpp:1735-1783)
+ *   tmd_compute_list_serial  the same, with the list kept on the device
+ *                       between LAMMPS neighbor builds (the reference uploads
+ *                       it every step, gpu/cuda/kernel.cu:1500-1521)
  *   tmd_descriptors_list ComputeTensorMDDensity::compute_peratom
  *                       (src/ML-TENSORMD/compute_tensormd_density.cpp:129-205)
  *   tmd_system_*,       the LAMMPS step loop around the pair style:
@@ -107,6 +110,21 @@ int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const 
                      int inum, const int *ilist, const int *numneigh,
                      const int *const *firstneigh, double scale, double *f, double *eatom,
                      double *vatom, double *energy, double *virial);
+
+/* tmd_compute_list for a caller that knows when its neighbor list was
+ * rebuilt (LAMMPS: neighbor->lastcall, the step of the last build; the
+ * reference re-packs and re-uploads the whole list every step,
+ * gpu/cuda/kernel.cu:1500-1521).  When list_serial (>= 0) equals that of
+ * the context's previous tmd_compute_list_serial call, with the same nlocal,
+ * nall and inum and no other list-mode call in between, the list (ilist,
+ * numneigh, firstneigh and the types of the atoms in ilist) is taken as
+ * unchanged and stays on the device: only x is packed and uploaded.  Any
+ * other serial packs and uploads everything.  Outputs as tmd_compute_list. */
+int tmd_compute_list_serial(tmd_ctx *ctx, long long list_serial, int nlocal, int nall,
+                            const double *x, const int *type, int inum, const int *ilist,
+                            const int *numneigh, const int *const *firstneigh, double scale,
+                            double *f, double *eatom, double *vatom, double *energy,
+                            double *virial);
 
 /* `compute tensormd/density`: the descriptor row of every atom in ilist,
  * G[nlocal][width] in the model file's width and [b][k][m] layout (rows of

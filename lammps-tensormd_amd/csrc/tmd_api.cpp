@@ -47,17 +47,42 @@ DeviceBuffer::~DeviceBuffer() { release(); }
 
 void DeviceBuffer::release()
 {
-  if (p) hipFree(p);
+  if (p && owned) hipFree(p);
   p = nullptr;
   bytes = 0;
+  owned = true;
 }
 
 hipError_t DeviceBuffer::ensure(size_t nbytes)
 {
   if (nbytes <= bytes && p) return hipSuccess;
+  if (!owned) return hipErrorOutOfMemory;  // a fixed window never grows
   release();
   size_t want = std::max<size_t>(nbytes, 256);
   hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return e;
+  }
+  bytes = want;
+  return hipSuccess;
+}
+
+PinnedBuffer::~PinnedBuffer() { release(); }
+
+void PinnedBuffer::release()
+{
+  if (p) hipHostFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+hipError_t PinnedBuffer::ensure(size_t nbytes)
+{
+  if (nbytes <= bytes && p) return hipSuccess;
+  release();
+  size_t want = std::max<size_t>(nbytes + nbytes / 4, 4096);
+  hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
   if (e != hipSuccess) {
     p = nullptr;
     return e;
@@ -452,6 +477,7 @@ void tmd_destroy(tmd_ctx *ctx)
   }
   hipStreamSynchronize(ctx->stream);
   ctx->list_ws = Workspace();
+  ctx->list_host.release();
   ctx->tab.release();
   ctx->params.release();
   hipStreamDestroy(ctx->stream);
@@ -492,167 +518,55 @@ int tmd_model_info(const tmd_ctx *ctx, int *info)
   return TMD_OK;
 }
 
-// Pair::compute over a LAMMPS full list; with Gout set, only the descriptor
-// rows of the listed atoms ([nlocal][gwidth_file], other rows zero)
-static int list_compute(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type, int inum,
-                        const int *ilist, const int *numneigh, const int *const *firstneigh, double scale,
-                        double *f, double *eatom, double *vatom, double *energy, double *virial,
-                        double *Gout)
-{
-  if (!ctx) {
-    set_err(nullptr, "tmd_compute_list: ctx is NULL");
-    return TMD_ERR_ARG;
-  }
-  if (nall < nlocal || inum < 0 || inum > nlocal || !x || !type || (!Gout && (!f || !energy)) ||
-      (inum > 0 && (!ilist || !numneigh || !firstneigh))) {
-    set_err(ctx, "tmd_compute_list: invalid arguments");
-    return TMD_ERR_ARG;
-  }
-  if (nall > kMaxAtoms) {
-    set_err(ctx, "tmd_compute_list: nall exceeds the 2^23-1 atom limit of the neighbor encoding");
-    return TMD_ERR_CAPACITY;
-  }
-  if (hipError_t de = hipSetDevice(ctx->device); de != hipSuccess)
-    return hip_fail(ctx, de, "hipSetDevice");
-  Workspace &w = ctx->list_ws;
-  hipStream_t s = ctx->stream;
-  // element per position (type map), packed positions
-  std::vector<double4> pos(std::max(nall, 1));
-  std::vector<int> elem(std::max(nall, 1), 0);
-  for (int i = 0; i < nall; i++) {
-    int t = type[i];
-    int e = (t >= 1 && t <= ctx->ntypes) ? ctx->type_map[t] : -1;
-    if (e < 0) {
-      set_err(ctx, "tmd_compute_list: atom of a type not mapped to a model element");
-      return TMD_ERR_ARG;
-    }
-    elem[i] = e;
-    pos[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], (double) e);
-  }
-  // padded neighbor list indexed by local atom
-  int cap = 8;
-  for (int ii = 0; ii < inum; ii++) {
-    int i = ilist[ii];
-    if (i < 0 || i >= nlocal) {
-      set_err(ctx, "tmd_compute_list: ilist entry out of range");
-      return TMD_ERR_ARG;
-    }
-    cap = std::max(cap, numneigh[i]);
-  }
-  cap = (cap + 7) & ~7;
-  std::vector<uint32_t> list((size_t) std::max(nlocal, 1) * cap, 0u);
-  std::vector<int> nn(std::max(nlocal, 1), 0);
-  std::vector<int> atoms;
-  atoms.reserve(inum);
-  for (int ii = 0; ii < inum; ii++) {
-    int i = ilist[ii];
-    atoms.push_back(i);
-    nn[i] = numneigh[i];
-    for (int p = 0; p < numneigh[i]; p++) {
-      int j = firstneigh[i][p] & 0x1FFFFFFF;  // NEIGHMASK
-      if (j < 0 || j >= nall) {
-        set_err(ctx, "tmd_compute_list: neighbor index out of range");
-        return TMD_ERR_ARG;
-      }
-      list[(size_t) i * cap + p] = (uint32_t) j | (kShiftCenter << kNbrJBits);
-    }
-  }
-  std::vector<int> row2atom;
-  make_rows(ctx->model.nelt, atoms, elem, w.rl, row2atom);
-  const int gwf = ctx->model.gwidth_file();
-  if (Gout) std::memset(Gout, 0, sizeof(double) * (size_t) nlocal * gwf);
-  if (w.rl.nrows == 0) {
-    if (Gout) return TMD_OK;
-    std::memset(f, 0, sizeof(double) * 3 * (size_t) nall);
-    if (vatom) std::memset(vatom, 0, sizeof(double) * 6 * (size_t) nall);
-    if (eatom) std::memset(eatom, 0, sizeof(double) * (size_t) nlocal);
-    *energy = 0.0;
-    if (virial) std::memset(virial, 0, 6 * sizeof(double));
-    return TMD_OK;
-  }
-  int rc = ensure_rows(ctx, w, w.rl.nrows);
-  if (rc != TMD_OK) return rc;
-  if ((rc = ensure_kept(ctx, w, w.rl.nrows, cap)) != TMD_OK) return rc;
-  hipError_t e;
-  if ((e = w.pos.ensure(sizeof(double4) * pos.size())) != hipSuccess) return hip_fail(ctx, e, "pos alloc");
-  if ((e = w.nbr.ensure(4 * list.size())) != hipSuccess) return hip_fail(ctx, e, "nbr alloc");
-  if ((e = w.nn.ensure(4 * nn.size())) != hipSuccess) return hip_fail(ctx, e, "nn alloc");
-  if ((e = w.f.ensure(sizeof(double) * 3 * (size_t) nall)) != hipSuccess) return hip_fail(ctx, e, "f alloc");
-  if (vatom && (e = w.vatom.ensure(sizeof(double) * 6 * (size_t) nall)) != hipSuccess)
-    return hip_fail(ctx, e, "vatom alloc");
-  if ((e = hipMemcpyAsync(w.pos.p, pos.data(), sizeof(double4) * pos.size(), hipMemcpyHostToDevice, s)) !=
-          hipSuccess ||
-      (e = hipMemcpyAsync(w.nbr.p, list.data(), 4 * list.size(), hipMemcpyHostToDevice, s)) != hipSuccess ||
-      (e = hipMemcpyAsync(w.nn.p, nn.data(), 4 * nn.size(), hipMemcpyHostToDevice, s)) != hipSuccess ||
-      (e = hipMemcpyAsync(w.row2atom.p, row2atom.data(), 4 * row2atom.size(), hipMemcpyHostToDevice, s)) !=
-          hipSuccess)
-    return hip_fail(ctx, e, "list upload");
-  // list mode: positions are already images (ghosts), the one cell is zero
-  CellInfo zero_cell;
-  std::memset(&zero_cell, 0, sizeof(zero_cell));
-  if ((e = w.cells.ensure(sizeof(CellInfo))) != hipSuccess) return hip_fail(ctx, e, "cell alloc");
-  if ((e = hipMemcpyAsync(w.cells.p, &zero_cell, sizeof(CellInfo), hipMemcpyHostToDevice, s)) != hipSuccess)
-    return hip_fail(ctx, e, "cell upload");
-  if ((rc = ensure_atoms(ctx, w, nlocal)) != TMD_OK) return rc;
-  if (Gout) {
-    Geom geo;
-    std::memset(&geo, 0, sizeof(geo));
-    geo.cells = (const CellInfo *) w.cells.p;
-    e = launch_descriptor(ctx->geo, geo, (const double4 *) w.pos.p, (const uint32_t *) w.nbr.p,
-                          (const int *) w.nn.p, cap, (const int *) w.row2atom.p, w.rl.nrows,
-                          (const float4 *) ctx->tab.p, (float *) w.G.p, (uint32_t *) w.kl.p, s);
-    if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
-    const int gw = ctx->model.gwidth(), nm = ctx->model.nm, nmf = ctx->model.max_moment_file + 1;
-    std::vector<float> g((size_t) w.rl.nrows * gw);
-    if ((e = hipMemcpyAsync(g.data(), w.G.p, 4 * g.size(), hipMemcpyDeviceToHost, s)) != hipSuccess ||
-        (e = hipStreamSynchronize(s)) != hipSuccess)
-      return hip_fail(ctx, e, "descriptor download");
-    // the model file's layout [b][k][max_moment_file + 1]
-    for (int r = 0; r < w.rl.nrows; r++) {
-      const int a = row2atom[r];
-      if (a < 0) continue;
-      for (int q = 0; q < gwf; q++)
-        Gout[(size_t) a * gwf + q] = g[(size_t) r * gw + (size_t) (q / nmf) * nm + q % nmf];
-    }
-    return TMD_OK;
-  }
-  Chain c;
-  c.geo.cells = (const CellInfo *) w.cells.p;
-  c.pos = (const double4 *) w.pos.p;
-  c.nbr = (const uint32_t *) w.nbr.p;
-  c.nn = (const int *) w.nn.p;
-  c.cap = cap;
-  c.scale = scale;
-  c.f = (double *) w.f.p;
-  c.fbytes = sizeof(double) * 3 * (size_t) nall;
-  c.vatom = vatom ? (double *) w.vatom.p : nullptr;
-  c.vbytes = sizeof(double) * 6 * (size_t) nall;
-  c.natoms = nlocal;
-  c.zero_ew = inum < nlocal;
-  rc = run_chain(ctx, w, c, s);
-  if (rc != TMD_OK) return rc;
-  double red[7];
-  if ((e = hipMemcpyAsync(f, w.f.p, sizeof(double) * 3 * (size_t) nall, hipMemcpyDeviceToHost, s)) != hipSuccess ||
-      (vatom && (e = hipMemcpyAsync(vatom, w.vatom.p, sizeof(double) * 6 * (size_t) nall,
-                                    hipMemcpyDeviceToHost, s)) != hipSuccess) ||
-      (e = hipMemcpyAsync(red, w.red.p, sizeof(red), hipMemcpyDeviceToHost, s)) != hipSuccess ||
-      (eatom && (e = hipMemcpyAsync(eatom, w.ew.p, sizeof(double) * (size_t) nlocal, hipMemcpyDeviceToHost,
-                                    s)) != hipSuccess))
-    return hip_fail(ctx, e, "result download");
-  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(ctx, e, "tmd_compute_list");
-  *energy = red[0];
-  if (virial)
-    for (int q = 0; q < 6; q++) virial[q] = red[1 + q];
-  return TMD_OK;
-}
-
 int tmd_compute_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type,
                      int inum, const int *ilist, const int *numneigh,
                      const int *const *firstneigh, double scale, double *f, double *eatom,
                      double *vatom, double *energy, double *virial)
 {
-  return list_compute(ctx, nlocal, nall, x, type, inum, ilist, numneigh, firstneigh, scale, f, eatom, vatom,
-                      energy, virial, nullptr);
+  ListArgs a;
+  a.nlocal = nlocal;
+  a.nall = nall;
+  a.x = x;
+  a.type = type;
+  a.inum = inum;
+  a.ilist = ilist;
+  a.numneigh = numneigh;
+  a.firstneigh = firstneigh;
+  a.scale = scale;
+  a.f = f;
+  a.eatom = eatom;
+  a.vatom = vatom;
+  a.energy = energy;
+  a.virial = virial;
+  return list_compute(ctx, a, ListResources(), -1);
+}
+
+int tmd_compute_list_serial(tmd_ctx *ctx, long long list_serial, int nlocal, int nall,
+                            const double *x, const int *type, int inum, const int *ilist,
+                            const int *numneigh, const int *const *firstneigh, double scale,
+                            double *f, double *eatom, double *vatom, double *energy, double *virial)
+{
+  if (list_serial < 0) {
+    set_err(ctx, "tmd_compute_list_serial: list_serial must be >= 0");
+    return TMD_ERR_ARG;
+  }
+  ListArgs a;
+  a.nlocal = nlocal;
+  a.nall = nall;
+  a.x = x;
+  a.type = type;
+  a.inum = inum;
+  a.ilist = ilist;
+  a.numneigh = numneigh;
+  a.firstneigh = firstneigh;
+  a.scale = scale;
+  a.f = f;
+  a.eatom = eatom;
+  a.vatom = vatom;
+  a.energy = energy;
+  a.virial = virial;
+  a.who = "tmd_compute_list_serial";
+  return list_compute(ctx, a, ListResources(), list_serial);
 }
 
 int tmd_descriptors_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, const int *type, int inum,
@@ -662,8 +576,18 @@ int tmd_descriptors_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, co
     set_err(ctx, "tmd_descriptors_list: G is NULL");
     return TMD_ERR_ARG;
   }
-  return list_compute(ctx, nlocal, nall, x, type, inum, ilist, numneigh, firstneigh, 1.0, nullptr, nullptr,
-                      nullptr, nullptr, nullptr, G);
+  ListArgs a;
+  a.nlocal = nlocal;
+  a.nall = nall;
+  a.x = x;
+  a.type = type;
+  a.inum = inum;
+  a.ilist = ilist;
+  a.numneigh = numneigh;
+  a.firstneigh = firstneigh;
+  a.Gout = G;
+  a.who = "tmd_descriptors_list";
+  return list_compute(ctx, a, ListResources(), -1);
 }
 
 int tmd_device_memory(size_t *free_bytes, size_t *total_bytes)

@@ -11,17 +11,21 @@
 
 namespace tmd {
 
-// Owning device allocation that only grows.
+// Device allocation that only grows, or (view()) a fixed window of memory
+// the buffer does not own: ensure() on a view never allocates and fails with
+// hipErrorOutOfMemory when the window is too small.
 struct DeviceBuffer {
   void *p = nullptr;
   size_t bytes = 0;
+  bool owned = true;
   DeviceBuffer() = default;
   DeviceBuffer(const DeviceBuffer &) = delete;
   DeviceBuffer &operator=(const DeviceBuffer &) = delete;
-  DeviceBuffer(DeviceBuffer &&o) noexcept : p(o.p), bytes(o.bytes)
+  DeviceBuffer(DeviceBuffer &&o) noexcept : p(o.p), bytes(o.bytes), owned(o.owned)
   {
     o.p = nullptr;
     o.bytes = 0;
+    o.owned = true;
   }
   DeviceBuffer &operator=(DeviceBuffer &&o) noexcept
   {
@@ -29,12 +33,33 @@ struct DeviceBuffer {
       release();
       p = o.p;
       bytes = o.bytes;
+      owned = o.owned;
       o.p = nullptr;
       o.bytes = 0;
+      o.owned = true;
     }
     return *this;
   }
   ~DeviceBuffer();
+  void release();
+  hipError_t ensure(size_t nbytes);
+  void set_view(void *base, size_t nbytes)
+  {
+    release();
+    p = base;
+    bytes = nbytes;
+    owned = false;
+  }
+};
+
+// Page-locked host allocation (hipHostMalloc) that only grows.
+struct PinnedBuffer {
+  void *p = nullptr;
+  size_t bytes = 0;
+  PinnedBuffer() = default;
+  PinnedBuffer(const PinnedBuffer &) = delete;
+  PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+  ~PinnedBuffer();
   void release();
   hipError_t ensure(size_t nbytes);
 };
@@ -66,6 +91,12 @@ struct tmd_ctx {
   tmd::DeviceBuffer tab, params;
   tmd::MlpParams mlp{};
   tmd::Workspace list_ws;
+  // list mode: pinned host staging, and what list_ws holds (the serial of
+  // the last tmd_compute_list_serial call and the list's shape)
+  tmd::PinnedBuffer list_host;
+  bool list_cached = false;
+  long long list_serial = 0;
+  int list_nlocal = 0, list_nall = 0, list_inum = 0, list_cap = 0;
   tmd::Server *server = nullptr;  // request loop (tmd_serve_start)
 };
 
@@ -111,6 +142,36 @@ struct Chain {
 };
 int ensure_atoms(tmd_ctx *ctx, Workspace &w, int natoms);
 int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEvents *pe = nullptr);
+// ---- list mode (Pair::compute over a LAMMPS full list) ----
+struct ListArgs {
+  int nlocal = 0, nall = 0;
+  const double *x = nullptr;
+  const int *type = nullptr;
+  int inum = 0;
+  const int *ilist = nullptr;
+  const int *numneigh = nullptr;
+  const int *const *firstneigh = nullptr;
+  double scale = 1.0;
+  double *f = nullptr, *eatom = nullptr, *vatom = nullptr, *energy = nullptr, *virial = nullptr;
+  double *Gout = nullptr;  // set: descriptor rows only ([nlocal][file width])
+  const char *who = "tmd_compute_list";
+};
+// Where one list-mode call's buffers live.  fixed = false: the context's
+// growable buffers (list_ws, list_host).  fixed = true: device buffers are
+// carved from [dev_base, dev_base + dev_bytes) and host staging from
+// [host, host + host_bytes); nothing is allocated, and a call that needs
+// more is refused with TMD_ERR_CAPACITY.
+struct ListResources {
+  bool fixed = false;
+  void *dev_base = nullptr;
+  size_t dev_bytes = 0;
+  unsigned char *host = nullptr;
+  size_t host_bytes = 0;
+};
+// serial >= 0 (tmd_compute_list_serial): the list is reused from the
+// previous call with the same serial and shape; serial < 0: always packed.
+int list_compute(tmd_ctx *ctx, const ListArgs &a, const ListResources &r, long long serial);
+
 bool invert3(const double *h, double *inv);
 double cell_volume(const double *h);
 int setup_cell(const double *h, double cut, CellInfo &c, std::string &err);

@@ -57,7 +57,10 @@ void PairTensorMDAMD::fail(int code, const char *where)
 
 /* pair_tensormd.cpp:141-184: f is overwritten for all local + ghost atoms,
    eatom/vatom are written, eng_vdwl = sum of scaled atomic energies, and the
-   global virial is the fdotr sum */
+   global virial is the fdotr sum.  The list is identified by the step of
+   the last neighbor build (neighbor->lastcall): between builds only the
+   positions go to the device (the reference repacks and uploads the whole
+   list every step, gpu/cuda/kernel.cu:1500-1521). */
 void PairTensorMDAMD::compute(int eflag, int vflag)
 {
   ev_init(eflag, vflag);
@@ -65,10 +68,18 @@ void PairTensorMDAMD::compute(int eflag, int vflag)
   const int nall = nlocal + atom->nghost;
   double energy = 0.0;
   const double scale_factor = scale[1][1];
-  int rc = tmd_compute_list(ctx, nlocal, nall, &atom->x[0][0], atom->type, list->inum, list->ilist,
-                            list->numneigh, list->firstneigh, scale_factor, &atom->f[0][0],
-                            eflag_atom ? eatom : nullptr, vflag_atom ? &vatom[0][0] : nullptr,
-                            &energy, nullptr);
+  double *eat = eflag_atom ? eatom : nullptr;
+  double *vat = vflag_atom ? &vatom[0][0] : nullptr;
+  int rc;
+  if (neighbor->lastcall >= 0)
+    rc = tmd_compute_list_serial(ctx, (long long) neighbor->lastcall, nlocal, nall, &atom->x[0][0],
+                                 atom->type, list->inum, list->ilist, list->numneigh,
+                                 list->firstneigh, scale_factor, &atom->f[0][0], eat, vat, &energy,
+                                 nullptr);
+  else
+    rc = tmd_compute_list(ctx, nlocal, nall, &atom->x[0][0], atom->type, list->inum, list->ilist,
+                          list->numneigh, list->firstneigh, scale_factor, &atom->f[0][0], eat, vat,
+                          &energy, nullptr);
   if (rc != TMD_OK) fail(rc, "pair tensormd/amd compute");
   if (eflag) eng_vdwl = energy;
   if (vflag_fdotr) virial_fdotr_compute();

@@ -88,6 +88,7 @@ def lib():
     L.tmd_compute_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int, _ip,
                                    _ip, ctypes.POINTER(_ip), ctypes.c_double, _dp, _dp, _dp, _dp,
                                    _dp]
+    L.tmd_compute_list_serial.argtypes = [_vp, ctypes.c_longlong] + L.tmd_compute_list.argtypes[1:]
     L.tmd_descriptors_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int, _ip,
                                        _ip, ctypes.POINTER(_ip), _dp]
     L.tmd_system_create.argtypes = [_vp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_double,
@@ -152,6 +153,25 @@ def device_memory():
     return f.value, t.value
 
 
+class NeighborList:
+    """A LAMMPS-style full neighbor list held for the C ABI: ilist,
+    numneigh and the firstneigh pointer array into one CSR buffer (entries
+    stored in ilist order)."""
+
+    def __init__(self, nlocal, ilist, numneigh, neigh_flat):
+        self.nlocal = int(nlocal)
+        self.ilist = _i32(ilist)
+        self.numneigh = _i32(numneigh)
+        self.flat = _i32(neigh_flat)
+        self.ptrs = (_ip * max(self.nlocal, 1))()
+        counts = self.numneigh[self.ilist].astype(np.int64)
+        starts = np.concatenate([[0], np.cumsum(counts)[:-1]]) if counts.size else counts
+        base = self.flat.ctypes.data
+        addr = (ctypes.c_void_p * max(self.nlocal, 1)).from_buffer(self.ptrs)
+        for i, st in zip(self.ilist.tolist(), starts.tolist()):
+            addr[i] = base + 4 * st
+
+
 class TensorMD:
     """One model on one device (PairTensorMD::coeff + read_potential_file)."""
 
@@ -207,34 +227,39 @@ class TensorMD:
                     width=int(v[3]), nlayers=int(v[4]))
 
     def compute_list(self, nlocal, x, types, ilist, numneigh, neigh_flat, scale=1.0,
-                     eatom=True, vatom=True):
-        """Pair::compute over a LAMMPS full neighbor list (CSR in ilist order)."""
+                     eatom=True, vatom=True, serial=None):
+        """Pair::compute over a LAMMPS full neighbor list (CSR in ilist order).
+
+        serial: call tmd_compute_list_serial with this list serial (the list
+        stays on the device while the serial repeats)."""
+        lst = NeighborList(nlocal, ilist, numneigh, neigh_flat)
+        return self.compute_list_ptrs(lst, x, types, scale=scale, eatom=eatom, vatom=vatom,
+                                      serial=serial)
+
+    def compute_list_ptrs(self, lst, x, types, scale=1.0, eatom=True, vatom=True, serial=None,
+                          out=None):
+        """compute_list over a prepared NeighborList; `out` (a dict from an
+        earlier call with the same shapes) is overwritten in place."""
         x = _f64(x).reshape(-1, 3)
         nall = x.shape[0]
         types = _i32(types)
-        ilist = _i32(ilist)
-        numneigh = _i32(numneigh)
-        neigh_flat = _i32(neigh_flat)
-        inum = ilist.size
-        # firstneigh[i] points into neigh_flat (entries stored in ilist order)
-        ptrs = (_ip * max(nlocal, 1))()
-        base = neigh_flat.ctypes.data
-        off = 0
-        for ii in range(inum):
-            i = int(ilist[ii])
-            ptrs[i] = ctypes.cast(base + 4 * off, _ip)
-            off += int(numneigh[i])
-        f = np.zeros((nall, 3))
-        ea = np.zeros(nlocal) if eatom else None
-        va = np.zeros((nall, 6)) if vatom else None
+        nlocal = lst.nlocal
+        if out is None:
+            out = dict(f=np.zeros((nall, 3)), eatom=np.zeros(nlocal) if eatom else None,
+                       vatom=np.zeros((nall, 6)) if vatom else None, energy=0.0,
+                       virial=np.zeros(6))
         e = ctypes.c_double()
-        vir = np.zeros(6)
-        rc = lib().tmd_compute_list(self.h, int(nlocal), int(nall), _p(x, _dp), _p(types, _ip),
-                                    int(inum), _p(ilist, _ip), _p(numneigh, _ip), ptrs,
-                                    float(scale), _p(f, _dp), _p(ea, _dp), _p(va, _dp),
-                                    ctypes.byref(e), _p(vir, _dp))
+        args = (int(nlocal), int(nall), _p(x, _dp), _p(types, _ip), int(lst.ilist.size),
+                _p(lst.ilist, _ip), _p(lst.numneigh, _ip), lst.ptrs, float(scale),
+                _p(out["f"], _dp), _p(out["eatom"], _dp), _p(out["vatom"], _dp),
+                ctypes.byref(e), _p(out["virial"], _dp))
+        if serial is not None:
+            rc = lib().tmd_compute_list_serial(self.h, int(serial), *args)
+        else:
+            rc = lib().tmd_compute_list(self.h, *args)
         _check(rc, self.h)
-        return dict(f=f, eatom=ea, vatom=va, energy=e.value, virial=vir)
+        out["energy"] = e.value
+        return out
 
     def descriptors_list(self, nlocal, x, types, ilist, numneigh, neigh_flat):
         """compute tensormd/density: descriptor rows [nlocal, width] (zero for

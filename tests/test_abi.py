@@ -144,3 +144,16 @@ def test_wide_slot_models_load(tmp_path, k, nelt):
                       rmax=5.0, num_filters=k, layer_sizes=[32, 1], seed=4)
     rc, msg = _create_rc(p, sym)
     assert rc == -3, msg
+
+
+def test_list_entry_points_reject_null_context():
+    """Argument checks of the list-mode variants come before any device work."""
+    L = T.lib()
+    e = ctypes.c_double()
+    f = np.zeros(3)
+    x = np.zeros(3)
+    t = np.ones(1, dtype=np.int32)
+    args = (1, 1, x.ctypes.data_as(T._dp), t.ctypes.data_as(T._ip), 0, None, None, None, 1.0,
+            f.ctypes.data_as(T._dp), None, None, ctypes.byref(e), None)
+    assert L.tmd_compute_list(None, *args) == -1
+    assert L.tmd_compute_list_serial(None, 0, *args) == -1

@@ -126,6 +126,20 @@ int tmd_compute_list_serial(tmd_ctx *ctx, long long list_serial, int nlocal, int
                             double *f, double *eatom, double *vatom, double *energy,
                             double *virial);
 
+/* tmd_compute_list with no allocation in the call.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as tmd_compute_list; its device buffers are
+ * windows of a fixed 64 MiB device array (one per device, held until the
+ * process ends) and its host staging a fixed 32 MiB static array.  A call
+ * that needs more (about 10,000 local atoms of a 4-element k = 24 model)
+ * returns TMD_ERR_CAPACITY.  Calls are serialized by a process-wide lock;
+ * the arrays keep the last call's contents. */
+int tmd_compute_list_noalloc(tmd_ctx *ctx, int nlocal, int nall, const double *x,
+                             const int *type, int inum, const int *ilist, const int *numneigh,
+                             const int *const *firstneigh, double scale, double *f,
+                             double *eatom, double *vatom, double *energy, double *virial);
+
 /* `compute tensormd/density`: the descriptor row of every atom in ilist,
  * G[nlocal][width] in the model file's width and [b][k][m] layout (rows of
  * atoms not in ilist are zero), from the same inputs as tmd_compute_list.

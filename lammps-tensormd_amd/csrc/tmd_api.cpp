@@ -461,6 +461,13 @@ int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes
   ctx->device = dev;
   if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
     return hip_fail(nullptr, e, "hipStreamCreate");
+  {
+    std::string serr;
+    if (noalloc_scratch_setup(dev, serr) != TMD_OK) {
+      set_err(nullptr, serr);
+      return TMD_ERR_DEVICE;
+    }
+  }
   int rc = upload_model(ctx.get());
   if (rc != TMD_OK) return rc;
   *out = ctx.release();

@@ -171,6 +171,9 @@ struct ListResources {
 // serial >= 0 (tmd_compute_list_serial): the list is reused from the
 // previous call with the same serial and shape; serial < 0: always packed.
 int list_compute(tmd_ctx *ctx, const ListArgs &a, const ListResources &r, long long serial);
+// The fixed device scratch of the no-allocation entry point on `device`
+// (tmd_noalloc.hip): looked up once per device, at context creation.
+int noalloc_scratch_setup(int device, std::string &err);
 
 bool invert3(const double *h, double *inv);
 double cell_volume(const double *h);

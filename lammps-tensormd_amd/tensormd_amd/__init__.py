@@ -89,6 +89,7 @@ def lib():
                                    _ip, ctypes.POINTER(_ip), ctypes.c_double, _dp, _dp, _dp, _dp,
                                    _dp]
     L.tmd_compute_list_serial.argtypes = [_vp, ctypes.c_longlong] + L.tmd_compute_list.argtypes[1:]
+    L.tmd_compute_list_noalloc.argtypes = L.tmd_compute_list.argtypes
     L.tmd_descriptors_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int, _ip,
                                        _ip, ctypes.POINTER(_ip), _dp]
     L.tmd_system_create.argtypes = [_vp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_double,
@@ -227,17 +228,18 @@ class TensorMD:
                     width=int(v[3]), nlayers=int(v[4]))
 
     def compute_list(self, nlocal, x, types, ilist, numneigh, neigh_flat, scale=1.0,
-                     eatom=True, vatom=True, serial=None):
+                     eatom=True, vatom=True, serial=None, noalloc=False):
         """Pair::compute over a LAMMPS full neighbor list (CSR in ilist order).
 
         serial: call tmd_compute_list_serial with this list serial (the list
-        stays on the device while the serial repeats)."""
+        stays on the device while the serial repeats); noalloc: call
+        tmd_compute_list_noalloc (fixed device scratch, no allocation)."""
         lst = NeighborList(nlocal, ilist, numneigh, neigh_flat)
         return self.compute_list_ptrs(lst, x, types, scale=scale, eatom=eatom, vatom=vatom,
-                                      serial=serial)
+                                      serial=serial, noalloc=noalloc)
 
     def compute_list_ptrs(self, lst, x, types, scale=1.0, eatom=True, vatom=True, serial=None,
-                          out=None):
+                          noalloc=False, out=None):
         """compute_list over a prepared NeighborList; `out` (a dict from an
         earlier call with the same shapes) is overwritten in place."""
         x = _f64(x).reshape(-1, 3)
@@ -253,7 +255,9 @@ class TensorMD:
                 _p(lst.ilist, _ip), _p(lst.numneigh, _ip), lst.ptrs, float(scale),
                 _p(out["f"], _dp), _p(out["eatom"], _dp), _p(out["vatom"], _dp),
                 ctypes.byref(e), _p(out["virial"], _dp))
-        if serial is not None:
+        if noalloc:
+            rc = lib().tmd_compute_list_noalloc(self.h, *args)
+        elif serial is not None:
             rc = lib().tmd_compute_list_serial(self.h, int(serial), *args)
         else:
             rc = lib().tmd_compute_list(self.h, *args)

@@ -157,3 +157,4 @@ def test_list_entry_points_reject_null_context():
             f.ctypes.data_as(T._dp), None, None, ctypes.byref(e), None)
     assert L.tmd_compute_list(None, *args) == -1
     assert L.tmd_compute_list_serial(None, 0, *args) == -1
+    assert L.tmd_compute_list_noalloc(None, *args) == -1

@@ -154,6 +154,68 @@ def serve_bench(model, symbols, nreq, seed=11):
     return out
 
 
+def ghost_list_from_system(system, x, cell):
+    """LAMMPS-style full neighbor list with explicit ghosts, from the device
+    system's list (cutoff rc + skin): an entry with a nonzero image shift
+    becomes a ghost position x_j + shift cell, one ghost per distinct
+    (j, shift), numbered after the local atoms."""
+    off, j, sh = system.neighbors()
+    n = len(x)
+    code = ((sh[:, 0] + 3) * 49 + (sh[:, 1] + 3) * 7 + (sh[:, 2] + 3)).astype(np.int64)
+    key = j.astype(np.int64) * 343 + code
+    image = code != 3 * 49 + 3 * 7 + 3
+    gkey, inv = np.unique(key[image], return_inverse=True)
+    flat = j.astype(np.int64)
+    flat[image] = n + inv
+    gj, gc = gkey // 343, gkey % 343
+    gsh = np.stack([gc // 49 - 3, (gc // 7) % 7 - 3, gc % 7 - 3], 1).astype(np.float64)
+    xall = np.concatenate([x, x[gj] + gsh @ cell])
+    owner = np.concatenate([np.arange(n), gj])
+    return (xall, owner, np.arange(n, dtype=np.int32), np.diff(off).astype(np.int32),
+            flat.astype(np.int32))
+
+
+def list_mode_bench(model, symbols, x, t, cell, prec, calls):
+    """The Pair::compute drop-in (tmd_compute_list) on the same system with
+    LAMMPS-style ghosts: host buffers in and out every call (x and types up,
+    f down, PCIe included).  Timed twice: packing and uploading the whole
+    list every call (what the reference does every step), and with the list
+    kept on the device between neighbor builds (tmd_compute_list_serial with
+    an unchanged serial: only x goes up)."""
+    from tensormd_amd import NeighborList, TensorMD
+    m = TensorMD(model, symbols, prec, 0.001, device=0)
+    s = m.system(x, t, cell, skin=1.0)
+    s.compute()
+    xall, owner, ilist, numneigh, flat = ghost_list_from_system(s, s.get()["x"], cell)
+    s.close()
+    tall = t[owner]
+    n = len(x)
+    lst = NeighborList(n, ilist, numneigh, flat)
+    out = m.compute_list_ptrs(lst, xall, tall, eatom=False, vatom=False)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        m.compute_list_ptrs(lst, xall, tall, eatom=False, vatom=False, out=out)
+    dt_full = time.perf_counter() - t0
+    m.compute_list_ptrs(lst, xall, tall, eatom=False, vatom=False, serial=1, out=out)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        m.compute_list_ptrs(lst, xall, tall, eatom=False, vatom=False, serial=1, out=out)
+    dt_ser = time.perf_counter() - t0
+    m.close()
+    return dict(natoms=n, nall=int(len(xall)), list_entries=int(len(flat)), calls=calls,
+                repack_every_call=dict(atom_steps_per_s=n * calls / dt_full,
+                                       ms_per_call=1e3 * dt_full / calls),
+                list_kept_on_device=dict(atom_steps_per_s=n * calls / dt_ser,
+                                         ms_per_call=1e3 * dt_ser / calls))
+
+
+def _force_errors(got, ref, n):
+    df = got["f"] - ref["f"]
+    return dict(max_abs_dF=float(abs(df).max()), rms_dF=float(np.sqrt(np.mean(df * df))),
+                dE_per_atom=abs(got["energy"] - ref["energy"]) / n,
+                max_abs_F=float(abs(ref["f"]).max()))
+
+
 def fp8_experiment(steps, warmup):
     """1,024,000-atom 4-element BCC alloy, MLP 256-256-128-1: FP32, BF16 and
     FP8 e4m3 MLPs side by side, with the device memory each system holds."""
@@ -167,13 +229,18 @@ def fp8_experiment(steps, warmup):
     x, t, cell = W.crystal("bcc", 3.2, (80, 80, 80), nelt=4, sigma=0.08, seed=9)
     v = W.maxwell_boltzmann(np.array(masses)[t - 1], 600.0, seed=10)
     n = len(x)
-    out = dict(natoms=n, steps=steps, mlp="256-256-128-1 tanh")
+    out = dict(natoms=n, steps=steps, mlp="256-256-128-1 tanh",
+               errors_note="BF16 and FP8 forces and energies against the FP32 GPU path at the "
+                           "same positions: the starting state, and the FP32 run's state after "
+                           "warmup + steps (the FP32 path itself is within 1e-5 eV/A and 1e-7 "
+                           "eV/atom of the FP64 oracle on these models)")
+    ref = {}
     for prec in ("fp32", "bf16", "fp8"):
         m = TensorMD(model, symbols, prec, 0.001, device=0)
         free0, _ = device_memory()
         s = m.system(x, t, cell, skin=1.0)
         s.set_velocities(v)
-        s.compute()
+        r0 = s.compute()
         free1, _ = device_memory()
         s.md_run(warmup, 0.001, history=False)
         s.get()
@@ -185,6 +252,15 @@ def fp8_experiment(steps, warmup):
         out[prec] = dict(atom_steps_per_s=n * steps / dt, ms_per_step=1e3 * dt / steps,
                          mlp_ms=prof["mlp"], system_hbm_bytes=int(free0 - free1),
                          hbm_bytes_per_atom=(free0 - free1) / n)
+        if prec == "fp32":
+            x_end = s.get()["x"]
+            s.set_positions(x_end)
+            ref = dict(start=r0, end=s.compute(), x_end=x_end)
+        else:
+            s.set_positions(ref["x_end"])
+            r1 = s.compute()
+            out[prec]["errors_vs_fp32_start"] = _force_errors(r0, ref["start"], n)
+            out[prec]["errors_vs_fp32_end"] = _force_errors(r1, ref["end"], n)
         print(f"bench: c4 {prec}: {out[prec]}", flush=True)
         s.close()
         m.close()
@@ -203,6 +279,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--c4-steps", type=int, default=200)
+    # calls of the Pair::compute drop-in (host buffers, LAMMPS-style ghosts)
+    # on the same system, per precision and list mode
+    ap.add_argument("--list-calls", type=int, default=20)
     # every N-th timed step of the headline precision is launched kernel by
     # kernel with HIP events around the kernels (0: off; the roofline then
     # uses the kernel times of the profile steps after the run)
@@ -248,7 +327,7 @@ def main():
             nbr_within_rc = neighbors_within(s, s.get()["x"], cell, 6.5)
         s.md_run(args.warmup, 0.001, history=False)
         s.get()  # synchronise
-        if prec == head and kt_every > 0:
+        if kt_every > 0:
             # kernel durations over the timed region: every kt_every-th step
             # is launched kernel by kernel with HIP events on the library
             # stream (the others replay the captured step graph)
@@ -257,7 +336,7 @@ def main():
         s.md_run(args.steps, 0.001, history=False)
         out = s.get()  # synchronises the stream
         dt = time.perf_counter() - t0
-        if prec == head and kt_every > 0:
+        if kt_every > 0:
             ktime[prec] = s.kernel_timing()
             s.set_kernel_timing(0)
         res[prec] = dict(time=dt, rate=natoms * args.steps / dt, energy=out["energy"],
@@ -306,20 +385,33 @@ def main():
     c = kt["kept_per_atom"]
     per = per_atom_counts(c, d=20, k=24, b=4, m=3, gw=384, sizes=sizes)
     dom = max(["descriptor", "mlp", "force"], key=lambda k: kt[k])
-    t_dom = kt[dom] * 1e-3
-    if dom == "mlp":
-        roof = dict(kernel="k_mlp (fused per-element MLP fwd+bwd, BF16 MFMA)", bound="mfma",
-                    achieved=per["mlp_flop"] * natoms / t_dom / 1e12, peak=PEAK["bf16_tflops"],
-                    unit="TFLOP/s")
-    else:
-        # fused FP32 gather/scatter kernels: arithmetic intensity 50-80 FLOP/B,
-        # so the binding roof is FP32 arithmetic, not HBM
-        roof = dict(kernel=f"k_{dom} (fused FP32 {dom} kernel)", bound="mfma",
-                    achieved=per[dom + "_flop"] * natoms / t_dom / 1e12, peak=PEAK["fp32_tflops"],
-                    unit="TFLOP/s")
-        roof["hbm_achieved_gbs"] = per[dom + "_bytes"] * natoms / t_dom / 1e9
-        roof["hbm_peak_gbs"] = PEAK["hbm_gbs"]
-    roof["frac"] = roof["achieved"] / roof["peak"]
+
+    def kernel_row(kname, prec, ms, c_kept):
+        """Roofline row of one kernel: algorithmic FLOP/s against its
+        arithmetic roof and algorithmic bytes/s against HBM."""
+        pk = per_atom_counts(c_kept, d=20, k=24, b=4, m=3, gw=384, sizes=sizes)
+        t_s = ms * 1e-3
+        flops = pk[kname + "_flop"] * natoms / t_s / 1e12
+        gbs = pk[kname + "_bytes"] * natoms / t_s / 1e9
+        if kname == "mlp":
+            peak = PEAK["bf16_tflops"] if prec == "bf16" else PEAK["fp32_tflops"]
+            row = dict(kernel=f"k_mlp_fast {prec} (fused per-element MLP fwd+bwd on "
+                              f"{'BF16' if prec == 'bf16' else 'FP32'} MFMA)", bound="mfma")
+        else:
+            # fused FP32 gather/scatter kernels without MFMA: arithmetic
+            # intensity 50-80 FLOP/B, so the binding roof is FP32 vector
+            # issue (VALU), not HBM
+            peak = PEAK["fp32_tflops"]
+            row = dict(kernel=f"k_{kname} (fused FP32 {kname} kernel, VALU)", bound="valu",
+                       valu_peak_without_packed_fp32=PEAK["fp32_tflops"] / 2)
+        row.update(precision=prec, achieved=flops, peak=peak, unit="TFLOP/s", frac=flops / peak,
+                   hbm_achieved_gbs=gbs, hbm_peak_gbs=PEAK["hbm_gbs"], hbm_frac=gbs / PEAK["hbm_gbs"],
+                   avg_launch_ms=ms, neighbors_within_rc=c_kept)
+        return row
+
+    roof = dict(kernel_row(dom, head, kt[dom], c))
+    roof["kernels"] = [kernel_row(k, pr, ktime[pr][k], ktime[pr]["kept_per_atom"])
+                       for pr in ktime for k in ("descriptor", "mlp", "force")]
     roof["avg_launch_ms"] = kt[dom]
     roof["avg_launch_samples"] = kt["samples"]
     roof["timing"] = timing
@@ -341,6 +433,12 @@ def main():
     roof["neighbors_within_rc_start"] = nbr_within_rc
     roof["neighbors_within_rc_end"] = state.get("neighbors_within_rc")
     roof["temperature_K_end"] = state.get("temperature_K")
+
+    listmode = None
+    if args.list_calls > 0:
+        listmode = {prec: list_mode_bench(model, symbols, x, t, cell, prec, args.list_calls)
+                    for prec in ("fp32", "bf16")}
+        print(f"bench: list mode {json.dumps(listmode)}", flush=True)
 
     fp8 = None
     if args.c4_steps > 0:
@@ -423,6 +521,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "fp8_experiment": fp8,
+        "list_mode": listmode,
         "nve_drift": drift or None,
     }
     print(json.dumps(line), flush=True)

@@ -52,6 +52,32 @@ void fnn_forward(const DenseNet &net, double r, double *y)
   }
 }
 
+// The analytic radial functions f_k(r) (Descriptor::pexp, morse, density,
+// sf; src/ML-TENSORMD/descriptor.cpp:51-132).
+void analytic_forward(const Model &m, double r, double *y)
+{
+  const std::vector<std::vector<double>> &p = m.rparams;
+  for (int k = 0; k < m.k; k++) {
+    switch (m.radial_fn) {
+      case RADIAL_PEXP:  // exp(-(r / rl)^pl)
+        y[k] = std::exp(-std::pow(r / p[0][k], p[1][k]));
+        break;
+      case RADIAL_MORSE: {  // D [exp(-2 gamma (r - r0)) - 2 exp(-gamma (r - r0))]
+        const double g = p[1][k] * (r - p[2][k]);
+        y[k] = p[0][k] * (std::exp(-2.0 * g) - 2.0 * std::exp(-g));
+        break;
+      }
+      case RADIAL_DENSITY:  // A exp(-beta (r - re) / re)
+        y[k] = p[0][k] * std::exp(-p[1][k] * ((r - p[2][k]) / p[2][k]));
+        break;
+      default: {  // exp(-eta (r - omega)^2 / rc^2)
+        const double d = r - p[1][k];
+        y[k] = std::exp(-p[0][k] * d * d / (m.rc * m.rc));
+      }
+    }
+  }
+}
+
 double cutoff_fc(const Model &m, double r)
 {
   if (m.fctype == 0) {
@@ -141,7 +167,10 @@ void build_table(Model &m, double delta)
   auto at = [&](int i, int j, int q) -> double & { return m.tab[((size_t) i * k + j) * 4 + q]; };
   std::vector<double> y(k);
   for (int i = 0; i < n; i++) {
-    fnn_forward(m.fnn, xs[i], y.data());
+    if (m.use_fnn)
+      fnn_forward(m.fnn, xs[i], y.data());
+    else
+      analytic_forward(m, xs[i], y.data());
     double fc = cutoff_fc(m, xs[i]);
     for (int j = 0; j < k; j++) at(i, j, 0) = y[j] * fc;
   }
@@ -199,8 +228,7 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
     m.T_sym = geti("is_T_symmetric", 0, false) == 1;
     m.fp32_model = geti("precision", 64, false) == 32;
     if (geti("tdnp", 0, false) == 1) throw std::string("TDNP models are not supported");
-    if (geti("use_fnn", 0, true) != 1)
-      throw std::string("only use_fnn=1 models are supported (the radial path the reference wires)");
+    m.use_fnn = geti("use_fnn", 0, true) == 1;
     m.masses = getd("masses");
     if ((int) m.masses.size() != m.nelt) throw std::string("masses size != nelt");
     std::vector<int> nums = z.at("numbers").as_ints();
@@ -217,32 +245,55 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
       int kk = 1;
       for (int j = 0; j < m.nelt; j++) m.eltij[i][j] = (i == j) ? 0 : kk++;  // tensormd.cpp:165-176
     }
-    // radial network
-    DenseNet &f = m.fnn;
-    f.nlayers = geti("fnn::nlayers", 0, true);
-    if (f.nlayers < 2) throw std::string("fnn::nlayers must be >= 2");  // fnn.cpp:64-67
-    std::vector<int> fs = z.at("fnn::layer_sizes").as_ints();
-    if ((int) fs.size() != f.nlayers) throw std::string("fnn::layer_sizes length");
-    f.sizes.assign(1, 1);
-    for (int v : fs) f.sizes.push_back(v);
-    f.act = geti("fnn::actfn", 1, true);
-    if (f.act != ACT_SOFTPLUS && f.act != ACT_SQUAREPLUS)
-      throw std::string("fnn::actfn must be softplus(1) or squareplus(3), the activations "
-                        "the reference radial network applies (nnp.cu:89-95)");
-    f.outbias = geti("fnn::apply_output_bias", 0, true) == 1;
-    f.resnet = geti("fnn::use_resnet_dt", 0, true) == 1;
-    for (int l = 0; l < f.nlayers; l++) {
-      f.W.push_back(getd("fnn::weights_0_" + std::to_string(l)));
-      if ((int) f.W[l].size() != f.sizes[l] * f.sizes[l + 1]) throw std::string("fnn weight shape");
-      if (l < f.nlayers - 1 || f.outbias)
-        f.B.push_back(getd("fnn::biases_0_" + std::to_string(l)));
-      else
-        f.B.emplace_back();
+    if (m.use_fnn) {
+      // radial network
+      DenseNet &f = m.fnn;
+      f.nlayers = geti("fnn::nlayers", 0, true);
+      if (f.nlayers < 2) throw std::string("fnn::nlayers must be >= 2");  // fnn.cpp:64-67
+      std::vector<int> fs = z.at("fnn::layer_sizes").as_ints();
+      if ((int) fs.size() != f.nlayers) throw std::string("fnn::layer_sizes length");
+      f.sizes.assign(1, 1);
+      for (int v : fs) f.sizes.push_back(v);
+      f.act = geti("fnn::actfn", 1, true);
+      if (f.act != ACT_SOFTPLUS && f.act != ACT_SQUAREPLUS)
+        throw std::string("fnn::actfn must be softplus(1) or squareplus(3), the activations "
+                          "the reference radial network applies (nnp.cu:89-95)");
+      f.outbias = geti("fnn::apply_output_bias", 0, true) == 1;
+      f.resnet = geti("fnn::use_resnet_dt", 0, true) == 1;
+      for (int l = 0; l < f.nlayers; l++) {
+        f.W.push_back(getd("fnn::weights_0_" + std::to_string(l)));
+        if ((int) f.W[l].size() != f.sizes[l] * f.sizes[l + 1]) throw std::string("fnn weight shape");
+        if (l < f.nlayers - 1 || f.outbias)
+          f.B.push_back(getd("fnn::biases_0_" + std::to_string(l)));
+        else
+          f.B.emplace_back();
+      }
+      m.k = geti("fnn::num_filters", 0, true);
+      if (f.sizes.back() != m.k) throw std::string("fnn output width != fnn::num_filters");
+    } else {
+      // analytic radial functions: descriptor::method and one parameter
+      // array per parameter of the function, of length num_filters
+      // (Descriptor::Descriptor, src/ML-TENSORMD/descriptor.h:37-87)
+      m.radial_fn = geti("descriptor::method", RADIAL_PEXP, false);
+      static const char *const keys[4][3] = {{"rl", "pl", nullptr},
+                                             {"D", "gamma", "r0"},
+                                             {"A", "beta", "re"},
+                                             {"eta", "omega", nullptr}};
+      if (m.radial_fn < 0 || m.radial_fn > 3) throw std::string("descriptor::method must be 0..3");
+      m.rparams.clear();
+      for (int q = 0; q < 3 && keys[m.radial_fn][q]; q++)
+        m.rparams.push_back(getd(std::string("descriptor::") + keys[m.radial_fn][q]));
+      m.k = (int) m.rparams[0].size();
+      for (const auto &v : m.rparams)
+        if ((int) v.size() != m.k) throw std::string("descriptor parameter arrays differ in length");
+      for (int k = 0; k < m.k; k++) {
+        const bool bad = (m.radial_fn == RADIAL_PEXP && !(m.rparams[0][k] != 0.0)) ||
+                         (m.radial_fn == RADIAL_DENSITY && !(m.rparams[2][k] != 0.0));
+        if (bad) throw std::string("descriptor parameter divides by zero");
+      }
     }
-    m.k = geti("fnn::num_filters", 0, true);
-    if (f.sizes.back() != m.k) throw std::string("fnn output width != fnn::num_filters");
     if (!filters_supported(m.k))
-      throw std::string("fnn::num_filters " + std::to_string(m.k) +
+      throw std::string("num_filters " + std::to_string(m.k) +
                         " is not supported: the descriptor kernels take 16, 24, 32, 64 or 128");
     // main networks
     int nl = geti("nlayers", 0, true);

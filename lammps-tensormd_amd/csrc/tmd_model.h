@@ -28,6 +28,10 @@ struct DenseNet {
   bool res_layer(int l) const { return resnet && l > 0 && l < nlayers - 1 && sizes[l] == sizes[l + 1]; }
 };
 
+// Analytic radial functions of a use_fnn = 0 model (Descriptor_fn,
+// src/ML-TENSORMD/descriptor.h:26), one parameter set per filter.
+enum RadialFn { RADIAL_PEXP = 0, RADIAL_MORSE = 1, RADIAL_DENSITY = 2, RADIAL_SF = 3 };
+
 struct Model {
   int nelt = 0;
   double rc = 0;
@@ -43,7 +47,10 @@ struct Model {
   std::vector<std::string> symbols;
   std::vector<double> T;  // [nm][nd]
   int eltij[8][8] = {};
+  bool use_fnn = true;       // radial functions from the FNN, else analytic
   DenseNet fnn;
+  int radial_fn = RADIAL_PEXP;             // analytic kind (use_fnn false)
+  std::vector<std::vector<double>> rparams;  // [parameter][filter]
   std::vector<DenseNet> mlp;  // per element
   // Kerley table, [n][k][4] = {y, s, c1, c2}
   int tab_n = 0;

@@ -25,10 +25,31 @@ def _symbol_codes(symbols):
     return np.array(codes, dtype=np.int32)
 
 
+RADIAL = {"pexp": 0, "morse": 1, "density": 2, "sf": 3}
+
+
+def analytic_radial_params(method, k, rmax, rng):
+    """Seeded parameter sets of the analytic radial functions
+    (src/ML-TENSORMD/descriptor.cpp:51-132), k filters spread over (0, rmax)
+    and scaled so that a dense crystal's descriptors are of order one."""
+    centers = np.linspace(0.3 * rmax, 0.9 * rmax, k)
+    if method == "pexp":   # exp(-(r / rl)^pl)
+        return {"rl": centers, "pl": rng.uniform(1.0, 4.0, k)}
+    if method == "morse":  # D (exp(-2 gamma (r - r0)) - 2 exp(-gamma (r - r0)))
+        return {"D": rng.uniform(0.005, 0.02, k), "gamma": rng.uniform(0.2, 0.6, k),
+                "r0": np.linspace(0.45 * rmax, 0.9 * rmax, k)}
+    if method == "density":  # A exp(-beta (r - re) / re)
+        return {"A": rng.uniform(0.1, 0.5, k), "beta": rng.uniform(1.0, 4.0, k), "re": centers}
+    if method == "sf":  # exp(-eta (r - omega)^2 / rc^2)
+        return {"eta": rng.uniform(5.0, 40.0, k), "omega": centers}
+    raise ValueError(method)
+
+
 def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn="tanh",
                 fnn_hidden=None, fnn_actfn="softplus", max_moment=3, t_symmetric=False,
                 use_resnet_dt=False,
-                apply_output_bias=True, fctype=0, seed=0, radial_scale=0.1, bias_sigma=0.05):
+                apply_output_bias=True, fctype=0, seed=0, radial_scale=0.1, bias_sigma=0.05,
+                radial=None):
     """Write a random-init TensorMD model (Xavier-normal FP64 weights).
 
     layer_sizes: main MLP widths, last must be 1 (e.g. [256, 128, 64, 1]).
@@ -36,6 +57,10 @@ def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn=
     its output width is num_filters.  radial_scale multiplies the radial
     network's output layer so that the descriptors of a dense crystal are of
     order one (trained models are normalised the same way by training).
+    radial: None for the radial network, or one of "pexp", "morse",
+    "density", "sf" for analytic radial functions (use_fnn 0, the
+    descriptor:: keys of src/ML-TENSORMD/descriptor.h:37-87) with seeded
+    parameters.
     """
     rng = np.random.default_rng(seed)
     nelt = len(symbols)
@@ -51,26 +76,32 @@ def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn=
     z["numbers"] = _symbol_codes(symbols)
     z["tdnp"] = i32(0)
     z["precision"] = i32(64)
-    z["use_fnn"] = i32(1)
-    z["fnn::nlayers"] = i32(len(fnn_sizes))
-    z["fnn::layer_sizes"] = i32(fnn_sizes)
-    z["fnn::num_filters"] = i32(k)
-    z["fnn::actfn"] = i32(ACT[fnn_actfn])
-    z["fnn::use_resnet_dt"] = i32(0)
-    z["fnn::apply_output_bias"] = i32(0)
 
     def xavier(n_in, n_out):
         return rng.normal(0.0, np.sqrt(2.0 / (n_in + n_out)), size=(n_in, n_out))
 
-    prev = 1
-    for l, w in enumerate(fnn_sizes):
-        W = xavier(prev, w)
-        if l == len(fnn_sizes) - 1:
-            W *= radial_scale
-        z[f"fnn::weights_0_{l}"] = W.reshape(w) if prev == 1 else W
-        if l < len(fnn_sizes) - 1:
-            z[f"fnn::biases_0_{l}"] = rng.normal(0.0, 0.5, size=w)
-        prev = w
+    if radial is None:
+        z["use_fnn"] = i32(1)
+        z["fnn::nlayers"] = i32(len(fnn_sizes))
+        z["fnn::layer_sizes"] = i32(fnn_sizes)
+        z["fnn::num_filters"] = i32(k)
+        z["fnn::actfn"] = i32(ACT[fnn_actfn])
+        z["fnn::use_resnet_dt"] = i32(0)
+        z["fnn::apply_output_bias"] = i32(0)
+        prev = 1
+        for l, w in enumerate(fnn_sizes):
+            W = xavier(prev, w)
+            if l == len(fnn_sizes) - 1:
+                W *= radial_scale
+            z[f"fnn::weights_0_{l}"] = W.reshape(w) if prev == 1 else W
+            if l < len(fnn_sizes) - 1:
+                z[f"fnn::biases_0_{l}"] = rng.normal(0.0, 0.5, size=w)
+            prev = w
+    else:
+        z["use_fnn"] = i32(0)
+        z["descriptor::method"] = i32(RADIAL[radial])
+        for key, val in analytic_radial_params(radial, k, rmax, rng).items():
+            z[f"descriptor::{key}"] = np.asarray(val, dtype=np.float64)
     sizes = list(layer_sizes)
     assert sizes[-1] == 1
     z["nlayers"] = i32(len(sizes))

@@ -58,6 +58,10 @@ struct Model {
   std::vector<int> mono;  // [nd][3] exponents
   int eltij[8][8];
   Dense fnn;
+  bool use_fnn = true;
+  // analytic radial functions (use_fnn 0): kind and [parameter][filter]
+  int radial_fn = 0;
+  std::vector<std::vector<double>> rparams;
   std::vector<Dense> mlp;
   std::vector<double> masses;
   // Kerley table: tab[(n*4 + q)*k + j], q = 0:y 1:s 2:c1 3:c2
@@ -264,6 +268,43 @@ double cutoff_fc(const Model &m, double r)
   return 1.0 + gamma * Z5 * Z - (gamma + 1.0) * Z5;
 }
 
+// Analytic radial functions of a use_fnn = 0 model, following
+// Descriptor::pexp/morse/density/sf (src/ML-TENSORMD/descriptor.cpp:51-132):
+//   0 PowerExp  exp(-(r / rl)^pl)
+//   1 Morse     D (exp(-2 gamma (r - r0)) - 2 exp(-gamma (r - r0)))
+//   2 Density   A exp(-beta (r - re) / re)
+//   3 SF        exp(-eta (r - omega)^2 / rc^2)
+// Pinned against the closed forms evaluated independently in numpy and, via
+// the table, against the GPU path.
+void radial_fn_values(const Model &m, double r, double *y)
+{
+  const auto &p = m.rparams;
+  for (int j = 0; j < m.k; j++) {
+    double v;
+    if (m.radial_fn == 0) {
+      v = std::exp(-std::pow(r / p[0][j], p[1][j]));
+    } else if (m.radial_fn == 1) {
+      double g = p[1][j] * (r - p[2][j]);
+      v = p[0][j] * (std::exp(-2.0 * g) - 2.0 * std::exp(-g));
+    } else if (m.radial_fn == 2) {
+      v = p[0][j] * std::exp(-p[1][j] * (r - p[2][j]) / p[2][j]);
+    } else {
+      double d = r - p[1][j];
+      v = std::exp(-p[0][j] * d * d / (m.rc * m.rc));
+    }
+    y[j] = v;
+  }
+}
+
+// the radial functions before the cutoff: the FNN or the analytic ones
+void radial_values(const Model &m, double r, double *y, DenseWork &w)
+{
+  if (m.use_fnn)
+    dense_forward(m.fnn, &r, y, w);
+  else
+    radial_fn_values(m, r, y);
+}
+
 void build_table(Model &m, double delta)
 {
   int n;
@@ -287,7 +328,7 @@ void build_table(Model &m, double delta)
   std::vector<double> y(k);
   for (int i = 0; i < n; i++) {
     double x = xs[i];
-    dense_forward(m.fnn, &x, y.data(), w);
+    radial_values(m, x, y.data(), w);
     double fc = cutoff_fc(m, x);
     for (int j = 0; j < k; j++) m.tab[((size_t) i * 4 + 0) * k + j] = y[j] * fc;
   }
@@ -798,7 +839,7 @@ int tmdo_finalize(void *h, double interp_dx)
     m->fctype = geti("fctype", 0, true);
     m->T_sym = geti("is_T_symmetric", 0, false) == 1;
     m->fp32_model = geti("precision", 64, false) == 32;
-    if (geti("use_fnn", 0, true) != 1) throw std::string("only use_fnn=1 models are supported");
+    m->use_fnn = geti("use_fnn", 0, true) == 1;
     if (geti("tdnp", 0, false) == 1) throw std::string("tdnp models are not supported");
     m->masses = getd("masses");
     build_monomials(*m);
@@ -807,31 +848,45 @@ int tmdo_finalize(void *h, double interp_dx)
       int kk = 1;
       for (int j = 0; j < m->nelt; j++) m->eltij[i][j] = (i == j) ? 0 : kk++;
     }
+    if (!m->use_fnn) {
+      // descriptor.h:37-87: descriptor::method (default PowerExp) and one
+      // array per parameter, num_filters long
+      m->radial_fn = geti("descriptor::method", 0, false);
+      const char *names[4][3] = {{"descriptor::rl", "descriptor::pl", nullptr},
+                                 {"descriptor::D", "descriptor::gamma", "descriptor::r0"},
+                                 {"descriptor::A", "descriptor::beta", "descriptor::re"},
+                                 {"descriptor::eta", "descriptor::omega", nullptr}};
+      if (m->radial_fn < 0 || m->radial_fn > 3) throw std::string("descriptor::method out of range");
+      for (int q = 0; q < 3 && names[m->radial_fn][q]; q++) m->rparams.push_back(getd(names[m->radial_fn][q]));
+      m->k = (int) m->rparams[0].size();
+    }
     // FNN
     Dense &f = m->fnn;
-    f.nlayers = geti("fnn::nlayers", 0, true);
-    auto fs = m->ikeys.at("fnn::layer_sizes");
-    f.sizes.assign(1, 1);
-    for (int v : fs) f.sizes.push_back(v);
-    f.act = geti("fnn::actfn", 0, true);
-    f.outbias = geti("fnn::apply_output_bias", 0, true) == 1;
-    f.resnet = geti("fnn::use_resnet_dt", 0, true) == 1;
-    {
-      int nres = 0;
-      for (int l = 1; l < f.nlayers; l++)
-        if (f.sizes[l] == f.sizes[l + 1]) nres++;
-      if (nres == 0) f.resnet = false;  // nnp.cpp:179-181
+    if (m->use_fnn) {
+      f.nlayers = geti("fnn::nlayers", 0, true);
+      auto fs = m->ikeys.at("fnn::layer_sizes");
+      f.sizes.assign(1, 1);
+      for (int v : fs) f.sizes.push_back(v);
+      f.act = geti("fnn::actfn", 0, true);
+      f.outbias = geti("fnn::apply_output_bias", 0, true) == 1;
+      f.resnet = geti("fnn::use_resnet_dt", 0, true) == 1;
+      {
+        int nres = 0;
+        for (int l = 1; l < f.nlayers; l++)
+          if (f.sizes[l] == f.sizes[l + 1]) nres++;
+        if (nres == 0) f.resnet = false;  // nnp.cpp:179-181
+      }
+      for (int l = 0; l < f.nlayers; l++) {
+        f.W.push_back(getd("fnn::weights_0_" + std::to_string(l)));
+        if (l < f.nlayers - 1 || f.outbias)
+          f.B.push_back(getd("fnn::biases_0_" + std::to_string(l)));
+        else
+          f.B.emplace_back();
+        if ((int) f.W[l].size() != f.sizes[l] * f.sizes[l + 1]) throw std::string("fnn weight shape");
+      }
+      m->k = geti("fnn::num_filters", 0, true);
+      if (f.sizes.back() != m->k) throw std::string("fnn output width != num_filters");
     }
-    for (int l = 0; l < f.nlayers; l++) {
-      f.W.push_back(getd("fnn::weights_0_" + std::to_string(l)));
-      if (l < f.nlayers - 1 || f.outbias)
-        f.B.push_back(getd("fnn::biases_0_" + std::to_string(l)));
-      else
-        f.B.emplace_back();
-      if ((int) f.W[l].size() != f.sizes[l] * f.sizes[l + 1]) throw std::string("fnn weight shape");
-    }
-    m->k = geti("fnn::num_filters", 0, true);
-    if (f.sizes.back() != m->k) throw std::string("fnn output width != num_filters");
     // MLP
     int nl = geti("nlayers", 0, true);
     auto ls = m->ikeys.at("layer_sizes");
@@ -911,13 +966,13 @@ int tmdo_radial(void *h, int n, const double *r, double *H, double *dH)
   return 0;
 }
 
-// Radial network FNN_k(r)*fc(r) evaluated directly (no table)
+// Radial functions (FNN or analytic) times fc(r), evaluated directly (no table)
 int tmdo_fnn(void *h, int n, const double *r, double *y)
 {
   auto *m = static_cast<Model *>(h);
   DenseWork w;
   for (int i = 0; i < n; i++) {
-    dense_forward(m->fnn, &r[i], y + (size_t) i * m->k, w);
+    radial_values(*m, r[i], y + (size_t) i * m->k, w);
     double fc = cutoff_fc(*m, r[i]);
     for (int j = 0; j < m->k; j++) y[(size_t) i * m->k + j] *= fc;
   }

@@ -496,6 +496,26 @@ def test_k128_model_vs_oracle():
     assert e["dE"] <= e_tol
 
 
+@pytest.mark.parametrize("method", ["pexp", "morse", "density", "sf"])
+def test_analytic_radial_models(tmp_path, method):
+    """use_fnn 0 models (analytic radial functions of
+    src/ML-TENSORMD/descriptor.cpp:51-132, tabulated like the radial
+    network, descriptor.cpp:146-172): FP32 path vs the FP64 oracle."""
+    say(f"case: analytic radial {method}, 3 elements, bcc 4x4x4 jittered")
+    p = W.write_model(str(tmp_path / f"{method}.npz"), symbols=["Mo", "Nb", "Ta"],
+                      masses=[95.95, 92.906, 180.95], rmax=5.5, num_filters=24,
+                      layer_sizes=[64, 32, 1], actfn="tanh", seed=70, radial=method)
+    om = oracle.OracleModel(p, 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (4, 4, 4), nelt=3, sigma=0.08, seed=71)
+    ref = om.compute_periodic(x, t - 1, cell)
+    m = TensorMD(p, ["Mo", "Nb", "Ta"], "fp32", 0.001)
+    got = m.system(x, t, cell, skin=1.0).compute()
+    e = _errs(got, ref, len(x))
+    say("errors", e, "max |F|", float(abs(ref["f"]).max()))
+    assert float(abs(ref["f"]).max()) > 1e-3  # the model exerts forces
+    assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
+
+
 @pytest.mark.parametrize("k,nelt", [(64, 5), (128, 3), (128, 5), (32, 8)])
 def test_wide_slot_models(tmp_path, k, nelt):
     """Models whose slots take several register passes in the descriptor and

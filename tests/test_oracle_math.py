@@ -286,3 +286,55 @@ def test_list_mode_scale_and_partial_ilist(c2):
         assert np.allclose(parts[0][k] + parts[1][k], full[k], atol=1e-11), k
     assert np.array_equal(parts[0]["eatom"][B], np.zeros(len(B)))
     assert np.allclose(parts[0]["eatom"][A], full["eatom"][A], atol=1e-13)
+
+
+@pytest.mark.parametrize("method", ["pexp", "morse", "density", "sf"])
+def test_analytic_radial_functions(tmp_path, method):
+    """use_fnn 0 models: the oracle's radial functions equal the closed forms
+    of src/ML-TENSORMD/descriptor.cpp:51-132 (evaluated here in numpy) times
+    the cosine cutoff; the Kerley table interpolates them with O(dx^3) error
+    in H and O(dx^2) in dH/dr."""
+    p = W.write_model(str(tmp_path / f"{method}.npz"), symbols=["A", "B"], masses=[10.0, 20.0],
+                      rmax=5.5, num_filters=16, layer_sizes=[32, 1], seed=61, radial=method)
+    om = oracle.OracleModel(p, 0.001)
+    with np.load(p) as z:
+        par = {k.split("::")[1]: z[k] for k in z.files if k.startswith("descriptor::") and k != "descriptor::method"}
+        assert int(z["use_fnn"]) == 0
+    r = np.linspace(0.5, 5.4, 97)
+    R = r[:, None]
+    if method == "pexp":
+        f = np.exp(-(R / par["rl"]) ** par["pl"])
+    elif method == "morse":
+        g = par["gamma"] * (R - par["r0"])
+        f = par["D"] * (np.exp(-2 * g) - 2 * np.exp(-g))
+    elif method == "density":
+        f = par["A"] * np.exp(-par["beta"] * (R - par["re"]) / par["re"])
+    else:
+        f = np.exp(-par["eta"] * (R - par["omega"]) ** 2 / 5.5 ** 2)
+    fc = 0.5 + 0.5 * np.cos(np.pi * R / 5.5)
+    assert om.k == 16
+    assert np.allclose(om.fnn(r), f * fc, rtol=1e-13, atol=1e-15)
+    H, dH = om.radial(r)
+    assert abs(H - f * fc).max() <= 1e-7 * max(1.0, abs(f).max())
+    h = 1e-6
+    num = (om.fnn(r + h) - om.fnn(r - h)) / (2 * h)
+    assert abs(dH - num).max() <= 2e-4 * max(1.0, abs(num).max())
+
+
+def test_analytic_model_forces_are_minus_energy_gradient(tmp_path):
+    """Forces of an analytic-radial (Morse) model: central finite differences
+    of the oracle energy, and zero net force."""
+    p = W.write_model(str(tmp_path / "morse.npz"), symbols=["A", "B"], masses=[10.0, 20.0],
+                      rmax=5.5, num_filters=16, layer_sizes=[32, 1], seed=62, radial="morse")
+    m = oracle.OracleModel(p, 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (2, 2, 2), nelt=2, sigma=0.1, seed=9)
+    out = m.compute_periodic(x, t - 1, cell)
+    h = 1e-5
+    for i in (0, 5, 11):
+        for a in range(3):
+            xp = x.copy(); xp[i, a] += h
+            xm = x.copy(); xm[i, a] -= h
+            fd = -(m.compute_periodic(xp, t - 1, cell)["energy"]
+                   - m.compute_periodic(xm, t - 1, cell)["energy"]) / (2 * h)
+            assert abs(fd - out["f"][i, a]) < 2e-8 + 1e-6 * abs(fd), (i, a)
+    assert abs(out["f"].sum(0)).max() < 1e-12

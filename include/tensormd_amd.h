@@ -93,7 +93,7 @@ int tmd_masses(const tmd_ctx *ctx, double *mass_by_type);
 /* map[t] = model element index for type t (0-based), -1 if unmapped */
 int tmd_type_map(const tmd_ctx *ctx, int *map);
 /* nelt, num_filters, max_moment, descriptor width, mlp layer count, as in the
- * model file (max_moment 0..3; a model with fewer than 4 moments is evaluated
+ * model file (max_moment 0..5; a model with fewer than 4 moments is evaluated
  * with zero first-layer weights on the moment channels it lacks) */
 int tmd_model_info(const tmd_ctx *ctx, int *info5);
 

@@ -112,17 +112,39 @@ int upload_model(tmd_ctx *ctx)
     g.slots[a] = 0;
     for (int b = 0; b < m.nelt; b++) g.slots[a] |= (uint32_t) m.eltij[a][b] << (4 * b);
   }
-  // the kernels hold T as the fixed multiplicity table plus the T-symmetric
-  // extras; check that the model's T is exactly that
+  // the kernels hold T as the fixed multiplicity table (plus the T-symmetric
+  // extras of max_moment 3); check that the model's T is exactly that
   g.tsym = m.T_sym ? 1 : 0;
-  if (m.nm != kNM || m.nd != kND) return (set_err(ctx, "max_moment 3 models only"), TMD_ERR_MODEL);
-  for (int mm = 0; mm < kNM; mm++)
-    for (int d = 0; d < kND; d++) {
-      double t = kDeg[d] == mm ? (double) kMult[d] : 0.0;
-      if (g.tsym && mm == 2 && d == 0) t = -1.0 / 3.0;
-      if (g.tsym && mm == 3 && d >= 1 && d <= 3) t = -3.0 / 5.0;
-      if (m.T[mm * m.nd + d] != t) return (set_err(ctx, "unexpected descriptor T table"), TMD_ERR_MODEL);
-    }
+  g.mm = m.max_moment;
+  if (m.max_moment < 3 || m.max_moment > 5 || m.nm != m.max_moment + 1)
+    return (set_err(ctx, "max_moment 3, 4 or 5 expected after padding"), TMD_ERR_MODEL);
+  if (m.max_moment == 3) {
+    if (m.nd != kND) return (set_err(ctx, "unexpected monomial count"), TMD_ERR_MODEL);
+    for (int mm = 0; mm < kNM; mm++)
+      for (int d = 0; d < kND; d++) {
+        double t = kDeg[d] == mm ? (double) kMult[d] : 0.0;
+        if (g.tsym && mm == 2 && d == 0) t = -1.0 / 3.0;
+        if (g.tsym && mm == 3 && d >= 1 && d <= 3) t = -3.0 / 5.0;
+        if (m.T[mm * m.nd + d] != t) return (set_err(ctx, "unexpected descriptor T table"), TMD_ERR_MODEL);
+      }
+  } else {
+    // max_moment 4, 5: T[deg d][d] = deg! / (a! b! c!), no T-symmetric extras
+    // (tensormd.cpp:217,229 apply them only up to max_moment 3)
+    int d = 0;
+    auto fact = [](int n) {
+      double v = 1;
+      for (int q = 2; q <= n; q++) v *= q;
+      return v;
+    };
+    for (int n = 0; n <= m.max_moment; n++)
+      for (int a = n; a >= 0; a--)
+        for (int b = n - a; b >= 0; b--, d++)
+          for (int mm = 0; mm < m.nm; mm++) {
+            const double t = mm == n ? fact(n) / (fact(a) * fact(b) * fact(n - a - b)) : 0.0;
+            if (d >= m.nd || m.T[mm * m.nd + d] != t)
+              return (set_err(ctx, "unexpected descriptor T table"), TMD_ERR_MODEL);
+          }
+  }
   g.rc2 = m.rc * m.rc;
   g.dx = m.dx;
   g.xdx = m.xdx;

@@ -44,6 +44,7 @@ struct ModelGeo {
   int k;
   int tab_n;
   int tsym;  // 1: T-symmetric descriptor
+  int mm;    // max_moment as evaluated: 3 (0-2 are padded to 3), 4 or 5
   uint32_t slots[kMaxElts];
   double rc2;
   double dx, xdx;
@@ -159,6 +160,17 @@ hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos, 
                         const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
                         const float *dEdG, const double *E, double scale, double *f, double *eatom,
                         double *watom, double *vatom, hipStream_t stream);
+
+// The same two kernels for max_moment 4 and 5 models (g.mm 4 or 5): 35 or 56
+// monomials (tmd_desc_hm.hip); launch_descriptor/launch_force call them.
+hipError_t launch_descriptor_hm(const ModelGeo &g, const Geom &geo, const double4 *pos,
+                                const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
+                                int nrows, const float4 *tab, float *G, uint32_t *kl,
+                                hipStream_t stream);
+hipError_t launch_force_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
+                           const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
+                           const float *dEdG, const double *E, double scale, double *f,
+                           double *eatom, double *watom, double *vatom, hipStream_t stream);
 
 // ---- MLP ----------------------------------------------------------------------
 // mode: 0 fp32, 1 bf16, 2 fp8 (e4m3 operands: weights with per-output-channel

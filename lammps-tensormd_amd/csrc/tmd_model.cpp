@@ -221,8 +221,7 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
     m.rc = getd("rmax").at(0);
     if (!(m.rc > 0)) throw std::string("rmax must be > 0");
     m.max_moment = geti("max_moment", 0, true);
-    if (m.max_moment < 0 || m.max_moment > 3)
-      throw std::string("this build evaluates max_moment 0..3 models (the shipped models use 3)");
+    if (m.max_moment < 0 || m.max_moment > 5) throw std::string("max_moment must be in 0..5");
     m.max_moment_file = m.max_moment;
     m.fctype = geti("fctype", 0, true);
     m.T_sym = geti("is_T_symmetric", 0, false) == 1;

@@ -516,6 +516,31 @@ def test_analytic_radial_models(tmp_path, method):
     assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
 
 
+@pytest.mark.parametrize("mm,k,nelt", [(4, 24, 4), (5, 16, 2), (5, 32, 3), (4, 128, 1)])
+def test_high_moment_models(tmp_path, mm, k, nelt):
+    """max_moment 4 and 5 models (35 and 56 monomials, tensormd.cpp:234-273;
+    the reference GPU path's calculate_dM_g/_h, kernel.cu:553-556): FP32 path
+    vs the FP64 oracle, energies, forces, virial and the descriptor rows."""
+    sym = ["Mo", "Nb", "Ta", "W"][:nelt]
+    say(f"case: max_moment {mm}, k={k}, {nelt} elements, bcc 4x4x4 jittered")
+    p = W.write_model(str(tmp_path / "hm.npz"), symbols=sym, masses=[50.0 + 10 * i for i in range(nelt)],
+                      rmax=5.5, num_filters=k, layer_sizes=[64, 32, 1], actfn="tanh", max_moment=mm,
+                      seed=80 + mm + k, radial_scale=0.2)
+    om = oracle.OracleModel(p, 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (4, 4, 4), nelt=nelt, sigma=0.08, seed=81)
+    ref = om.compute_periodic(x, t - 1, cell, want_desc=True)
+    m = TensorMD(p, sym, "fp32", 0.001)
+    assert m.info()["max_moment"] == mm and m.info()["width"] == nelt * k * (mm + 1)
+    s = m.system(x, t, cell, skin=1.0)
+    got = s.compute()
+    e = _errs(got, ref, len(x))
+    G, _ = s.descriptors(m.info()["width"])
+    dG = float(abs(G - ref["G"]).max())
+    say("errors", e, "max |dG|", dG, "max |G|", float(abs(ref["G"]).max()))
+    assert dG <= 1e-5 * max(1.0, float(abs(ref["G"]).max()))
+    assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
+
+
 @pytest.mark.parametrize("k,nelt", [(64, 5), (128, 3), (128, 5), (32, 8)])
 def test_wide_slot_models(tmp_path, k, nelt):
     """Models whose slots take several register passes in the descriptor and

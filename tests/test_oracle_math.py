@@ -338,3 +338,61 @@ def test_analytic_model_forces_are_minus_energy_gradient(tmp_path):
                    - m.compute_periodic(xm, t - 1, cell)["energy"]) / (2 * h)
             assert abs(fd - out["f"][i, a]) < 2e-8 + 1e-6 * abs(fd), (i, a)
     assert abs(out["f"].sum(0)).max() < 1e-12
+
+
+# Literal rows 4 and 5 of the multiplicity table, src/ML-TENSORMD/tensormd.cpp:234-273
+T_LITERAL_45 = {
+    4: [1, 4, 4, 6, 12, 6, 4, 12, 12, 4, 1, 4, 6, 4, 1],
+    5: [1, 5, 5, 10, 20, 10, 10, 30, 30, 10, 5, 20, 30, 20, 5, 1, 5, 10, 10, 5, 1],
+}
+
+
+@pytest.mark.parametrize("mm", [4, 5])
+def test_high_moment_T_and_contraction(tmp_path, mm):
+    """max_moment 4 and 5: T equals the reference's literal rows (no
+    T-symmetric extras above 3), sum_d T[m][d] M_d^2 = |u|^(2m), and G is the
+    brute-force rank-m tensor contraction (3^m index tuples)."""
+    p = W.write_model(str(tmp_path / f"m{mm}.npz"), symbols=["A", "B"], masses=[10.0, 20.0],
+                      rmax=5.0, num_filters=16, layer_sizes=[32, 1], max_moment=mm,
+                      t_symmetric=True, seed=63)
+    m = oracle.OracleModel(p, 0.001)
+    assert m.nd == (mm + 1) * (mm + 2) * (mm + 3) // 6 and m.nm == mm + 1
+    T = m.T()
+    first = {n: n * (n + 1) * (n + 2) // 6 for n in range(mm + 2)}
+    for n in range(mm + 1):
+        row = np.zeros(m.nd)
+        lit = T_LITERAL_45.get(n)
+        if lit is None:  # rows 0-3: the max_moment-3 table
+            lit = [T_LITERAL.get((n, d), 0) for d in range(first[n], first[n + 1])]
+        row[first[n]:first[n + 1]] = lit
+        assert np.array_equal(T[n], row), n
+    rng = np.random.default_rng(4)
+    u = rng.normal(size=(20, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    M, _ = m.monomials(u)
+    assert np.allclose(M ** 2 @ T.T, 1.0, atol=1e-12)
+    x, t, cell = W.crystal("bcc", 3.2, (3, 3, 3), nelt=2, sigma=0.1, seed=5)
+    out = m.compute_periodic(x, t - 1, cell, want_desc=True)
+    off, nj, ns = oracle.neighbors(x, cell, m.rc)
+    i = 4
+    ei = t[i] - 1
+    terms = {}
+    for q in range(off[i], off[i + 1]):
+        d = x[nj[q]] - x[i] + ns[q] @ cell
+        r = np.linalg.norm(d)
+        if not (1e-10 < r * r < m.rc ** 2):
+            continue
+        b = 0 if t[nj[q]] - 1 == ei else 1
+        H, _ = m.radial(np.array([r]))
+        terms.setdefault(b, []).append((H[0], d / r))
+    G = out["G"][i].reshape(2, m.k, mm + 1)
+    import itertools
+    for b in range(2):
+        for k in (0, 9):
+            ref = [sum(h[k] for h, _ in terms[b])]
+            for order in range(1, mm + 1):
+                tot = 0.0
+                for idx in itertools.product(range(3), repeat=order):
+                    tot += sum(h[k] * np.prod(uu[list(idx)]) for h, uu in terms[b]) ** 2
+                ref.append(tot)
+            assert np.allclose(G[b, k], ref, rtol=1e-11, atol=1e-13), (b, k)

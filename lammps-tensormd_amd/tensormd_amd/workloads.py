@@ -49,7 +49,7 @@ def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn=
                 fnn_hidden=None, fnn_actfn="softplus", max_moment=3, t_symmetric=False,
                 use_resnet_dt=False,
                 apply_output_bias=True, fctype=0, seed=0, radial_scale=0.1, bias_sigma=0.05,
-                radial=None):
+                radial=None, tdnp=None):
     """Write a random-init TensorMD model (Xavier-normal FP64 weights).
 
     layer_sizes: main MLP widths, last must be 1 (e.g. [256, 128, 64, 1]).
@@ -61,6 +61,12 @@ def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn=
     "density", "sf" for analytic radial functions (use_fnn 0, the
     descriptor:: keys of src/ML-TENSORMD/descriptor.h:37-87) with seeded
     parameters.
+    tdnp: None, or a dict describing a temperature-dependent model (tdnp 1,
+    the H::, U:: and S:: networks of src/ML-TENSORMD/tdnp.cpp:60-148) that
+    replaces the main MLP: "H" hidden and output widths of the network from
+    the descriptor (its output gains the electron-temperature column when
+    read), "U" and "S" widths ending in 1, optional "actfn", "resnet" and
+    "sommerfeld".  layer_sizes is then ignored.
     """
     rng = np.random.default_rng(seed)
     nelt = len(symbols)
@@ -102,26 +108,46 @@ def write_model(path, *, symbols, masses, rmax, num_filters, layer_sizes, actfn=
         z["descriptor::method"] = i32(RADIAL[radial])
         for key, val in analytic_radial_params(radial, k, rmax, rng).items():
             z[f"descriptor::{key}"] = np.asarray(val, dtype=np.float64)
-    sizes = list(layer_sizes)
-    assert sizes[-1] == 1
-    z["nlayers"] = i32(len(sizes))
     z["max_moment"] = i32(max_moment)
-    z["actfn"] = i32(ACT[actfn])
     z["fctype"] = i32(fctype)
-    z["layer_sizes"] = i32(sizes)
-    z["use_resnet_dt"] = i32(1 if use_resnet_dt else 0)
-    z["apply_output_bias"] = i32(1 if apply_output_bias else 0)
     z["is_T_symmetric"] = i32(1 if t_symmetric else 0)
-    for e in range(nelt):
-        prev = gw
-        for l, w in enumerate(sizes):
-            W = xavier(prev, w)
-            z[f"weights_{e}_{l}"] = W.reshape(prev) if w == 1 else W
-            if l < len(sizes) - 1:
-                z[f"biases_{e}_{l}"] = rng.normal(0.0, bias_sigma, size=w)
-            elif apply_output_bias:
-                z[f"biases_{e}_{l}"] = np.array(rng.normal(0.0, 0.1))
-            prev = w
+
+    def nets(scope, sizes, n_in, act, resnet, outbias):
+        z[f"{scope}nlayers"] = i32(len(sizes))
+        z[f"{scope}actfn"] = i32(ACT[act])
+        z[f"{scope}layer_sizes"] = i32(sizes)
+        z[f"{scope}use_resnet_dt"] = i32(1 if resnet else 0)
+        z[f"{scope}apply_output_bias"] = i32(1 if outbias else 0)
+        for e in range(nelt):
+            prev = n_in
+            for l, w in enumerate(sizes):
+                W = xavier(prev, w)
+                z[f"{scope}weights_{e}_{l}"] = W.reshape(prev) if w == 1 else W
+                if l < len(sizes) - 1:
+                    z[f"{scope}biases_{e}_{l}"] = rng.normal(0.0, bias_sigma, size=w)
+                elif outbias:
+                    z[f"{scope}biases_{e}_{l}"] = (np.array(rng.normal(0.0, 0.1)) if w == 1
+                                                   else rng.normal(0.0, 0.1, size=w))
+                prev = w
+
+    if tdnp is None:
+        sizes = list(layer_sizes)
+        assert sizes[-1] == 1
+        nets("", sizes, gw, actfn, use_resnet_dt, apply_output_bias)
+    else:
+        z["tdnp"] = i32(1)
+        if tdnp.get("sommerfeld"):
+            z["tdnp::Sommerfeld"] = i32(1)
+        act = tdnp.get("actfn", actfn)
+        res = tdnp.get("resnet", use_resnet_dt)
+        h = list(tdnp["H"])
+        # H's last layer is linear and without bias: the reference appends a
+        # zero column to its weights only (tdnp.cpp:91-104)
+        nets("H::", h, gw, act, res, False)
+        for scope in ("U", "S"):
+            sz = list(tdnp[scope])
+            assert sz[-1] == 1
+            nets(f"{scope}::", sz, h[-1] + 1, act, res, apply_output_bias)
     np.savez(path, **z)
     return path
 

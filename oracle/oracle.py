@@ -64,6 +64,8 @@ def lib():
         L.tmdo_num_threads.restype = ctypes.c_int
         L.tmdo_set_threads.argtypes = [ctypes.c_int]
         L.tmdo_set_emulation.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.tmdo_set_etemp.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.tmdo_tdnp_last.argtypes = [ctypes.c_void_p, ctypes.c_long, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -132,6 +134,23 @@ class OracleModel:
         """0 = the FP64 chain; otherwise a bitmask of EMUL_* stages in FP32."""
         lib().tmdo_set_emulation(self.h, int(flags))
         return self
+
+    def set_etemp(self, etemp_ev):
+        """Electron temperature (eV) of a TDNP model."""
+        lib().tmdo_set_etemp(self.h, float(etemp_ev))
+        return self
+
+    def tdnp_last(self, n):
+        """TDNP results of the last compute call on n atoms (nlocal in list
+        mode): per-atom electron entropy and free energy (scaled), and their
+        sums S and A."""
+        sa = np.zeros(n)
+        aa = np.zeros(n)
+        S = np.zeros(1)
+        A = np.zeros(1)
+        if lib().tmdo_tdnp_last(self.h, int(n), _p(sa, _dp), _p(aa, _dp), _p(S, _dp), _p(A, _dp)) != 0:
+            raise RuntimeError(lib().tmdo_last_error().decode())
+        return dict(satom=sa, aatom=aa, entropy=float(S[0]), free_energy=float(A[0]))
 
     # ---- pieces -------------------------------------------------------
     def T(self):

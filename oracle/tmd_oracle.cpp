@@ -70,6 +70,16 @@ struct Model {
   std::vector<double> tab;
   bool ready = false;
   std::string err;
+  // TDNP (tdnp = 1): per element H, U and S networks instead of the main
+  // MLP; the electron temperature (eV) and the per-atom electron entropy and
+  // free energy of the last compute call (scaled), with their sums
+  bool tdnp = false;
+  bool sommerfeld = false;
+  int ydim = 0;
+  std::vector<Dense> tdH, tdU, tdS;
+  double etemp = 0.0;
+  std::vector<double> last_s, last_a;
+  double last_S = 0.0, last_A = 0.0;
   // Reduced-precision emulation (tmdo_set_emulation): bit 0 descriptor,
   // bit 1 MLP, bit 2 force stage in FP32; bit 3 runs the emulated chain
   // even when no stage is FP32 (it must then agree with the FP64 chain).
@@ -408,9 +418,44 @@ struct Nbr {
 
 struct AtomOut {
   double E;
+  double s = 0.0, a = 0.0;  // TDNP: electron entropy and free energy (unscaled)
   std::vector<double> F;  // [nnbr][3] dE_i/dDelta (scaled)
   std::vector<double> G, dEdG;
 };
+
+// TDNP atom (TDNP::forward/backward, src/ML-TENSORMD/tdnp.cpp:150-218, with
+// set_element_tdnp_device and the axpy of backward_tdnp_GPU,
+// gpu/cuda/nnp.cu:294-316): y = H(G) with the electron temperature T (eV)
+// appended as a last column (tdnp.cpp:83-113 give H's last layer a zero
+// weight column that the temperature then overwrites); u = U(y), s = S(y)
+// (times T for a Sommerfeld model); the atom's energy is u, its free energy
+// a = u - T s, and the gradient that drives the forces is dA/dG: the
+// backward pass of H seeded with dU/dy - T dS/dy (T^2 for Sommerfeld) with
+// the temperature column's gradient zeroed.  Pinned by torch autograd of
+// the composed networks and by finite differences of the free energy.
+void tdnp_atom(const Model &m, int ei, const std::vector<double> &G, double &u, double &s, double &a,
+               std::vector<double> &g)
+{
+  const Dense &H = m.tdH[ei], &U = m.tdU[ei], &S = m.tdS[ei];
+  DenseWork wh, wu, ws;
+  std::vector<double> y(m.ydim);
+  dense_forward(H, G.data(), y.data(), wh);
+  const double T = m.etemp;
+  y[m.ydim - 1] = T;
+  dense_forward(U, y.data(), &u, wu);
+  dense_forward(S, y.data(), &s, ws);
+  if (m.sommerfeld) s *= T;
+  a = u - T * s;
+  std::vector<double> gu(m.ydim), gs(m.ydim);
+  const double one = 1.0;
+  dense_backward(U, wu, &one, gu.data());
+  dense_backward(S, ws, &one, gs.data());
+  const double sc = m.sommerfeld ? T * T : T;
+  std::vector<double> gy(m.ydim - 1);
+  for (int j = 0; j < m.ydim - 1; j++) gy[j] = gu[j] - sc * gs[j];
+  g.assign(H.sizes[0], 0.0);
+  dense_backward(H, wh, gy.data(), g.data());
+}
 
 void atom_chain_fp64(const Model &m, int ei, const std::vector<Nbr> &nb, double scale, AtomOut &out,
                      bool keep_desc)
@@ -452,13 +497,17 @@ void atom_chain_fp64(const Model &m, int ei, const std::vector<Nbr> &nb, double 
       G[(b * k + j) * nm + 0] = Pp[0] < 0 ? -g0 : g0;  // kernel.cu:1048,1058
     }
   // MLP
-  const Dense &net = m.mlp[ei];
-  DenseWork w;
   double E;
-  dense_forward(net, G.data(), &E, w);
   std::vector<double> g(gw);
-  double one = 1.0;
-  dense_backward(net, w, &one, g.data());
+  if (m.tdnp) {
+    tdnp_atom(m, ei, G, E, out.s, out.a, g);
+  } else {
+    const Dense &net = m.mlp[ei];
+    DenseWork w;
+    dense_forward(net, G.data(), &E, w);
+    double one = 1.0;
+    dense_backward(net, w, &one, g.data());
+  }
   out.E = E;
   if (keep_desc) {
     out.G = G;
@@ -706,7 +755,10 @@ void atom_chain_emul(const Model &m, int ei, const std::vector<Nbr> &nb, double 
       G[(b * k + j) * nm + 0] = (double) Pp[0];  // sign(P0) sqrt(P0^2)
     }
   std::vector<double> g;
-  out.E = mlp_emul<RM>(m.mlp[ei], G, g);
+  if (m.tdnp)  // the TDNP networks stay FP64 in the emulation
+    tdnp_atom(m, ei, G, out.E, out.s, out.a, g);
+  else
+    out.E = mlp_emul<RM>(m.mlp[ei], G, g);
   if (keep_desc) {
     out.G = G;
     out.dEdG = g;
@@ -790,6 +842,29 @@ void *tmdo_new() { return new Model(); }
 
 void tmdo_free(void *h) { delete static_cast<Model *>(h); }
 
+// Electron temperature (eV) of a TDNP model (PairTensorMD::etemp).
+int tmdo_set_etemp(void *h, double etemp)
+{
+  static_cast<Model *>(h)->etemp = etemp;
+  return 0;
+}
+
+// TDNP results of the last compute call: per-atom electron entropy and
+// free energy (scaled; zero for atoms without a row) and their sums.
+int tmdo_tdnp_last(void *h, long n, double *satom, double *aatom, double *S, double *A)
+{
+  auto *m = static_cast<Model *>(h);
+  if (!m->tdnp || (long) m->last_s.size() != n) {
+    g_err = "no TDNP results of that size";
+    return -1;
+  }
+  std::copy(m->last_s.begin(), m->last_s.end(), satom);
+  std::copy(m->last_a.begin(), m->last_a.end(), aatom);
+  *S = m->last_S;
+  *A = m->last_A;
+  return 0;
+}
+
 // Selects the reduced-precision emulation (flags as Model::emul; 0 = the
 // FP64 chain).
 int tmdo_set_emulation(void *h, int flags)
@@ -817,7 +892,7 @@ int tmdo_set_doubles(void *h, const char *key, const double *v, long n)
 int tmdo_finalize(void *h, double interp_dx)
 {
   auto *m = static_cast<Model *>(h);
-  auto geti = [&](const char *k, int def, bool req) -> int {
+  auto geti = [&](const std::string &k, int def, bool req) -> int {
     auto it = m->ikeys.find(k);
     if (it == m->ikeys.end() || it->second.empty()) {
       if (req) throw std::string("missing key ") + k;
@@ -840,7 +915,7 @@ int tmdo_finalize(void *h, double interp_dx)
     m->T_sym = geti("is_T_symmetric", 0, false) == 1;
     m->fp32_model = geti("precision", 64, false) == 32;
     m->use_fnn = geti("use_fnn", 0, true) == 1;
-    if (geti("tdnp", 0, false) == 1) throw std::string("tdnp models are not supported");
+    m->tdnp = geti("tdnp", 0, false) == 1;
     m->masses = getd("masses");
     build_monomials(*m);
     // element slots: tensormd.cpp:165-176
@@ -887,36 +962,52 @@ int tmdo_finalize(void *h, double interp_dx)
       m->k = geti("fnn::num_filters", 0, true);
       if (f.sizes.back() != m->k) throw std::string("fnn output width != num_filters");
     }
-    // MLP
-    int nl = geti("nlayers", 0, true);
-    auto ls = m->ikeys.at("layer_sizes");
-    int act = geti("actfn", 0, true);
-    bool outb = geti("apply_output_bias", 0, true) == 1;
-    bool res = geti("use_resnet_dt", 0, true) == 1;
-    int nin = m->nelt * m->k * m->nm;
-    for (int e = 0; e < m->nelt; e++) {
-      Dense d;
-      d.nlayers = nl;
-      d.sizes.assign(1, nin);
-      for (int v : ls) d.sizes.push_back(v);
-      d.act = act;
-      d.outbias = outb;
-      d.resnet = res;
-      int nres = 0;
-      for (int l = 1; l < nl; l++)
-        if (d.sizes[l] == d.sizes[l + 1]) nres++;
-      if (nres == 0) d.resnet = false;
-      for (int l = 0; l < nl; l++) {
-        d.W.push_back(getd("weights_" + std::to_string(e) + "_" + std::to_string(l)));
-        if (l < nl - 1 || outb)
-          d.B.push_back(getd("biases_" + std::to_string(e) + "_" + std::to_string(l)));
-        else
-          d.B.emplace_back();
-        if ((int) d.W[l].size() != d.sizes[l] * d.sizes[l + 1])
-          throw std::string("mlp weight shape");
+    // the per-element networks of one scope ("" for the main MLP, "H::",
+    // "U::", "S::" for TDNP; nnp.cpp:110-205, tdnp.cpp:60-148)
+    auto read_nets = [&](const std::string &scope, int nin, std::vector<Dense> &out) {
+      int nl = geti(scope + "nlayers", 0, true);
+      auto ls = m->ikeys.at(scope + "layer_sizes");
+      int act = geti(scope + "actfn", 0, true);
+      bool outb = geti(scope + "apply_output_bias", 0, true) == 1;
+      bool res = geti(scope + "use_resnet_dt", 0, true) == 1;
+      for (int e = 0; e < m->nelt; e++) {
+        Dense d;
+        d.nlayers = nl;
+        d.sizes.assign(1, nin);
+        for (int v : ls) d.sizes.push_back(v);
+        d.act = act;
+        d.outbias = outb;
+        d.resnet = res;
+        int nres = 0;
+        for (int l = 1; l < nl; l++)
+          if (d.sizes[l] == d.sizes[l + 1]) nres++;
+        if (nres == 0) d.resnet = false;
+        for (int l = 0; l < nl; l++) {
+          d.W.push_back(getd(scope + "weights_" + std::to_string(e) + "_" + std::to_string(l)));
+          if (l < nl - 1 || outb)
+            d.B.push_back(getd(scope + "biases_" + std::to_string(e) + "_" + std::to_string(l)));
+          else
+            d.B.emplace_back();
+          if ((int) d.W[l].size() != d.sizes[l] * d.sizes[l + 1])
+            throw std::string(scope + "weight shape");
+        }
+        out.push_back(d);
       }
-      if (d.sizes.back() != 1) throw std::string("mlp output width must be 1");
-      m->mlp.push_back(d);
+    };
+    int nin = m->nelt * m->k * m->nm;
+    if (m->tdnp) {
+      m->sommerfeld = geti("tdnp::Sommerfeld", 0, false) == 1;
+      read_nets("H::", nin, m->tdH);
+      m->ydim = m->tdH[0].sizes.back() + 1;  // plus the temperature column
+      read_nets("U::", m->ydim, m->tdU);
+      read_nets("S::", m->ydim, m->tdS);
+      for (int e = 0; e < m->nelt; e++)
+        if (m->tdU[e].sizes.back() != 1 || m->tdS[e].sizes.back() != 1)
+          throw std::string("U and S output widths must be 1");
+    } else {
+      read_nets("", nin, m->mlp);
+      for (const Dense &d : m->mlp)
+        if (d.sizes.back() != 1) throw std::string("mlp output width must be 1");
     }
     if (!(interp_dx > 0)) throw std::string("interp dx must be > 0");
     build_table(*m, interp_dx);
@@ -991,6 +1082,10 @@ int tmdo_monomials(void *h, int n, const double *u, double *M, double *dMdu)
 int tmdo_mlp(void *h, int e, int nrows, const double *G, double *E, double *dEdG)
 {
   auto *m = static_cast<Model *>(h);
+  if (m->tdnp) {
+    g_err = "a TDNP model has no single MLP";
+    return -1;
+  }
   int gw = m->nelt * m->k * m->nm;
   std::vector<int> bad(1, 0);
 #pragma omp parallel for schedule(static)
@@ -1047,10 +1142,21 @@ int tmdo_compute_list(void *h, int nlocal, int nall, const double *x, const int 
   }
   // serial, deterministic scatter in list order
   double etot = 0.0;
+  if (m->tdnp) {
+    m->last_s.assign(nlocal, 0.0);
+    m->last_a.assign(nlocal, 0.0);
+    m->last_S = m->last_A = 0.0;
+  }
   for (int ii = 0; ii < inum; ii++) {
     int i = ilist[ii];
     etot += outs[ii].E * scale;
     if (eatom) eatom[i] = outs[ii].E * scale;
+    if (m->tdnp) {
+      m->last_s[i] = outs[ii].s * scale;
+      m->last_a[i] = outs[ii].a * scale;
+      m->last_S += m->last_s[i];
+      m->last_A += m->last_a[i];
+    }
     long c = 0;
     for (long p = neigh_off[ii]; p < neigh_off[ii + 1]; p++, c++) {
       int j = neigh[p] & 0x1FFFFFFF;
@@ -1308,9 +1414,20 @@ int tmdo_compute_periodic(void *h, int n, const double *x, const int *elem, cons
   if (vatom) std::fill(vatom, vatom + 6 * (size_t) n, 0.0);
   double vir[6] = {0, 0, 0, 0, 0, 0};
   double etot = 0.0;
+  if (m->tdnp) {
+    m->last_s.assign(n, 0.0);
+    m->last_a.assign(n, 0.0);
+    m->last_S = m->last_A = 0.0;
+  }
   for (int i = 0; i < n; i++) {
     etot += outs[i].E * scale;
     if (eatom) eatom[i] = outs[i].E * scale;
+    if (m->tdnp) {
+      m->last_s[i] = outs[i].s * scale;
+      m->last_a[i] = outs[i].a * scale;
+      m->last_S += m->last_s[i];
+      m->last_A += m->last_a[i];
+    }
     long c = 0;
     for (long p = off[i]; p < off[i + 1]; p++, c++) {
       const double *F = &outs[i].F[3 * c];

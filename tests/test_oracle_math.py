@@ -396,3 +396,88 @@ def test_high_moment_T_and_contraction(tmp_path, mm):
                     tot += sum(h[k] * np.prod(uu[list(idx)]) for h, uu in terms[b]) ** 2
                 ref.append(tot)
             assert np.allclose(G[b, k], ref, rtol=1e-11, atol=1e-13), (b, k)
+
+
+def _tdnp_model(tmp_path, sommerfeld=False, resnet=False, act="softplus"):
+    return W.write_model(str(tmp_path / f"tdnp_{int(sommerfeld)}_{int(resnet)}_{act}.npz"),
+                         symbols=["Mo", "Ni"], masses=[95.95, 58.69], rmax=5.5, num_filters=16,
+                         layer_sizes=[1], seed=64,
+                         tdnp=dict(H=[32, 32, 8], U=[16, 1], S=[16, 1], sommerfeld=sommerfeld,
+                                   resnet=resnet, actfn=act))
+
+
+@pytest.mark.parametrize("sommerfeld,resnet,act", [(False, False, "softplus"), (True, True, "squareplus"),
+                                                  (False, True, "tanh")])
+def test_tdnp_against_torch(tmp_path, sommerfeld, resnet, act):
+    """TDNP atoms (tdnp.cpp:150-218): the oracle's atomic energy u, entropy
+    s, free energy a = u - T s and the gradient dA/dG that drives the forces
+    equal torch autograd of the composed H, U, S networks with the electron
+    temperature appended to H's output."""
+    torch = pytest.importorskip("torch")
+    p = _tdnp_model(tmp_path, sommerfeld, resnet, act)
+    m = oracle.OracleModel(p, 0.001)
+    T = 0.35  # eV
+    m.set_etemp(T)
+    x, t, cell = W.crystal("bcc", 3.2, (2, 2, 2), nelt=2, sigma=0.1, seed=12)
+    out = m.compute_periodic(x, t - 1, cell, want_desc=True, scale=0.8)
+    tl = m.tdnp_last(len(x))
+    acts = {"softplus": torch.nn.functional.softplus, "tanh": torch.tanh,
+            "squareplus": lambda z: 0.5 * (z + torch.sqrt(z * z + 4.0))}
+    f = acts[act]
+    with np.load(p) as z:
+        def net(scope, e, h):
+            L = int(z[f"{scope}nlayers"])
+            sizes = [h.shape[1]] + list(z[f"{scope}layer_sizes"])
+            for l in range(L):
+                Wl = torch.tensor(z[f"{scope}weights_{e}_{l}"]).reshape(sizes[l], sizes[l + 1])
+                zz = h @ Wl
+                key = f"{scope}biases_{e}_{l}"
+                if key in z.files:
+                    zz = zz + torch.tensor(z[key]).reshape(-1)
+                if l == L - 1:
+                    return zz
+                a = f(zz)
+                h = a + h if (resnet and l > 0 and sizes[l] == sizes[l + 1]) else a
+        G = torch.tensor(out["G"], requires_grad=True)
+        A_tot = 0.0
+        for e in range(2):
+            rows = np.nonzero(t - 1 == e)[0]
+            g = G[rows]
+            y = net("H::", e, g)
+            y = torch.cat([y, torch.full((len(rows), 1), T, dtype=y.dtype)], 1)
+            u = net("U::", e, y).reshape(-1)
+            s = net("S::", e, y).reshape(-1)
+            if sommerfeld:
+                s = s * T
+            a = u - T * s
+            assert np.allclose(out["eatom"][rows], 0.8 * u.detach().numpy(), rtol=1e-12, atol=1e-12)
+            assert np.allclose(tl["satom"][rows], 0.8 * s.detach().numpy(), rtol=1e-12, atol=1e-12)
+            assert np.allclose(tl["aatom"][rows], 0.8 * a.detach().numpy(), rtol=1e-12, atol=1e-12)
+            A_tot = A_tot + a.sum()
+        A_tot.backward()
+    assert np.allclose(out["dEdG"], G.grad.numpy(), rtol=1e-10, atol=1e-12)
+    assert abs(tl["free_energy"] - tl["aatom"].sum()) < 1e-12
+    assert abs(out["energy"] - out["eatom"].sum()) < 1e-12
+
+
+def test_tdnp_forces_are_minus_free_energy_gradient(tmp_path):
+    """With an electron temperature the forces are -dA/dx (A = U - T S),
+    while the reported energy is U (tensormd.cpp:471-485)."""
+    p = _tdnp_model(tmp_path, sommerfeld=True)
+    m = oracle.OracleModel(p, 0.001).set_etemp(0.5)
+    x, t, cell = W.crystal("bcc", 3.2, (2, 2, 2), nelt=2, sigma=0.1, seed=13)
+    out = m.compute_periodic(x, t - 1, cell)
+    U0 = out["energy"]
+    A0 = m.tdnp_last(len(x))["free_energy"]
+    assert abs(A0 - U0) > 1e-4  # the entropy term matters
+    h = 1e-5
+    for i in (0, 7):
+        for a in range(3):
+            xp = x.copy(); xp[i, a] += h
+            xm = x.copy(); xm[i, a] -= h
+            m.compute_periodic(xp, t - 1, cell)
+            Ap = m.tdnp_last(len(x))["free_energy"]
+            m.compute_periodic(xm, t - 1, cell)
+            Am = m.tdnp_last(len(x))["free_energy"]
+            fd = -(Ap - Am) / (2 * h)
+            assert abs(fd - out["f"][i, a]) < 2e-8 + 1e-6 * abs(fd), (i, a)

@@ -22,6 +22,10 @@
  *   tmd_compute_list_serial  the same, with the list kept on the device
  *                       between LAMMPS neighbor builds (the reference uploads
  *                       it every step, gpu/cuda/kernel.cu:1500-1521)
+ *   tmd_set_electron_temperature, tmd_tdnp_results  the TDNP state of
+ *                       PairTensorMD (etemp, eentropy, free_energy, satom,
+ *                       aatom; pair_tensormd.cpp:123-137, :171-178) and
+ *                       TDNP::forward/backward (src/ML-TENSORMD/tdnp.cpp:150-218)
  *   tmd_descriptors_list ComputeTensorMDDensity::compute_peratom
  *                       (src/ML-TENSORMD/compute_tensormd_density.cpp:129-205)
  *   tmd_system_*,       the LAMMPS step loop around the pair style:
@@ -139,6 +143,23 @@ int tmd_compute_list_noalloc(tmd_ctx *ctx, int nlocal, int nall, const double *x
                              const int *type, int inum, const int *ilist, const int *numneigh,
                              const int *const *firstneigh, double scale, double *f,
                              double *eatom, double *vatom, double *energy, double *virial);
+
+/* ---- TDNP: temperature-dependent models (tdnp = 1) ----------------------------
+ * The electron temperature in eV (PairTensorMD::etemp: the pair_coeff
+ * keyword "etemp T" gives T / 11604.522060401008 eV, pair_tensormd.cpp:33,
+ * 245-247; fix tensormd/etemp resets it every N steps,
+ * fix_tensormd_etemp.cpp:80-88).  0 until set.  A TDNP model's atomic energy
+ * is the electron internal energy u and its forces are -dA/dx with the free
+ * energy A = U - T S (tdnp.cpp:150-218).  Accepted for any model; only TDNP
+ * models read it. */
+int tmd_set_electron_temperature(tmd_ctx *ctx, double etemp_ev);
+/* The electron entropy S and free energy A (scaled sums; PairTensorMD::
+ * eentropy and free_energy, read by compute tensormd/tdpe) and, where
+ * non-NULL, the per-atom satom/aatom[nlocal] (zero for atoms not in ilist)
+ * of the context's last tmd_compute_list or tmd_compute_list_serial call.
+ * TMD_ERR_MODEL for a model without TDNP, TMD_ERR_STATE before such a call. */
+int tmd_tdnp_results(const tmd_ctx *ctx, double *entropy, double *free_energy, double *satom,
+                     double *aatom);
 
 /* `compute tensormd/density`: the descriptor row of every atom in ilist,
  * G[nlocal][width] in the model file's width and [b][k][m] layout (rows of

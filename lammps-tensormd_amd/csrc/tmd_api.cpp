@@ -99,6 +99,78 @@ static uint16_t bf16_bits(float f)
   return (uint16_t) (u >> 16);
 }
 
+// TDNP networks H, U, S: per layer, all elements packed, W [K][N], its
+// transpose and the biases in FP32; the electron temperature as a device
+// scalar.
+int upload_tdnp(tmd_ctx *ctx)
+{
+  const Model &m = ctx->model;
+  const std::vector<DenseNet> *nets[3] = {&m.tdH, &m.tdU, &m.tdS};
+  size_t total = 0;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t) 255; };
+  size_t off[3][kMaxLayers][3];
+  for (int q = 0; q < 3; q++) {
+    const DenseNet &n0 = (*nets[q])[0];
+    if (n0.nlayers > kMaxLayers) return (set_err(ctx, "TDNP network with more than 8 layers"), TMD_ERR_MODEL);
+    for (int l = 0; l < n0.nlayers; l++) {
+      const size_t per = (size_t) n0.sizes[l] * n0.sizes[l + 1];
+      off[q][l][0] = total;
+      total = al(total + per * m.nelt * 4);
+      off[q][l][1] = total;
+      total = al(total + per * m.nelt * 4);
+      off[q][l][2] = total;
+      total = al(total + (size_t) n0.sizes[l + 1] * m.nelt * 4);
+    }
+  }
+  hipError_t e = ctx->tdnp_params.ensure(total);
+  if (e != hipSuccess) return hip_fail(ctx, e, "TDNP parameter alloc");
+  std::vector<unsigned char> host(total, 0);
+  unsigned char *base = (unsigned char *) ctx->tdnp_params.p;
+  TdnpParams &tp = ctx->tdnp;
+  std::memset(&tp, 0, sizeof(tp));
+  for (int q = 0; q < 3; q++) {
+    const DenseNet &n0 = (*nets[q])[0];
+    TdnpNet &tn = tp.net[q];
+    tn.nl = n0.nlayers;
+    tn.act = n0.act;
+    for (int l = 0; l <= n0.nlayers; l++) tn.sizes[l] = n0.sizes[l];
+    for (int l = 0; l < n0.nlayers; l++) {
+      const int K = n0.sizes[l], N = n0.sizes[l + 1];
+      const size_t per = (size_t) K * N;
+      float *W = (float *) &host[off[q][l][0]];
+      float *Wt = (float *) &host[off[q][l][1]];
+      float *b = (float *) &host[off[q][l][2]];
+      bool hasb = false;
+      for (int el = 0; el < m.nelt; el++) {
+        const DenseNet &net = (*nets[q])[el];
+        for (int i = 0; i < K; i++)
+          for (int j = 0; j < N; j++) {
+            const float w = (float) net.W[l][(size_t) i * N + j];
+            W[el * per + (size_t) i * N + j] = w;
+            Wt[el * per + (size_t) j * K + i] = w;
+          }
+        if (!net.B[l].empty()) {
+          hasb = true;
+          for (int j = 0; j < N; j++) b[(size_t) el * N + j] = (float) net.B[l][j];
+        }
+      }
+      tn.W[l] = (const float *) (base + off[q][l][0]);
+      tn.Wt[l] = (const float *) (base + off[q][l][1]);
+      tn.b[l] = hasb ? (const float *) (base + off[q][l][2]) : nullptr;
+      tn.res[l] = n0.res_layer(l) ? 1 : 0;
+    }
+  }
+  tp.ydim = m.ydim;
+  tp.sommerfeld = m.sommerfeld ? 1 : 0;
+  if ((e = hipMemcpy(ctx->tdnp_params.p, host.data(), total, hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(ctx, e, "TDNP parameter upload");
+  if ((e = ctx->etemp_dev.ensure(sizeof(double))) != hipSuccess) return hip_fail(ctx, e, "etemp alloc");
+  if ((e = hipMemcpy(ctx->etemp_dev.p, &ctx->etemp, sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(ctx, e, "etemp upload");
+  tp.etemp = (const double *) ctx->etemp_dev.p;
+  return TMD_OK;
+}
+
 // Upload table and MLP parameters.
 int upload_model(tmd_ctx *ctx)
 {
@@ -157,6 +229,7 @@ int upload_model(tmd_ctx *ctx)
   e = hipMemcpy(ctx->tab.p, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(ctx, e, "table upload");
 
+  if (m.tdnp) return upload_tdnp(ctx);
   // MLP: per layer, all elements packed
   const DenseNet &n0 = m.mlp[0];
   const int L = n0.nlayers;
@@ -330,6 +403,7 @@ int ensure_rows(tmd_ctx *ctx, Workspace &w, int nrows)
   if ((e = w.E.ensure((size_t) nrows * 8)) != hipSuccess) return hip_fail(ctx, e, "E alloc");
   if ((e = w.row2atom.ensure((size_t) nrows * 4)) != hipSuccess) return hip_fail(ctx, e, "row alloc");
   if ((e = w.red.ensure(4096 * 8)) != hipSuccess) return hip_fail(ctx, e, "red alloc");
+  if (m.tdnp && (e = w.sa.ensure((size_t) nrows * 16)) != hipSuccess) return hip_fail(ctx, e, "TDNP alloc");
   return TMD_OK;
 }
 
@@ -362,9 +436,17 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
                         (const float4 *) ctx->tab.p, (float *) w.G.p, kl, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
   if (pe && (e = hipEventRecord(pe->ev[1], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
-  e = launch_mlp(mode, w.rl, ctx->mlp, (const float *) w.G.p, (double *) w.E.p, (float *) w.dEdG.p, s,
-                 c.rtile, c.dyn);
-  if (e != hipSuccess) return hip_fail(ctx, e, "mlp launch");
+  if (ctx->model.tdnp) {
+    if (w.sa.bytes < (size_t) w.rl.nrows * 16)
+      return (set_err(ctx, "internal: TDNP buffer not allocated for these rows"), TMD_ERR_STATE);
+    e = launch_tdnp(w.rl, ctx->tdnp, row2atom, (const float *) w.G.p, (double *) w.E.p, (double *) w.sa.p,
+                    (float *) w.dEdG.p, s, c.dyn);
+    if (e != hipSuccess) return hip_fail(ctx, e, "TDNP launch");
+  } else {
+    e = launch_mlp(mode, w.rl, ctx->mlp, (const float *) w.G.p, (double *) w.E.p, (float *) w.dEdG.p, s,
+                   c.rtile, c.dyn);
+    if (e != hipSuccess) return hip_fail(ctx, e, "mlp launch");
+  }
   if (pe && (e = hipEventRecord(pe->ev[2], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
   if (c.dyn)
     e = launch_zero_atoms(c.f, 3, (int) (c.fbytes / (3 * sizeof(double))), c.dyn, s);
@@ -458,6 +540,10 @@ int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes
     set_err(nullptr, err);
     return TMD_ERR_MODEL;
   }
+  if (ctx->model.tdnp && ctx->opt.mlp_precision != TMD_MLP_FP32) {
+    set_err(nullptr, "TDNP models run the FP32 path only (mlp_precision must be TMD_MLP_FP32)");
+    return TMD_ERR_MODEL;
+  }
   // PairTensorMD::setup_eltmap (pair_tensormd.cpp:349-370)
   ctx->ntypes = ntypes;
   ctx->type_map.assign(ntypes + 1, -1);
@@ -543,7 +629,7 @@ int tmd_model_info(const tmd_ctx *ctx, int *info)
   info[1] = ctx->model.k;
   info[2] = ctx->model.max_moment_file;
   info[3] = ctx->model.gwidth_file();
-  info[4] = ctx->model.mlp[0].nlayers;
+  info[4] = ctx->model.tdnp ? ctx->model.tdH[0].nlayers : ctx->model.mlp[0].nlayers;
   return TMD_OK;
 }
 
@@ -617,6 +703,35 @@ int tmd_descriptors_list(tmd_ctx *ctx, int nlocal, int nall, const double *x, co
   a.Gout = G;
   a.who = "tmd_descriptors_list";
   return list_compute(ctx, a, ListResources(), -1);
+}
+
+int tmd_set_electron_temperature(tmd_ctx *ctx, double etemp_ev)
+{
+  if (!ctx || !(etemp_ev >= 0.0)) {
+    set_err(ctx, "tmd_set_electron_temperature: invalid arguments");
+    return TMD_ERR_ARG;
+  }
+  ctx->etemp = etemp_ev;
+  if (!ctx->model.tdnp) return TMD_OK;  // only TDNP models read it
+  if (hipError_t de = hipSetDevice(ctx->device); de != hipSuccess) return hip_fail(ctx, de, "hipSetDevice");
+  // stream-ordered: evaluations already queued keep the temperature they
+  // were queued with
+  hipError_t e = hipMemcpyAsync(ctx->etemp_dev.p, &ctx->etemp, sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? TMD_OK : hip_fail(ctx, e, "etemp upload");
+}
+
+int tmd_tdnp_results(const tmd_ctx *ctx, double *entropy, double *free_energy, double *satom, double *aatom)
+{
+  if (!ctx) return TMD_ERR_ARG;
+  if (!ctx->model.tdnp) return TMD_ERR_MODEL;
+  if (!ctx->td_valid) return TMD_ERR_STATE;
+  if (entropy) *entropy = ctx->td_entropy;
+  if (free_energy) *free_energy = ctx->td_free_energy;
+  if (satom) std::copy(ctx->td_satom.begin(), ctx->td_satom.end(), satom);
+  if (aatom) std::copy(ctx->td_aatom.begin(), ctx->td_aatom.end(), aatom);
+  return TMD_OK;
 }
 
 int tmd_device_memory(size_t *free_bytes, size_t *total_bytes)

@@ -190,6 +190,32 @@ bool mlp_fast_ok(const MlpParams &mp, int mode);
 hipError_t launch_quant_fp8(const float *W, const float *swf, const float *swb, uint8_t *Af,
                             uint8_t *Ab, int K, int N, int nelt, hipStream_t stream);
 
+// ---- TDNP: temperature-dependent models (src/ML-TENSORMD/tdnp.cpp) ------------------
+// One network of every element: layer l of element e is W[l] + e K N
+// ([K][N]), Wt[l] + e N K ([N][K]) and b[l] + e N (null without bias).
+struct TdnpNet {
+  int nl;
+  int act;
+  int sizes[kMaxLayers + 1];
+  int res[kMaxLayers];
+  const float *W[kMaxLayers];
+  const float *Wt[kMaxLayers];
+  const float *b[kMaxLayers];
+};
+// net[0] = H (descriptor -> ydim - 1, linear last layer), net[1] = U and
+// net[2] = S (ydim -> 1); etemp: device scalar (eV), read at launch time.
+struct TdnpParams {
+  TdnpNet net[3];
+  int ydim;
+  int sommerfeld;
+  const double *etemp;
+};
+// Per row: E[row] = u (FP64), sa[row] = {s, a} (FP64, a = u - T s), and
+// dEdG[row] = dA/dG.  Padding rows get zeros.
+hipError_t launch_tdnp(const RowLayout &rl, const TdnpParams &tp, const int *row2atom, const float *G,
+                       double *E, double *sa, float *dEdG, hipStream_t stream,
+                       const BatchDims *dyn = nullptr);
+
 // ---- batches of structures (serving) ---------------------------------------------
 // Capacities of one batch slot (grids of its captured graph).
 struct BatchCaps {

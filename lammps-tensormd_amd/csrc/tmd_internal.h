@@ -72,6 +72,7 @@ struct Workspace {
   DeviceBuffer pos, nbr, nn, f, vatom, ew;
   DeviceBuffer G, dEdG, E, row2atom, red, cells;
   DeviceBuffer kl;
+  DeviceBuffer sa;  // TDNP models: per row electron entropy and free energy (FP64 pairs)
 };
 
 struct Server;
@@ -90,6 +91,15 @@ struct tmd_ctx {
   tmd::ModelGeo geo{};
   tmd::DeviceBuffer tab, params;
   tmd::MlpParams mlp{};
+  // TDNP models: network parameters, the electron temperature (eV; a device
+  // copy that launches read, so captured graphs see updates) and the
+  // entropy and free energy of the last list-mode evaluation (scaled)
+  tmd::DeviceBuffer tdnp_params, etemp_dev;
+  tmd::TdnpParams tdnp{};
+  double etemp = 0.0;
+  std::vector<double> td_satom, td_aatom;
+  double td_entropy = 0.0, td_free_energy = 0.0;
+  bool td_valid = false;
   tmd::Workspace list_ws;
   // list mode: pinned host staging, and what list_ws holds (the serial of
   // the last tmd_compute_list_serial call and the list's shape)

@@ -374,8 +374,33 @@ int list_compute(tmd_ctx *ctx, const ListArgs &a, const ListResources &r, long l
       (a.eatom && (e = hipMemcpyAsync(a.eatom, w.ew.p, sizeof(double) * (size_t) a.nlocal, hipMemcpyDeviceToHost,
                                       s)) != hipSuccess))
     return hip_fail(ctx, e, "result download");
+  // TDNP: the per-row electron entropy and free energy (the context's
+  // allocating paths keep them for tmd_tdnp_results)
+  std::vector<double> sa;
+  const bool td = model.tdnp && !r.fixed;
+  if (td) {
+    sa.resize(2 * (size_t) w.rl.nrows);
+    if ((e = hipMemcpyAsync(sa.data(), w.sa.p, 8 * sa.size(), hipMemcpyDeviceToHost, s)) != hipSuccess)
+      return hip_fail(ctx, e, "TDNP download");
+  }
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(ctx, e, who);
   drain.done = true;
+  if (td) {
+    ctx->td_satom.assign(a.nlocal, 0.0);
+    ctx->td_aatom.assign(a.nlocal, 0.0);
+    double S = 0.0, A = 0.0;
+    for (int rr = 0; rr < w.rl.nrows; rr++) {  // atoms in row order (fixed)
+      const int at = hs.row2atom[rr];
+      if (at < 0) continue;
+      ctx->td_satom[at] = a.scale * sa[2 * (size_t) rr];
+      ctx->td_aatom[at] = a.scale * sa[2 * (size_t) rr + 1];
+      S += ctx->td_satom[at];
+      A += ctx->td_aatom[at];
+    }
+    ctx->td_entropy = S;
+    ctx->td_free_energy = A;
+    ctx->td_valid = true;
+  }
   *a.energy = red[0];
   if (a.virial)
     for (int q = 0; q < 6; q++) a.virial[q] = red[1 + q];

@@ -130,7 +130,7 @@ void build_T(Model &m)
 void pad_moments(Model &m)
 {
   const int nm0 = m.max_moment + 1, rows = m.nelt * m.k;
-  for (DenseNet &d : m.mlp) {
+  for (DenseNet &d : m.tdnp ? m.tdH : m.mlp) {
     const int N = d.sizes[1];
     std::vector<double> W((size_t) rows * 4 * N, 0.0);
     for (int r = 0; r < rows; r++)
@@ -226,7 +226,7 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
     m.fctype = geti("fctype", 0, true);
     m.T_sym = geti("is_T_symmetric", 0, false) == 1;
     m.fp32_model = geti("precision", 64, false) == 32;
-    if (geti("tdnp", 0, false) == 1) throw std::string("TDNP models are not supported");
+    m.tdnp = geti("tdnp", 0, false) == 1;
     m.use_fnn = geti("use_fnn", 0, true) == 1;
     m.masses = getd("masses");
     if ((int) m.masses.size() != m.nelt) throw std::string("masses size != nelt");
@@ -294,37 +294,56 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
     if (!filters_supported(m.k))
       throw std::string("num_filters " + std::to_string(m.k) +
                         " is not supported: the descriptor kernels take 16, 24, 32, 64 or 128");
-    // main networks
-    int nl = geti("nlayers", 0, true);
-    std::vector<int> ls = z.at("layer_sizes").as_ints();
-    if ((int) ls.size() != nl || nl < 2) throw std::string("layer_sizes length");
-    int act = geti("actfn", 0, true);
-    if (act < 0 || act > 3) throw std::string("actfn must be 0..3");
-    bool outb = geti("apply_output_bias", 0, true) == 1;
-    bool res = geti("use_resnet_dt", 0, true) == 1;
-    int nres = 0;
-    for (int l = 1; l < nl; l++)
-      if (ls[l - 1] == ls[l]) nres++;
-    if (nres == 0) res = false;  // nnp.cpp:179-181
-    if (ls.back() != 1) throw std::string("the last layer must have width 1");
-    for (int e = 0; e < m.nelt; e++) {
-      DenseNet d;
-      d.nlayers = nl;
-      d.sizes.assign(1, m.gwidth());
-      for (int v : ls) d.sizes.push_back(v);
-      d.act = act;
-      d.outbias = outb;
-      d.resnet = res;
-      for (int l = 0; l < nl; l++) {
-        d.W.push_back(getd("weights_" + std::to_string(e) + "_" + std::to_string(l)));
-        if ((int) d.W[l].size() != d.sizes[l] * d.sizes[l + 1])
-          throw std::string("weights_" + std::to_string(e) + "_" + std::to_string(l) + " shape");
-        if (l < nl - 1 || outb)
-          d.B.push_back(getd("biases_" + std::to_string(e) + "_" + std::to_string(l)));
-        else
-          d.B.emplace_back();
+    // per-element networks of one scope: "" for the main MLP
+    // (NeuralNetworkPotential::setup_global, nnp.cpp:110-205); "H::", "U::",
+    // "S::" for TDNP (TDNP::setup_one, tdnp.cpp:60-148)
+    auto read_nets = [&](const std::string &scope, int nin, std::vector<DenseNet> &out) {
+      int nl = geti(scope + "nlayers", 0, true);
+      std::vector<int> ls = z.at(scope + "layer_sizes").as_ints();
+      if ((int) ls.size() != nl || nl < 1) throw std::string(scope + "layer_sizes length");
+      int act = geti(scope + "actfn", 0, true);
+      if (act < 0 || act > 3) throw std::string(scope + "actfn must be 0..3");
+      bool outb = geti(scope + "apply_output_bias", 0, true) == 1;
+      bool res = geti(scope + "use_resnet_dt", 0, true) == 1;
+      int nres = 0;
+      for (int l = 1; l < nl; l++)
+        if (ls[l - 1] == ls[l]) nres++;
+      if (nres == 0) res = false;  // nnp.cpp:179-181
+      for (int e = 0; e < m.nelt; e++) {
+        DenseNet d;
+        d.nlayers = nl;
+        d.sizes.assign(1, nin);
+        for (int v : ls) d.sizes.push_back(v);
+        d.act = act;
+        d.outbias = outb;
+        d.resnet = res;
+        for (int l = 0; l < nl; l++) {
+          const std::string wk = scope + "weights_" + std::to_string(e) + "_" + std::to_string(l);
+          d.W.push_back(getd(wk));
+          if ((int) d.W[l].size() != d.sizes[l] * d.sizes[l + 1]) throw std::string(wk + " shape");
+          if (l < nl - 1 || outb) {
+            d.B.push_back(getd(scope + "biases_" + std::to_string(e) + "_" + std::to_string(l)));
+            if ((int) d.B[l].size() < d.sizes[l + 1]) throw std::string(scope + "bias shape");
+          } else {
+            d.B.emplace_back();
+          }
+        }
+        out.push_back(std::move(d));
       }
-      m.mlp.push_back(std::move(d));
+    };
+    if (m.tdnp) {
+      m.sommerfeld = geti("tdnp::Sommerfeld", 0, false) == 1;
+      read_nets("H::", m.gwidth(), m.tdH);
+      m.ydim = m.tdH[0].sizes.back() + 1;  // plus the electron-temperature column
+      read_nets("U::", m.ydim, m.tdU);
+      read_nets("S::", m.ydim, m.tdS);
+      for (int e = 0; e < m.nelt; e++)
+        if (m.tdU[e].sizes.back() != 1 || m.tdS[e].sizes.back() != 1)
+          throw std::string("TDNP U and S networks must end in width 1");
+    } else {
+      read_nets("", m.gwidth(), m.mlp);
+      if (m.mlp[0].nlayers < 2) throw std::string("layer_sizes length");
+      if (m.mlp[0].sizes.back() != 1) throw std::string("the last layer must have width 1");
     }
     if (m.max_moment < 3) pad_moments(m);
     if (!(dx > 0)) throw std::string("interp dx must be > 0");

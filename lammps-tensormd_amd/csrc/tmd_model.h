@@ -52,6 +52,13 @@ struct Model {
   int radial_fn = RADIAL_PEXP;             // analytic kind (use_fnn false)
   std::vector<std::vector<double>> rparams;  // [parameter][filter]
   std::vector<DenseNet> mlp;  // per element
+  // TDNP models (tdnp = 1): per element H (descriptor -> ydim - 1 outputs,
+  // linear last layer), U and S (ydim -> 1) instead of mlp; the last of the
+  // ydim inputs of U and S is the electron temperature
+  bool tdnp = false;
+  bool sommerfeld = false;
+  int ydim = 0;
+  std::vector<DenseNet> tdH, tdU, tdS;
   // Kerley table, [n][k][4] = {y, s, c1, c2}
   int tab_n = 0;
   double dx = 0, xdx = 0;

@@ -92,6 +92,8 @@ def lib():
     L.tmd_compute_list_noalloc.argtypes = L.tmd_compute_list.argtypes
     L.tmd_descriptors_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int, _ip,
                                        _ip, ctypes.POINTER(_ip), _dp]
+    L.tmd_set_electron_temperature.argtypes = [_vp, ctypes.c_double]
+    L.tmd_tdnp_results.argtypes = [_vp, _dp, _dp, _dp, _dp]
     L.tmd_system_create.argtypes = [_vp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_double,
                                     ctypes.POINTER(_vp)]
     L.tmd_system_destroy.argtypes = [_vp]
@@ -264,6 +266,21 @@ class TensorMD:
         _check(rc, self.h)
         out["energy"] = e.value
         return out
+
+    def set_electron_temperature(self, etemp_ev):
+        """Electron temperature (eV) read by TDNP models."""
+        _check(lib().tmd_set_electron_temperature(self.h, float(etemp_ev)), self.h)
+
+    def tdnp_results(self, nlocal):
+        """Electron entropy and free energy (scaled sums) and their per-atom
+        values of the last compute_list call of a TDNP model."""
+        S = ctypes.c_double()
+        A = ctypes.c_double()
+        sa = np.zeros(nlocal)
+        aa = np.zeros(nlocal)
+        _check(lib().tmd_tdnp_results(self.h, ctypes.byref(S), ctypes.byref(A), _p(sa, _dp), _p(aa, _dp)),
+               self.h)
+        return dict(entropy=S.value, free_energy=A.value, satom=sa, aatom=aa)
 
     def descriptors_list(self, nlocal, x, types, ilist, numneigh, neigh_flat):
         """compute tensormd/density: descriptor rows [nlocal, width] (zero for

@@ -158,3 +158,10 @@ def test_list_entry_points_reject_null_context():
     assert L.tmd_compute_list(None, *args) == -1
     assert L.tmd_compute_list_serial(None, 0, *args) == -1
     assert L.tmd_compute_list_noalloc(None, *args) == -1
+
+
+def test_tdnp_entry_points_reject_null_context():
+    L = T.lib()
+    d = ctypes.c_double()
+    assert L.tmd_set_electron_temperature(None, 0.1) == -1
+    assert L.tmd_tdnp_results(None, ctypes.byref(d), ctypes.byref(d), None, None) == -1

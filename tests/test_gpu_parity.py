@@ -541,6 +541,52 @@ def test_high_moment_models(tmp_path, mm, k, nelt):
     assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
 
 
+@pytest.mark.parametrize("sommerfeld,resnet", [(False, False), (True, True)])
+def test_tdnp_models(tmp_path, sommerfeld, resnet):
+    """TDNP models (tdnp.cpp:150-218): the atomic energies are the electron
+    internal energy u, the forces -dA/dx with A = U - T S, and the entropy
+    and free energy (sums and per atom) are returned beside them.  List mode
+    and the device-resident system against the FP64 oracle, at two electron
+    temperatures set on the same context."""
+    say(f"case: TDNP sommerfeld={sommerfeld} resnet={resnet}, 2 elements, bcc 4x4x4")
+    p = W.write_model(str(tmp_path / "tdnp.npz"), symbols=["Mo", "Ni"], masses=[95.95, 58.69], rmax=5.5,
+                      num_filters=16, layer_sizes=[1], seed=90 + int(sommerfeld),
+                      tdnp=dict(H=[64, 32, 16], U=[32, 1], S=[32, 1], sommerfeld=sommerfeld,
+                                resnet=resnet, actfn="softplus"))
+    om = oracle.OracleModel(p, 0.001)
+    x, t, cell = W.crystal("bcc", 3.2, (4, 4, 4), nelt=2, sigma=0.08, seed=91)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, om.rc + 1.0)
+    tall = t[owner]
+    n = len(x)
+    m = TensorMD(p, ["Mo", "Ni"], "fp32", 0.001)
+    sysm = m.system(x, t, cell, skin=1.0)
+    for T in (0.25, 0.6):
+        om.set_etemp(T)
+        m.set_electron_temperature(T)
+        ref = om.compute_list(n, xall, tall - 1, ilist, numneigh, flat, scale=0.9)
+        rtd = om.tdnp_last(n)
+        got = m.compute_list(n, xall, tall, ilist, numneigh, flat, scale=0.9)
+        gtd = m.tdnp_results(n)
+        e = _errs(got, ref, n)
+        dS = abs(gtd["entropy"] - rtd["entropy"]) / n
+        dA = abs(gtd["free_energy"] - rtd["free_energy"]) / n
+        say("T", T, "list errors", e, "dS/atom", dS, "dA/atom", dA,
+            "A-U per atom", (rtd["free_energy"] - ref["energy"]) / n)
+        assert abs(rtd["free_energy"] - ref["energy"]) / n > 1e-4  # the entropy term matters
+        assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
+        assert dS <= E_TOL and dA <= E_TOL
+        assert abs(gtd["satom"] - rtd["satom"]).max() <= 1e-6
+        assert abs(gtd["aatom"] - rtd["aatom"]).max() <= 1e-6
+        assert abs(got["eatom"] - ref["eatom"]).max() <= 1e-6
+        # the device-resident system reads the same temperature
+        refp = om.compute_periodic(x, t - 1, cell)
+        sysm.set_positions(x)
+        gp = sysm.compute()
+        ep = _errs(gp, refp, n)
+        say("T", T, "system errors", ep)
+        assert ep["dF"] <= F_TOL and ep["dE"] <= E_TOL
+
+
 @pytest.mark.parametrize("k,nelt", [(64, 5), (128, 3), (128, 5), (32, 8)])
 def test_wide_slot_models(tmp_path, k, nelt):
     """Models whose slots take several register passes in the descriptor and

@@ -36,17 +36,38 @@ PairTensorMDAMD::PairTensorMDAMD(LAMMPS *lmp) : Pair(lmp)
   ctx = nullptr;
   cutmax = 0.0;
   etemp = 0.0;
+  etemp_sent = 0.0;
+  tdpe_flag = 0;
+  eentropy = free_energy = 0.0;
+  satom = aatom = nullptr;
+  maxsatom = 0;
   scale = nullptr;
 }
 
 PairTensorMDAMD::~PairTensorMDAMD()
 {
   if (ctx) tmd_destroy(ctx);
+  memory->destroy(satom);
+  memory->destroy(aatom);
   if (allocated) {
     memory->destroy(setflag);
     memory->destroy(cutsq);
     memory->destroy(scale);
     delete[] map;
+  }
+}
+
+/* pair_tensormd.cpp:123-137: per-atom electron entropy and free energy
+   arrays beside eatom when the etemp keyword was given */
+void PairTensorMDAMD::ev_setup(int eflag, int vflag, int alloc)
+{
+  Pair::ev_setup(eflag, vflag, alloc);
+  if (tdpe_flag && alloc && maxsatom < maxeatom) {
+    memory->destroy(satom);
+    memory->destroy(aatom);
+    maxsatom = maxeatom;
+    memory->create(satom, maxsatom, "pair:tensormd:tdnp:satom");
+    memory->create(aatom, maxsatom, "pair:tensormd:tdnp:aatom");
   }
 }
 
@@ -64,6 +85,11 @@ void PairTensorMDAMD::fail(int code, const char *where)
 void PairTensorMDAMD::compute(int eflag, int vflag)
 {
   ev_init(eflag, vflag);
+  if (etemp != etemp_sent) {  // fix tensormd/etemp/amd or extract("etemp") changed it
+    int trc = tmd_set_electron_temperature(ctx, etemp);
+    if (trc != TMD_OK) fail(trc, "pair tensormd/amd electron temperature");
+    etemp_sent = etemp;
+  }
   const int nlocal = atom->nlocal;
   const int nall = nlocal + atom->nghost;
   double energy = 0.0;
@@ -82,6 +108,18 @@ void PairTensorMDAMD::compute(int eflag, int vflag)
                           &energy, nullptr);
   if (rc != TMD_OK) fail(rc, "pair tensormd/amd compute");
   if (eflag) eng_vdwl = energy;
+  // TDNP (tensormd.cpp:471-485): the energy is the electron internal energy;
+  // entropy and free energy beside it (zero for a model without TDNP)
+  if (tdpe_flag) {
+    const bool per_atom = eflag_atom && satom && aatom;
+    rc = tmd_tdnp_results(ctx, &eentropy, &free_energy, per_atom ? satom : nullptr,
+                          per_atom ? aatom : nullptr);
+    if (rc == TMD_ERR_MODEL) {
+      eentropy = free_energy = 0.0;
+    } else if (rc != TMD_OK) {
+      fail(rc, "pair tensormd/amd TDNP results");
+    }
+  }
   if (vflag_fdotr) virial_fdotr_compute();
 }
 
@@ -128,6 +166,7 @@ void PairTensorMDAMD::coeff(int narg, char **arg)
       read_elt = false;
     } else if (key == "etemp") {
       etemp = utils::numeric(FLERR, value(), false, lmp) / 11604.522060401008;  // K -> eV
+      tdpe_flag = 1;
       i += 2;
       read_elt = false;
     } else if (key == "cmax") {
@@ -166,6 +205,8 @@ void PairTensorMDAMD::coeff(int narg, char **arg)
     error->all(FLERR, "pair_coeff: cannot load {}: {}", file, tmd_last_error(nullptr));
   }
   if ((rc = tmd_cutoff(ctx, &cutmax)) != TMD_OK) fail(rc, "pair_coeff");
+  if ((rc = tmd_set_electron_temperature(ctx, etemp)) != TMD_OK) fail(rc, "pair_coeff");
+  etemp_sent = etemp;
   std::vector<int> tmap(n + 1, -1);
   std::vector<double> mass(n + 1, 0.0);
   if ((rc = tmd_type_map(ctx, tmap.data())) != TMD_OK) fail(rc, "pair_coeff");

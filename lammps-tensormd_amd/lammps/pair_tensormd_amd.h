@@ -40,7 +40,16 @@ class PairTensorMDAMD : public Pair {
   void *extract(const char *, int &) override;
   double memory_usage() override;
 
-  double etemp;  // accepted for input compatibility; TDNP models are rejected
+  void ev_setup(int, int, int alloc = 1) override;
+
+  // TDNP state, as PairTensorMD (pair_tensormd.h): electron temperature in
+  // eV (pair_coeff "etemp T", fix tensormd/etemp/amd), tdpe_flag set by the
+  // etemp keyword, and the electron entropy and free energy of the last
+  // compute (sums and, with per-atom energies, per atom)
+  double etemp;
+  int tdpe_flag;
+  double eentropy, free_energy;
+  double *satom, *aatom;
   tmd_ctx *context() const { return ctx; }
   // the pair_coeff arguments after "* *" (model file, symbols, keywords),
   // for compute tensormd/density/amd, which loads the model itself
@@ -51,6 +60,8 @@ class PairTensorMDAMD : public Pair {
   double cutmax;
   double **scale;  // fix adapt scaling (one factor for all types)
   std::vector<std::string> kwargs;
+  double etemp_sent;  // the temperature the context holds
+  int maxsatom;
 
   void allocate();
   void fail(int code, const char *where);

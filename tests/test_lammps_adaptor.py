@@ -16,6 +16,7 @@ LMP_SRC = "/root/reference/src"
 @pytest.mark.skipif(not os.path.exists(os.path.join(LMP_SRC, "pair.h")) or
                     shutil.which("g++") is None, reason="LAMMPS headers not available")
 @pytest.mark.parametrize("src", ["pair_tensormd_amd.cpp", "compute_tensormd_density_amd.cpp",
+                                 "compute_tensormd_tdpe_amd.cpp", "fix_tensormd_etemp_amd.cpp",
                                  "tensormd_amd_plugin.cpp"])
 def test_adaptor_compiles(src):
     here = os.path.join(ROOT, "lammps-tensormd_amd", "lammps")

@@ -313,11 +313,11 @@ __device__ inline void write_G(const ModelGeo &g, float (*acc)[Tiling<K>::KT][Ti
 template <int K, int NS, bool MP>
 __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const double4 *pos,
                                                    const uint32_t *nbr, const int *nn, int cap,
-                                                   const int *row2atom, int nrows,
+                                                   const int *row2atom, int rbase, int nrows,
                                                    const float4 *tab, float *G, uint32_t *kl)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int row = blockIdx.x;
+  const int row = rbase + (int) blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int gw = g.nelt * K * kNM;
@@ -528,14 +528,14 @@ __device__ inline void store_P(float (*acc)[Tiling<K>::KT][Tiling<K>::DT], float
 // NS, MP: as for k_descriptor
 template <int K, int NS, bool MP>
 __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, Geom geo, const double4 *pos, int cap,
-                                              const int *row2atom, int nrows, const float4 *tab,
+                                              const int *row2atom, int rbase, int nrows, const float4 *tab,
                                               const uint32_t *kl, const float *dEdG,
                                               const double *E, double scale, double *f,
                                               double *eatom, double *watom, double *vatom)
 {
   using T = Tiling<K>;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int row = blockIdx.x;
+  const int row = rbase + (int) blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int i = row2atom[row];
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, 
 
 template <int K, int NS, bool MP>
 hipError_t desc_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
-                     const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
+                     const int *nn, int cap, const int *row2atom, int rbase, int nrows, const float4 *tab,
                      float *G, uint32_t *kl, hipStream_t s)
 {
   size_t offs[3];
@@ -646,13 +646,14 @@ hipError_t desc_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, con
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
       ae != hipSuccess)
     return ae;
-  k_descriptor<K, NS, MP><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl);
+  k_descriptor<K, NS, MP><<<nrows - rbase, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, rbase, nrows, tab,
+                                                         G, kl);
   return hipGetLastError();
 }
 
 template <int K, int NS, bool MP>
 hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
-                      const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
+                      const int *row2atom, int rbase, int nrows, const float4 *tab, const uint32_t *kl,
                       const float *dEdG, const double *E, double scale, double *f, double *eatom,
                       double *watom, double *vatom, hipStream_t s)
 {
@@ -663,8 +664,8 @@ hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, in
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
       ae != hipSuccess)
     return ae;
-  k_force<K, NS, MP><<<nrows, 64, lds, s>>>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E,
-                                            scale, f, eatom, watom, vatom);
+  k_force<K, NS, MP><<<nrows - rbase, 64, lds, s>>>(g, geo, pos, cap, row2atom, rbase, nrows, tab, kl, dEdG, E,
+                                                    scale, f, eatom, watom, vatom);
   return hipGetLastError();
 }
 
@@ -673,16 +674,16 @@ hipError_t force_impl(const ModelGeo &g, const Geom &geo, const double4 *pos, in
 // the largest such count (k = 64: 4, k = 128: 2) take several passes.
 template <int K, bool FORCE>
 hipError_t dispatch(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
-                    const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
+                    const int *nn, int cap, const int *row2atom, int rbase, int nrows, const float4 *tab,
                     float *G, uint32_t *kl, const float *dEdG, const double *E, double scale,
                     double *f, double *eatom, double *watom, double *vatom, hipStream_t s)
 {
   constexpr int per = Tiling<K>::KT * Tiling<K>::DT;
   constexpr int kMaxNS = 8 * per <= 128 ? 8 : (4 * per <= 128 ? 4 : (2 * per <= 128 ? 2 : 1));
 #define TMD_LAUNCH(NS, MP)                                                                          \
-  return FORCE ? force_impl<K, NS, MP>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale, \
-                                       f, eatom, watom, vatom, s)                                   \
-               : desc_impl<K, NS, MP>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, s);
+  return FORCE ? force_impl<K, NS, MP>(g, geo, pos, cap, row2atom, rbase, nrows, tab, kl, dEdG, E,   \
+                                       scale, f, eatom, watom, vatom, s)                            \
+               : desc_impl<K, NS, MP>(g, geo, pos, nbr, nn, cap, row2atom, rbase, nrows, tab, G, kl, s);
   static_assert(kMaxNS >= 2, "every filter count holds two slots per pass");
   if (g.nelt < 1 || g.nelt > kMaxElts) return hipErrorInvalidValue;
   if (g.nelt <= 1) TMD_LAUNCH(1, false)
@@ -704,13 +705,14 @@ hipError_t dispatch(const ModelGeo &g, const Geom &geo, const double4 *pos, cons
 hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *pos,
                              const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
                              int nrows, const float4 *tab, float *G, uint32_t *kl,
-                             hipStream_t stream)
+                             hipStream_t stream, int row_begin)
 {
-  if (nrows <= 0) return hipSuccess;
-  if (g.mm > 3) return launch_descriptor_hm(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, stream);
-#define TMD_K_CASE(K)                                                                         \
-  case K:                                                                                     \
-    return dispatch<K, false>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, nullptr, \
+  if (nrows <= row_begin) return hipSuccess;
+  if (g.mm > 3)
+    return launch_descriptor_hm(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, stream, row_begin);
+#define TMD_K_CASE(K)                                                                                    \
+  case K:                                                                                                \
+    return dispatch<K, false>(g, geo, pos, nbr, nn, cap, row2atom, row_begin, nrows, tab, G, kl, nullptr, \
                               nullptr, 0, nullptr, nullptr, nullptr, nullptr, stream);
   switch (g.k) {
     TMD_K_CASE(16)
@@ -726,17 +728,16 @@ hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *
 hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
                         const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
                         const float *dEdG, const double *E, double scale, double *f, double *eatom,
-                        double *watom, double *vatom, hipStream_t stream)
+                        double *watom, double *vatom, hipStream_t stream, int row_begin)
 {
-  if (nrows <= 0) return hipSuccess;
+  if (nrows <= row_begin) return hipSuccess;
   if (g.mm > 3)
     return launch_force_hm(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale, f, eatom, watom, vatom,
-                           stream);
-#define TMD_K_CASE(K)                                                                            \
-  case K:                                                                                        \
-    return dispatch<K, true>(g, geo, pos, nullptr, nullptr, cap, row2atom, nrows, tab, nullptr,  \
-                             const_cast<uint32_t *>(kl), dEdG, E, scale, f, eatom, watom, vatom, \
-                             stream);
+                           stream, row_begin);
+#define TMD_K_CASE(K)                                                                                       \
+  case K:                                                                                                   \
+    return dispatch<K, true>(g, geo, pos, nullptr, nullptr, cap, row2atom, row_begin, nrows, tab, nullptr,  \
+                             const_cast<uint32_t *>(kl), dEdG, E, scale, f, eatom, watom, vatom, stream);
   switch (g.k) {
     TMD_K_CASE(16)
     TMD_K_CASE(24)

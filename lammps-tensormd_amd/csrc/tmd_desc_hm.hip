@@ -168,14 +168,14 @@ __device__ inline void build_P_hm(const ModelGeo &g, const Box &cell, const doub
 template <int K, int MM>
 __global__ __launch_bounds__(64) void k_descriptor_hm(ModelGeo g, Geom geo, const double4 *pos,
                                                       const uint32_t *nbr, const int *nn, int cap,
-                                                      const int *row2atom, int nrows, const float4 *tab,
-                                                      float *G, uint32_t *kl)
+                                                      const int *row2atom, int rbase, int nrows,
+                                                      const float4 *tab, float *G, uint32_t *kl)
 {
   using Mo = Moments<MM>;
   using Lo = HmLayout<K, MM>;
   constexpr int ND = Mo::ND, NM = Mo::NM;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int row = blockIdx.x;
+  const int row = rbase + (int) blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int gw = g.nelt * K * NM;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(64) void k_descriptor_hm(ModelGeo g, Geom geo, cons
 
 template <int K, int MM>
 __global__ __launch_bounds__(64) void k_force_hm(ModelGeo g, Geom geo, const double4 *pos, int cap,
-                                                 const int *row2atom, int nrows, const float4 *tab,
+                                                 const int *row2atom, int rbase, int nrows, const float4 *tab,
                                                  const uint32_t *kl, const float *dEdG, const double *E,
                                                  double scale, double *f, double *eatom, double *watom,
                                                  double *vatom)
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(64) void k_force_hm(ModelGeo g, Geom geo, const dou
   constexpr int ND = Mo::ND, NM = Mo::NM;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double facc[9];
-  const int row = blockIdx.x;
+  const int row = rbase + (int) blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int i = row2atom[row];
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(64) void k_force_hm(ModelGeo g, Geom geo, const dou
 
 template <int K, int MM>
 hipError_t desc_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr, const int *nn,
-                   int cap, const int *row2atom, int nrows, const float4 *tab, float *G, uint32_t *kl,
+                   int cap, const int *row2atom, int rbase, int nrows, const float4 *tab, float *G, uint32_t *kl,
                    hipStream_t s)
 {
   size_t off[4];
@@ -374,13 +374,14 @@ hipError_t desc_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, const
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
       e != hipSuccess)
     return e;
-  k_descriptor_hm<K, MM><<<nrows, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl);
+  k_descriptor_hm<K, MM><<<nrows - rbase, 64, lds, s>>>(g, geo, pos, nbr, nn, cap, row2atom, rbase, nrows, tab, G,
+                                                        kl);
   return hipGetLastError();
 }
 
 template <int K, int MM>
 hipError_t force_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap, const int *row2atom,
-                    int nrows, const float4 *tab, const uint32_t *kl, const float *dEdG, const double *E,
+                    int rbase, int nrows, const float4 *tab, const uint32_t *kl, const float *dEdG, const double *E,
                     double scale, double *f, double *eatom, double *watom, double *vatom, hipStream_t s)
 {
   size_t off[4];
@@ -390,8 +391,8 @@ hipError_t force_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, int 
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
       e != hipSuccess)
     return e;
-  k_force_hm<K, MM><<<nrows, 64, lds, s>>>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale, f,
-                                           eatom, watom, vatom);
+  k_force_hm<K, MM><<<nrows - rbase, 64, lds, s>>>(g, geo, pos, cap, row2atom, rbase, nrows, tab, kl, dEdG, E,
+                                                   scale, f, eatom, watom, vatom);
   return hipGetLastError();
 }
 
@@ -399,14 +400,14 @@ hipError_t force_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, int 
 
 hipError_t launch_descriptor_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, const uint32_t *nbr,
                                 const int *nn, int cap, const int *row2atom, int nrows, const float4 *tab,
-                                float *G, uint32_t *kl, hipStream_t stream)
+                                float *G, uint32_t *kl, hipStream_t stream, int row_begin)
 {
-  if (nrows <= 0) return hipSuccess;
+  if (nrows <= row_begin) return hipSuccess;
   if (g.nelt < 1 || g.nelt > kMaxElts) return hipErrorInvalidValue;
-#define TMD_HM_CASE(K)                                                                                   \
-  case K:                                                                                                \
-    return g.mm == 4 ? desc_hm<K, 4>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, stream)   \
-                     : desc_hm<K, 5>(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, stream);
+#define TMD_HM_CASE(K)                                                                                        \
+  case K:                                                                                                     \
+    return g.mm == 4 ? desc_hm<K, 4>(g, geo, pos, nbr, nn, cap, row2atom, row_begin, nrows, tab, G, kl, stream) \
+                     : desc_hm<K, 5>(g, geo, pos, nbr, nn, cap, row2atom, row_begin, nrows, tab, G, kl, stream);
   if (g.mm != 4 && g.mm != 5) return hipErrorInvalidValue;
   switch (g.k) {
     TMD_HM_CASE(16)
@@ -422,16 +423,16 @@ hipError_t launch_descriptor_hm(const ModelGeo &g, const Geom &geo, const double
 hipError_t launch_force_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap, const int *row2atom,
                            int nrows, const float4 *tab, const uint32_t *kl, const float *dEdG, const double *E,
                            double scale, double *f, double *eatom, double *watom, double *vatom,
-                           hipStream_t stream)
+                           hipStream_t stream, int row_begin)
 {
-  if (nrows <= 0) return hipSuccess;
+  if (nrows <= row_begin) return hipSuccess;
   if (g.nelt < 1 || g.nelt > kMaxElts) return hipErrorInvalidValue;
-#define TMD_HM_CASE(K)                                                                                       \
-  case K:                                                                                                    \
-    return g.mm == 4 ? force_hm<K, 4>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale, f, eatom, \
-                                      watom, vatom, stream)                                                 \
-                     : force_hm<K, 5>(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale, f, eatom, \
-                                      watom, vatom, stream);
+#define TMD_HM_CASE(K)                                                                                     \
+  case K:                                                                                                  \
+    return g.mm == 4 ? force_hm<K, 4>(g, geo, pos, cap, row2atom, row_begin, nrows, tab, kl, dEdG, E, scale, \
+                                      f, eatom, watom, vatom, stream)                                      \
+                     : force_hm<K, 5>(g, geo, pos, cap, row2atom, row_begin, nrows, tab, kl, dEdG, E, scale, \
+                                      f, eatom, watom, vatom, stream);
   if (g.mm != 4 && g.mm != 5) return hipErrorInvalidValue;
   switch (g.k) {
     TMD_HM_CASE(16)

@@ -150,7 +150,7 @@ __host__ __device__ inline size_t kept_cache_words(int cap) { return (size_t) kK
 hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *pos,
                              const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
                              int nrows, const float4 *tab, float *G, uint32_t *kl,
-                             hipStream_t stream);
+                             hipStream_t stream, int row_begin = 0);
 
 // Force/virial over the descriptor's kept lists: dEdG rows -> f (atomics,
 // FP64, must be zeroed); per atom: eatom[i] = scale * E[row] and
@@ -159,18 +159,19 @@ hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *
 hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
                         const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
                         const float *dEdG, const double *E, double scale, double *f, double *eatom,
-                        double *watom, double *vatom, hipStream_t stream);
+                        double *watom, double *vatom, hipStream_t stream, int row_begin = 0);
 
 // The same two kernels for max_moment 4 and 5 models (g.mm 4 or 5): 35 or 56
 // monomials (tmd_desc_hm.hip); launch_descriptor/launch_force call them.
 hipError_t launch_descriptor_hm(const ModelGeo &g, const Geom &geo, const double4 *pos,
                                 const uint32_t *nbr, const int *nn, int cap, const int *row2atom,
                                 int nrows, const float4 *tab, float *G, uint32_t *kl,
-                                hipStream_t stream);
+                                hipStream_t stream, int row_begin = 0);
 hipError_t launch_force_hm(const ModelGeo &g, const Geom &geo, const double4 *pos, int cap,
                            const int *row2atom, int nrows, const float4 *tab, const uint32_t *kl,
                            const float *dEdG, const double *E, double scale, double *f,
-                           double *eatom, double *watom, double *vatom, hipStream_t stream);
+                           double *eatom, double *watom, double *vatom, hipStream_t stream,
+                           int row_begin = 0);
 
 // ---- MLP ----------------------------------------------------------------------
 // mode: 0 fp32, 1 bf16, 2 fp8 (e4m3 operands: weights with per-output-channel
@@ -178,9 +179,11 @@ hipError_t launch_force_hm(const ModelGeo &g, const Geom &geo, const double4 *po
 // E[row] (FP64 output sum) and dEdG[row][width] (fp32).  rtile: rows per
 // workgroup (16, 32 or 64; 0 = default).
 // With dyn, rl gives the capacity (grid) and dyn->rl the live layout.
+// Rows [row_begin, row_end) only (row_end < 0: to rl.nrows); both bounds
+// are multiples of 64 (element blocks start on such rows).
 hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, double *E,
                       float *dEdG, hipStream_t stream, int rtile = 0,
-                      const BatchDims *dyn = nullptr);
+                      const BatchDims *dyn = nullptr, int row_begin = 0, int row_end = -1);
 size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out);
 // the fast (aligned) MLP kernel applies: required by the FP8 mode
 bool mlp_fast_ok(const MlpParams &mp, int mode);

@@ -236,14 +236,14 @@ __device__ inline void wave_gemm(const typename Traits<MODE>::T *lds, int ld, co
 
 template <int MODE, int ACT, int R>
 __global__ __launch_bounds__(256) void k_mlp(RowLayout rl_cap, MlpParams mp, const float *Gd, double *E,
-                                             float *dEdG, const BatchDims *dyn)
+                                             float *dEdG, const BatchDims *dyn, int rbase)
 {
   const RowLayout rl = dyn ? dyn->rl : rl_cap;
   using T = typename Traits<MODE>::T;
   constexpr int CT = Traits<MODE>::CT, RT = R / 16;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sm = reinterpret_cast<T *>(smem_raw);
-  const int row0 = blockIdx.x * R;
+  const int row0 = rbase + (int) blockIdx.x * R;
   if (row0 >= rl.nrows) return;
   const int e = tile_elem(rl, row0);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -643,14 +643,14 @@ constexpr int kMlpWaves = R == 64 ? 2 : ((MODE == 1 || R == 16) ? 4 : 3);
 
 template <int MODE, int ACT, int R>
 __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayout rl_cap, MlpParams mp, const float *Gd,
-                                                  double *E, float *dEdG, const BatchDims *dyn)
+                                                  double *E, float *dEdG, const BatchDims *dyn, int rbase)
 {
   const RowLayout rl = dyn ? dyn->rl : rl_cap;
   using T = typename Traits<MODE>::T;
   constexpr int RT = R / 16;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sm = reinterpret_cast<T *>(smem_raw);
-  const int row0 = blockIdx.x * R;
+  const int row0 = rbase + (int) blockIdx.x * R;
   if (row0 >= rl.nrows) return;
   const int e = tile_elem(rl, row0);
   // wave index made wave-uniform for the compiler: tile bounds and guards
@@ -815,9 +815,10 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
 namespace {
 template <int MODE, int R>
 hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, double *E, float *dEdG,
-                    size_t lds, hipStream_t s, const BatchDims *dyn)
+                    size_t lds, hipStream_t s, const BatchDims *dyn, int rb, int re)
 {
-  dim3 grid(rl.nrows / R);
+  if (rb % R || re % R) return hipErrorInvalidValue;
+  dim3 grid((re - rb) / R);
   const bool fast = mlp_fast_ok(mp, MODE);
   if constexpr (MODE == 2) {  // FP8 exists on the fast path only
     if (!fast) return hipErrorInvalidConfiguration;
@@ -827,7 +828,7 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);              \
         ae != hipSuccess)                                                                        \
       return ae;                                                                                 \
-    k_mlp_fast<2, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                         \
+    k_mlp_fast<2, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn, rb);                         \
     break;
     switch (mp.act) {
       TMD_MLP8_CASE(0)
@@ -846,13 +847,13 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);            \
           ae != hipSuccess)                                                                      \
         return ae;                                                                               \
-      k_mlp_fast<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                    \
+      k_mlp_fast<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn, rb);                    \
     } else {                                                                                     \
       if (hipError_t ae = hipFuncSetAttribute((const void *) k_mlp<MODE, A, R>,                  \
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);            \
           ae != hipSuccess)                                                                      \
         return ae;                                                                               \
-      k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn);                         \
+      k_mlp<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn, rb);                         \
     }                                                                                            \
     break;
     switch (mp.act) {
@@ -947,9 +948,12 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
 }
 
 hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, double *E,
-                      float *dEdG, hipStream_t s, int rtile, const BatchDims *dyn)
+                      float *dEdG, hipStream_t s, int rtile, const BatchDims *dyn, int row_begin,
+                      int row_end)
 {
-  if (rl.nrows <= 0) return hipSuccess;
+  if (row_end < 0 || row_end > rl.nrows) row_end = rl.nrows;
+  if (row_begin < 0 || row_begin > row_end) return hipErrorInvalidValue;
+  if (row_end == row_begin) return hipSuccess;
   const size_t limit = 160 * 1024;
   // 32-row tiles by default: several workgroups per CU hide the LDS/L2
   // latency better than one 64-row tile, at twice the (L2-resident) weight
@@ -962,18 +966,18 @@ hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const 
     size_t lds = mlp_lds_bytes(mp, mode, R, &m);
     if (lds > limit) continue;
     if (mode == 0) {
-      if (R == 64) return launch_r<0, 64>(rl, m, G, E, dEdG, lds, s, dyn);
-      if (R == 32) return launch_r<0, 32>(rl, m, G, E, dEdG, lds, s, dyn);
-      return launch_r<0, 16>(rl, m, G, E, dEdG, lds, s, dyn);
+      if (R == 64) return launch_r<0, 64>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+      if (R == 32) return launch_r<0, 32>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+      return launch_r<0, 16>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
     }
     if (mode == 2) {
-      if (R == 64) return launch_r<2, 64>(rl, m, G, E, dEdG, lds, s, dyn);
-      if (R == 32) return launch_r<2, 32>(rl, m, G, E, dEdG, lds, s, dyn);
-      return launch_r<2, 16>(rl, m, G, E, dEdG, lds, s, dyn);
+      if (R == 64) return launch_r<2, 64>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+      if (R == 32) return launch_r<2, 32>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+      return launch_r<2, 16>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
     }
-    if (R == 64) return launch_r<1, 64>(rl, m, G, E, dEdG, lds, s, dyn);
-    if (R == 32) return launch_r<1, 32>(rl, m, G, E, dEdG, lds, s, dyn);
-    return launch_r<1, 16>(rl, m, G, E, dEdG, lds, s, dyn);
+    if (R == 64) return launch_r<1, 64>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+    if (R == 32) return launch_r<1, 32>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+    return launch_r<1, 16>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
   }
   return hipErrorInvalidConfiguration;
 }

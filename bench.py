@@ -275,7 +275,7 @@ def main():
     ap.add_argument("--cells", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--precisions", default="fp32,bf16")
+    ap.add_argument("--precisions", default="fp32,bf16,bf16x3")
     ap.add_argument("--serve-requests", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--c4-steps", type=int, default=200)
@@ -394,9 +394,10 @@ def main():
         flops = pk[kname + "_flop"] * natoms / t_s / 1e12
         gbs = pk[kname + "_bytes"] * natoms / t_s / 1e9
         if kname == "mlp":
-            peak = PEAK["bf16_tflops"] if prec == "bf16" else PEAK["fp32_tflops"]
+            peak = PEAK["bf16_tflops"] if prec.startswith("bf16") else PEAK["fp32_tflops"]
+            unit = {"bf16": "BF16", "bf16x3": "BF16, three products per product (split operands)"}
             row = dict(kernel=f"k_mlp_fast {prec} (fused per-element MLP fwd+bwd on "
-                              f"{'BF16' if prec == 'bf16' else 'FP32'} MFMA)", bound="mfma")
+                              f"{unit.get(prec, 'FP32')} MFMA)", bound="mfma")
         else:
             # fused FP32 gather/scatter kernels without MFMA: arithmetic
             # intensity 50-80 FLOP/B, so the binding roof is FP32 vector
@@ -517,6 +518,9 @@ def main():
         "fp32_atom_steps_per_s": fp["rate"],
         "fp32_ms_per_step": 1e3 * fp["time"] / args.steps,
         "bf16_over_fp32": bf["rate"] / fp["rate"],
+        "bf16x3_atom_steps_per_s": res["bf16x3"]["rate"] if "bf16x3" in res else None,
+        "bf16x3_ms_per_step": 1e3 * res["bf16x3"]["time"] / args.steps if "bf16x3" in res else None,
+        "bf16x3_over_fp32": res["bf16x3"]["rate"] / fp["rate"] if "bf16x3" in res else None,
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serving,

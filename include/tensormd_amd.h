@@ -63,6 +63,9 @@ extern "C" {
 #define TMD_MLP_FP32 0 /* plain FP32 path */
 #define TMD_MLP_BF16 1 /* BF16 operands, FP32 accumulation */
 #define TMD_MLP_FP8 2  /* FP8 e4m3 operands, per-output-channel FP32 scales */
+#define TMD_MLP_BF16X3 3 /* split BF16: operands as hi + lo BF16 pairs, three
+                            BF16 MFMA products per product, FP32 accumulation
+                            (about 16 significant bits) */
 
 typedef struct tmd_ctx tmd_ctx;
 typedef struct tmd_system tmd_system;

@@ -99,6 +99,14 @@ static uint16_t bf16_bits(float f)
   return (uint16_t) (u >> 16);
 }
 
+static float bf16_float(uint16_t h)
+{
+  const uint32_t u = (uint32_t) h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
 // TDNP networks H, U, S: per layer, all elements packed, W [K][N], its
 // transpose and the biases in FP32; the electron temperature as a device
 // scalar.
@@ -249,7 +257,8 @@ int upload_model(tmd_ctx *ctx)
     for (int l = 0; l <= L; l++) t.sizes[l] = n0.sizes[l];
     fast = mlp_fast_ok(t, mode);
   }
-  const size_t aesz = mode == TMD_MLP_FP32 ? 4 : (mode == TMD_MLP_BF16 ? 2 : 1);
+  // bytes per weight: FP32, BF16, FP8, or a BF16 hi + lo pair
+  const size_t aesz = mode == TMD_MLP_FP32 ? 4 : (mode == TMD_MLP_BF16 ? 2 : (mode == TMD_MLP_FP8 ? 1 : 4));
   std::vector<Off> offs(L);
   auto al = [](size_t x) { return (x + 255) & ~(size_t) 255; };
   for (int l = 0; l < L; l++) {
@@ -313,8 +322,10 @@ int upload_model(tmd_ctx *ctx)
         // fragment order: [tile t][step s][lane][v]; lane = (group g) * 16 +
         // (row in tile); forward outputs n = 16 t + row, inner k = s KS + g V
         // + v (KS, V = 16, 4 for FP32, whose four MFMAs per step take the
-        // values v; 32, 8 for BF16); backward swaps K and N
+        // values v; 32, 8 for BF16); backward swaps K and N.  Split BF16:
+        // the lane's 8 high parts, then its 8 low parts.
         const bool f32 = mode == TMD_MLP_FP32;
+        const bool split = mode == TMD_MLP_BF16X3;
         const int KS = f32 ? 16 : 32, V = f32 ? 4 : 8;
         const double *Wd = net.W[l].data();
         auto frag = [&](unsigned char *dst, int out, int inner, bool fwd) {
@@ -326,10 +337,18 @@ int upload_model(tmd_ctx *ctx)
                   const int o = 16 * t + lane % 16, in = st * KS + (lane / 16) * V + v;
                   const double w = fwd ? Wd[(size_t) in * N + o] : Wd[(size_t) o * N + in];
                   const size_t q = (((size_t) t * nst + st) * 64 + lane) * V + v;
-                  if (f32)
+                  if (f32) {
                     reinterpret_cast<float *>(dst)[q] = (float) w;
-                  else
+                  } else if (split) {
+                    const float wf = (float) w;
+                    const uint16_t hi = bf16_bits(wf);
+                    const float hf = bf16_float(hi);
+                    const size_t q2 = (((size_t) t * nst + st) * 64 + lane) * 2 * V + v;
+                    reinterpret_cast<uint16_t *>(dst)[q2] = hi;
+                    reinterpret_cast<uint16_t *>(dst)[q2 + V] = bf16_bits(wf - hf);
+                  } else {
                     reinterpret_cast<uint16_t *>(dst)[q] = bf16_bits((float) w);
+                  }
                 }
         };
         frag(&host[offs[l].af] + el * per * aesz, N, K, true);
@@ -378,6 +397,9 @@ int upload_model(tmd_ctx *ctx)
     return (set_err(ctx, "MLP too wide for the fused kernel's LDS budget"), TMD_ERR_MODEL);
   if (fp8 && !mlp_fast_ok(mp, mode))
     return (set_err(ctx, "the FP8 MLP needs hidden and input widths that are multiples of 64"),
+            TMD_ERR_MODEL);
+  if (mode == TMD_MLP_BF16X3 && !mlp_fast_ok(mp, mode))
+    return (set_err(ctx, "the split-BF16 MLP needs hidden and input widths that are multiples of 64"),
             TMD_ERR_MODEL);
   e = hipMemcpy(ctx->params.p, host.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(ctx, e, "parameter upload");
@@ -530,7 +552,7 @@ int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes
     ctx->opt = *opt;
   else
     tmd_options_default(&ctx->opt);
-  if (ctx->opt.mlp_precision < TMD_MLP_FP32 || ctx->opt.mlp_precision > TMD_MLP_FP8) {
+  if (ctx->opt.mlp_precision < TMD_MLP_FP32 || ctx->opt.mlp_precision > TMD_MLP_BF16X3) {
     set_err(nullptr, "tmd_create: unknown mlp_precision");
     return TMD_ERR_ARG;
   }
@@ -582,6 +604,11 @@ int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes
   return TMD_OK;
 }
 
+tmd_ctx::~tmd_ctx()
+{
+  if (stream) hipStreamDestroy(stream);
+}
+
 void tmd_destroy(tmd_ctx *ctx)
 {
   if (!ctx) return;
@@ -595,7 +622,6 @@ void tmd_destroy(tmd_ctx *ctx)
   ctx->list_host.release();
   ctx->tab.release();
   ctx->params.release();
-  hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 

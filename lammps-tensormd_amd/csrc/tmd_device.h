@@ -175,7 +175,8 @@ hipError_t launch_force_hm(const ModelGeo &g, const Geom &geo, const double4 *po
 
 // ---- MLP ----------------------------------------------------------------------
 // mode: 0 fp32, 1 bf16, 2 fp8 (e4m3 operands: weights with per-output-channel
-// scales, activations and gradients with per-row scales; FP32 accumulation).
+// scales, activations and gradients with per-row scales; FP32 accumulation),
+// 3 split bf16 (hi + lo operands, three BF16 products each).
 // E[row] (FP64 output sum) and dEdG[row][width] (fp32).  rtile: rows per
 // workgroup (16, 32 or 64; 0 = default).
 // With dyn, rl gives the capacity (grid) and dyn->rl the live layout.

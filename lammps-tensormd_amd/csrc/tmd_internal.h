@@ -108,6 +108,10 @@ struct tmd_ctx {
   long long list_serial = 0;
   int list_nlocal = 0, list_nall = 0, list_inum = 0, list_cap = 0;
   tmd::Server *server = nullptr;  // request loop (tmd_serve_start)
+  tmd_ctx() = default;
+  tmd_ctx(const tmd_ctx &) = delete;
+  tmd_ctx &operator=(const tmd_ctx &) = delete;
+  ~tmd_ctx();  // destroys the stream (also when tmd_create fails part way)
 };
 
 namespace tmd {

@@ -16,6 +16,11 @@
 //
 // FP32 mode: v_mfma_f32_16x16x4_f32; BF16 mode: v_mfma_f32_16x16x32_bf16
 // (operands rounded to BF16, FP32 accumulation; H/D kept as BF16 in LDS).
+// Split-BF16 mode (fast kernel only): every operand x is carried as
+// hi = bf16(x) and lo = bf16(x - hi), and each product as the three BF16
+// MFMAs hi*hi + hi*lo + lo*hi with FP32 accumulation (the dropped lo*lo term
+// and the split leave ~2^-16 relative error per product, against 2^-8 for
+// plain BF16); H/D stay FP32 in LDS.
 // Operand layouts (16x16 tiles): A lane l holds row l%16, k-group l/16;
 // B lane l holds column l%16, k-group l/16; C lane l holds rows
 // 4*(l/16)+i, column l%16.
@@ -84,6 +89,12 @@ struct Traits<1> {
 template <>  // FP8 operands, BF16 activations in LDS
 struct Traits<2> {
   using T = uint16_t;
+  static constexpr int KS = 32;
+  static constexpr int CT = 4;
+};
+template <>  // split-BF16 operands, FP32 activations in LDS
+struct Traits<3> {
+  using T = float;
   static constexpr int KS = 32;
   static constexpr int CT = 4;
 };
@@ -398,6 +409,22 @@ template <>
 struct Frag<2> {
   using A = long;  // 8 FP8 e4m3 values
 };
+// split BF16: the high and low parts of 8 values
+struct SplitBf16 {
+  bf16x8 hi, lo;
+};
+template <>
+struct Frag<3> {
+  using A = SplitBf16;
+};
+
+__device__ inline SplitBf16 split8(const f32x8 &x)
+{
+  SplitBf16 r;
+  r.hi = __builtin_convertvector(x, bf16x8);
+  r.lo = __builtin_convertvector(x - __builtin_convertvector(r.hi, f32x8), bf16x8);
+  return r;
+}
 
 // 8 floats -> 8 FP8 e4m3 (round to nearest even), first value in the low byte
 __device__ inline long pack_fp8(const float *v)
@@ -420,6 +447,8 @@ __device__ inline typename Frag<MODE>::A ld_a(const void *A, int t, int s, int n
     return reinterpret_cast<const f32x4 *>(A)[idx];
   else if constexpr (MODE == 1)
     return reinterpret_cast<const bf16x8 *>(A)[idx];
+  else if constexpr (MODE == 3)
+    return reinterpret_cast<const SplitBf16 *>(A)[idx];
   else
     return reinterpret_cast<const long *>(A)[idx];
 }
@@ -434,6 +463,8 @@ __device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k,
     return *reinterpret_cast<const f32x4 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 4);
   } else if constexpr (MODE == 1) {
     return *reinterpret_cast<const bf16x8 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8);
+  } else if constexpr (MODE == 3) {
+    return split8(*reinterpret_cast<const f32x8 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8));
   } else {
     uint4 u = *reinterpret_cast<const uint4 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8);
     float v[8] = {__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
@@ -483,6 +514,9 @@ __device__ inline typename Frag<MODE>::A cvt_graw(const GRaw<MODE> &g, float ri)
   } else if constexpr (MODE == 1) {
     f32x8 xv = {g.lo.x, g.lo.y, g.lo.z, g.lo.w, g.hi.x, g.hi.y, g.hi.z, g.hi.w};
     return __builtin_convertvector(xv, bf16x8);
+  } else if constexpr (MODE == 3) {
+    f32x8 xv = {g.lo.x, g.lo.y, g.lo.z, g.lo.w, g.hi.x, g.hi.y, g.hi.z, g.hi.w};
+    return split8(xv);
   } else {
     float v[8] = {g.lo.x * ri, g.lo.y * ri, g.lo.z * ri, g.lo.w * ri,
                   g.hi.x * ri, g.hi.y * ri, g.hi.z * ri, g.hi.w * ri};
@@ -500,6 +534,10 @@ __device__ inline f32x4 mma(typename Frag<MODE>::A a, typename Frag<MODE>::A b, 
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], b[3], c, 0, 0, 0);
   } else if constexpr (MODE == 1) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  } else if constexpr (MODE == 3) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
   } else {
     return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
   }
@@ -641,6 +679,10 @@ __device__ inline void gemm_n(const void *W, int t0, int ntw, const T *X, int ld
 template <int MODE, int R>
 constexpr int kMlpWaves = R == 64 ? 2 : ((MODE == 1 || R == 16) ? 4 : 3);
 
+// bytes per weight of the fast kernel's A operands
+template <int MODE>
+constexpr size_t kAsz = MODE == 0 ? 4 : (MODE == 1 ? 2 : (MODE == 2 ? 1 : 4));
+
 template <int MODE, int ACT, int R>
 __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayout rl_cap, MlpParams mp, const float *Gd,
                                                   double *E, float *dEdG, const BatchDims *dyn, int rbase)
@@ -665,7 +707,7 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
   // ---------------- forward ----------------
   for (int l = 0; l < L - 1; l++) {
     const int K = mp.sizes[l], N = mp.sizes[l + 1];
-    constexpr size_t esz = MODE == 0 ? 4 : (MODE == 1 ? 2 : 1);
+    constexpr size_t esz = kAsz<MODE>;
     const void *W = (const unsigned char *) mp.Af[l] + (size_t) e * mp.wstride[l] * esz;
     const float *bias = mp.b[l] ? mp.b[l] + (size_t) e * mp.bstride[l] : nullptr;
     T *H = sm + mp.offH[l];
@@ -706,7 +748,7 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
           float h[4], dz[4];
 #pragma unroll
           for (int i = 0; i < 4; i++)
-            h[i] = act_f<ACT, MODE != 0>(acc[j][r][i] * (sr * ws[i]) + bv[i], dz[i]);
+            h[i] = act_f<ACT, MODE == 1 || MODE == 2>(acc[j][r][i] * (sr * ws[i]) + bv[i], dz[i]);
           if (mp.res[l]) {
             float x[4];
             ld4(Hin + row * ldi + c0, x);
@@ -756,7 +798,7 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
   for (int l = L - 2; l >= 0; l--) {
     const int K = mp.sizes[l], N = mp.sizes[l + 1];
     // Gin^T [K x R] = W_l [K x N] G_l^T [N x R]
-    constexpr size_t esz = MODE == 0 ? 4 : (MODE == 1 ? 2 : 1);
+    constexpr size_t esz = kAsz<MODE>;
     const void *W = (const unsigned char *) mp.Ab[l] + (size_t) e * mp.wstride[l] * esz;
     const T *Gl = sm + mp.offD[l];
     const int ldg = mp.ld[l + 1];
@@ -820,15 +862,15 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
   if (rb % R || re % R) return hipErrorInvalidValue;
   dim3 grid((re - rb) / R);
   const bool fast = mlp_fast_ok(mp, MODE);
-  if constexpr (MODE == 2) {  // FP8 exists on the fast path only
+  if constexpr (MODE == 2 || MODE == 3) {  // FP8 and split BF16 exist on the fast path only
     if (!fast) return hipErrorInvalidConfiguration;
 #define TMD_MLP8_CASE(A)                                                                         \
   case A:                                                                                        \
-    if (hipError_t ae = hipFuncSetAttribute((const void *) k_mlp_fast<2, A, R>,                  \
+    if (hipError_t ae = hipFuncSetAttribute((const void *) k_mlp_fast<MODE, A, R>,               \
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);              \
         ae != hipSuccess)                                                                        \
       return ae;                                                                                 \
-    k_mlp_fast<2, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn, rb);                         \
+    k_mlp_fast<MODE, A, R><<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn, rb);                  \
     break;
     switch (mp.act) {
       TMD_MLP8_CASE(0)
@@ -914,13 +956,16 @@ hipError_t launch_quant_fp8(const float *W, const float *swf, const float *swb, 
 size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
 {
   MlpParams m = mp;
-  size_t esz = mode == 0 ? 4 : 2;  // BF16 activations in LDS for the BF16 and FP8 modes
+  // FP32 activations in LDS for the FP32 and split-BF16 modes, BF16 for
+  // the BF16 and FP8 modes
+  const bool f32lds = mode == 0 || mode == 3;
+  size_t esz = f32lds ? 4 : 2;
   size_t off = 0;
   int maxw = 0;
   bool anyres = false;
   for (int l = 0; l <= mp.nl - 1; l++) {
     int w = mp.sizes[l];
-    m.ld[l] = (mode == 0) ? w + 4 : w + 8;
+    m.ld[l] = f32lds ? w + 4 : w + 8;
   }
   for (int l = 0; l < mp.nl - 1; l++)
     if (mp.res[l]) anyres = true;
@@ -937,7 +982,7 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
       off += (size_t) R * m.ld[l + 1];
     }
   }
-  m.ldGH = (mode == 0) ? maxw + 4 : maxw + 8;
+  m.ldGH = f32lds ? maxw + 4 : maxw + 8;
   m.offGH = (int) off;
   if (anyres) off += (size_t) R * m.ldGH;
   size_t bytes = (off * esz + 15) & ~(size_t) 15;
@@ -969,6 +1014,11 @@ hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const 
       if (R == 64) return launch_r<0, 64>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
       if (R == 32) return launch_r<0, 32>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
       return launch_r<0, 16>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+    }
+    if (mode == 3) {
+      if (R == 64) return launch_r<3, 64>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+      if (R == 32) return launch_r<3, 32>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+      return launch_r<3, 16>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
     }
     if (mode == 2) {
       if (R == 64) return launch_r<2, 64>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);

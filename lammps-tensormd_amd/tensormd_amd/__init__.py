@@ -17,7 +17,7 @@ TMD_OK = 0
 TMD_PENDING = 1
 ERRORS = {-1: "TMD_ERR_ARG", -2: "TMD_ERR_MODEL", -3: "TMD_ERR_DEVICE", -4: "TMD_ERR_NOMEM",
           -5: "TMD_ERR_CAPACITY", -6: "TMD_ERR_STATE"}
-PRECISION = {"fp32": 0, "bf16": 1, "fp8": 2}
+PRECISION = {"fp32": 0, "bf16": 1, "fp8": 2, "bf16x3": 3}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)

@@ -76,14 +76,16 @@ def test_bad_interp_is_model_error(synth_models):
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device path")
 def test_fp8_precision_is_accepted(synth_models):
-    """The FP8 MLP mode passes option checks (the model's widths are checked
-    against the FP8 kernel's alignment after device initialisation)."""
-    rc, msg = _create_rc(synth_models["c1"], ["Cu"], precision="fp8")
-    assert rc == -3, msg
+    """The FP8 and split-BF16 MLP modes pass option checks (the model's
+    widths are checked against the fast kernel's alignment after device
+    initialisation); an unknown mode is an argument error."""
+    for prec in ("fp8", "bf16x3"):
+        rc, msg = _create_rc(synth_models["c1"], ["Cu"], precision=prec)
+        assert rc == -3, msg
     L = T.lib()
     opt = T.Options()
     L.tmd_options_default(ctypes.byref(opt))
-    opt.mlp_precision = 3  # no such mode
+    opt.mlp_precision = 4  # no such mode
     arr = (ctypes.c_char_p * 1)(b"Cu")
     h = ctypes.c_void_p()
     rc = L.tmd_create(str(synth_models["c1"]).encode(), arr, 1, ctypes.byref(opt), ctypes.byref(h))

@@ -444,6 +444,9 @@ def main():
     fp8 = None
     if args.c4_steps > 0:
         fp8 = fp8_experiment(args.c4_steps, 20)
+        print("bench: c4 " + json.dumps({k: dict(ms_per_step=v["ms_per_step"], mlp_ms=v["mlp_ms"])
+                                         for k, v in fp8.items() if isinstance(v, dict) and "mlp_ms" in v}),
+              flush=True)
 
     serving = None
     if args.serve_requests > 0:
@@ -496,7 +499,9 @@ def main():
 
     if "bf16" not in res or "fp32" not in res:
         print(json.dumps({"partial": True, "rates": {k: v["rate"] for k, v in res.items()},
-                          "roofline": roof}), flush=True)
+                          "roofline": roof, "cpu_baseline": cpu, "serving": serving,
+                          "fp8_experiment": fp8, "list_mode": listmode, "nve_drift": drift or None}),
+              flush=True)
         return
     bf, fp = res["bf16"], res["fp32"]
     line = {

@@ -487,9 +487,10 @@ __device__ inline void force_chunk(const ModelGeo &g, const Box &cell, const dou
       for (int q = 0; q < 6; q++) cf[3 + q] = v[q];
     }
     // the chunk's own-force and virial sums, in a fixed lane order, into the
-    // atom's LDS accumulators (nothing FP64 is live through the filter loop)
+    // atom's LDS accumulators (nothing FP64 is live through the filter loop);
+    // only each neighbor's first lane holds values
 #pragma unroll
-    for (int q = 0; q < 9; q++) cf[q] = wave_sum_d(cf[q]);
+    for (int q = 0; q < 9; q++) cf[q] = wave_sum_d_lp<LP>(cf[q]);
     if (lane == 0) {
 #pragma unroll
       for (int q = 0; q < 9; q++) facc[q] += cf[q];

@@ -130,6 +130,21 @@ __device__ inline double wave_sum_d(double v)
   return v;
 }
 
+// The same sum when only lanes that are multiples of LP (1, 2, 4 or 8) hold
+// non-zero values; the total is exact in lane 0 (the first steps, which
+// would only add zeros there, are skipped).
+template <int LP>
+__device__ inline double wave_sum_d_lp(double v)
+{
+  if constexpr (LP < 2) v += dpp_d<kDppXor1>(v);
+  if constexpr (LP < 4) v += dpp_d<kDppXor2>(v);
+  if constexpr (LP < 8) v += dpp_d<kDppRor4>(v);
+  v += dpp_d<kDppRor8>(v);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
 __device__ inline unsigned lane_rank(unsigned long long m)
 {
   return __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));

@@ -86,8 +86,8 @@ struct MlpParams {
   const uint16_t *Wtb[kMaxLayers];
   const uint16_t *Wb16[kMaxLayers];
   // fast path: the MFMA A operands of hidden layer l in the context's
-  // precision (FP32, BF16, or FP8 e4m3 divided by the per-output-channel
-  // scale), stored in fragment order: tile t of 16 output channels, step s
+  // precision (FP32, BF16, a BF16 hi + lo pair, or FP8 e4m3 divided by the
+  // per-output-channel scale), stored in fragment order: tile t of 16 output channels, step s
   // of KS inner values, lane, then the lane's values, so that a wave loads
   // one fragment as one contiguous block.  Af: forward (outputs N, inner K),
   // Ab: backward (outputs K, inner N).  Element e at byte offset
@@ -105,6 +105,7 @@ struct MlpParams {
   int offH[kMaxLayers], offD[kMaxLayers], ld[kMaxLayers + 1];
   int offGH, ldGH;
   int offRS;   // FP8: per-row scale and inverse scale [2][R] (floats, byte offset)
+  int offQ, ldq;  // FP8: the next GEMM's B operand rows, quantized [R][ldq] (bytes)
   int storeD;  // D_l kept in LDS (otherwise recomputed from H_l: tanh)
 };
 

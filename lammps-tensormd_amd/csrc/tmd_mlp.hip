@@ -36,6 +36,7 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 __device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16) f); }
 typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
 __device__ inline float bf2f(uint16_t v) { return __uint_as_float(((uint32_t) v) << 16); }
 
 // activation value and derivative (nnp.cu:9-49, activation.hpp:158-261).
@@ -398,16 +399,17 @@ template <>
 struct Frag<0> {
   using A = f32x4;
 };
-// inner values per fast-path step: 16 (FP32: four MFMAs) or 32
+// inner values per fast-path step: 16 (FP32: four MFMAs), 32 (BF16), or
+// 128 (FP8: one K=128 MFMA)
 template <int MODE>
-constexpr int kFastKS = MODE == 0 ? 16 : 32;
+constexpr int kFastKS = MODE == 0 ? 16 : (MODE == 2 ? 128 : 32);
 template <>
 struct Frag<1> {
   using A = bf16x8;
 };
 template <>
 struct Frag<2> {
-  using A = long;  // 8 FP8 e4m3 values
+  using A = i32x8;  // 32 FP8 e4m3 values
 };
 // split BF16: the high and low parts of 8 values
 struct SplitBf16 {
@@ -450,7 +452,7 @@ __device__ inline typename Frag<MODE>::A ld_a(const void *A, int t, int s, int n
   else if constexpr (MODE == 3)
     return reinterpret_cast<const SplitBf16 *>(A)[idx];
   else
-    return reinterpret_cast<const long *>(A)[idx];
+    return reinterpret_cast<const i32x8 *>(A)[idx];
 }
 
 // B operand (activations of rows r0..r0+15) from a row-major LDS tile;
@@ -465,15 +467,63 @@ __device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k,
     return *reinterpret_cast<const bf16x8 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8);
   } else if constexpr (MODE == 3) {
     return split8(*reinterpret_cast<const f32x8 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8));
-  } else {
-    uint4 u = *reinterpret_cast<const uint4 *>(X + (r0 + (lane & 15)) * ld + k + (lane >> 4) * 8);
-    float v[8] = {__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
-                  __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u),
-                  __uint_as_float(u.z << 16), __uint_as_float(u.z & 0xffff0000u),
-                  __uint_as_float(u.w << 16), __uint_as_float(u.w & 0xffff0000u)};
+  } else {  // FP8: 32 bytes of the quantized rows (X is the byte buffer Q)
+    const uint4 *p = reinterpret_cast<const uint4 *>(reinterpret_cast<const uint8_t *>(X) +
+                                                     (r0 + (lane & 15)) * ld + k + (lane >> 4) * 32);
+    const uint4 u0 = p[0], u1 = p[1];
+    return i32x8{(int) u0.x, (int) u0.y, (int) u0.z, (int) u0.w, (int) u1.x, (int) u1.y, (int) u1.z, (int) u1.w};
+  }
+}
+
+// FP8: the row scales of the next GEMM's B operand from the row maxima that
+// the producing epilogue recorded (rmax, float bits; |x| >= 0 orders as
+// unsigned), which are then cleared for the next producer.  Threads 0..R-1.
+__device__ inline void scales_from_max(unsigned *rmax, int R, float *rs)
+{
+  const int r = threadIdx.x;
+  if (r < R) {
+    const float m = __uint_as_float(rmax[r]);
+    const float sc = m > 0.0f ? m * (1.0f / 440.0f) : 1.0f;
+    rs[r] = sc;
+    rs[R + r] = 1.0f / sc;
+    rmax[r] = 0u;
+  }
+}
+
+// the largest |v| of 4 values into the row's recorded maximum
+__device__ inline void note_max(unsigned *rmax, int row, const float v[4])
+{
+  const float m = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
+  atomicMax(rmax + row, __float_as_uint(m));
+}
+
+// FP8: rows [0, R) of width w (a multiple of 8) divided by their row scale
+// (rinv = 1 / scale) and rounded to e4m3 into Q [R][ldq], once per GEMM, so
+// that the GEMM's B operands are plain 32-byte LDS reads.  The source is
+// the BF16 LDS tile X or, with XG, the FP32 descriptor rows in HBM.
+template <bool XG>
+__device__ inline void quantize_rows(const uint16_t *X, int ld, const float *G, int gw, int grow0, int w, int R,
+                                     const float *rinv, uint8_t *Q, int ldq)
+{
+  const int per = w / 8;
+  for (int q = threadIdx.x; q < R * per; q += 256) {
+    const int r = q / per, c = (q - r * per) * 8;
+    float v[8];
+    if constexpr (XG) {
+      const float4 *p = reinterpret_cast<const float4 *>(G + (size_t) (grow0 + r) * gw + c);
+      const float4 a = p[0], b = p[1];
+      v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+    } else {
+      const uint4 u = *reinterpret_cast<const uint4 *>(X + r * ld + c);
+      v[0] = __uint_as_float(u.x << 16), v[1] = __uint_as_float(u.x & 0xffff0000u);
+      v[2] = __uint_as_float(u.y << 16), v[3] = __uint_as_float(u.y & 0xffff0000u);
+      v[4] = __uint_as_float(u.z << 16), v[5] = __uint_as_float(u.z & 0xffff0000u);
+      v[6] = __uint_as_float(u.w << 16), v[7] = __uint_as_float(u.w & 0xffff0000u);
+    }
+    const float ri = rinv[r];
 #pragma unroll
-    for (int q = 0; q < 8; q++) v[q] *= ri;
-    return pack_fp8(v);
+    for (int i = 0; i < 8; i++) v[i] *= ri;
+    *reinterpret_cast<long *>(Q + r * ldq + c) = pack_fp8(v);
   }
 }
 
@@ -518,9 +568,8 @@ __device__ inline typename Frag<MODE>::A cvt_graw(const GRaw<MODE> &g, float ri)
     f32x8 xv = {g.lo.x, g.lo.y, g.lo.z, g.lo.w, g.hi.x, g.hi.y, g.hi.z, g.hi.w};
     return split8(xv);
   } else {
-    float v[8] = {g.lo.x * ri, g.lo.y * ri, g.lo.z * ri, g.lo.w * ri,
-                  g.hi.x * ri, g.hi.y * ri, g.hi.z * ri, g.hi.w * ri};
-    return pack_fp8(v);
+    static_assert(MODE != 2, "FP8 layer-0 operands are quantized into LDS first");
+    return {};
   }
 }
 
@@ -539,7 +588,9 @@ __device__ inline f32x4 mma(typename Frag<MODE>::A a, typename Frag<MODE>::A b, 
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, c, 0, 0, 0);
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, c, 0, 0, 0);
   } else {
-    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
+    // e4m3 A and B (format 0), unit E8M0 block scales (127 = 2^0): the
+    // per-channel and per-row FP32 scales are applied in the epilogue
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 0, 0, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f);
   }
 }
 
@@ -703,6 +754,11 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
   const int cq = 4 * (lane >> 4);  // first of the lane's 4 output columns in a tile
   const int rq = lane & 15;        // the lane's row in a row tile
   float *rs = reinterpret_cast<float *>(smem_raw + mp.offRS);  // FP8 row scales [2][R]
+  uint8_t *Q = smem_raw + mp.offQ;                              // FP8 B operand rows [R][ldq]
+  unsigned *rmax = reinterpret_cast<unsigned *>(smem_raw + mp.offRS) + 2 * R;  // FP8 row maxima [R]
+  if constexpr (MODE == 2) {
+    if (threadIdx.x < R) rmax[threadIdx.x] = 0u;
+  }
 
   // ---------------- forward ----------------
   for (int l = 0; l < L - 1; l++) {
@@ -717,10 +773,17 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
     const int ldi = mp.ld[l];
     const int NT = N / 16;
     if constexpr (MODE == 2) {
+      // layer 0: a pass over the descriptor rows; later layers: the maxima
+      // the previous epilogue recorded
       if (l == 0)
         row_scales<true>(nullptr, 0, Gd, gw, row0, K, R, rs);
       else
-        row_scales<false>((const uint16_t *) Hin, ldi, nullptr, 0, 0, K, R, rs);
+        scales_from_max(rmax, R, rs);
+      __syncthreads();
+      if (l == 0)
+        quantize_rows<true>(nullptr, 0, Gd, gw, row0, K, R, rs + R, Q, mp.ldq);
+      else
+        quantize_rows<false>((const uint16_t *) Hin, ldi, nullptr, 0, 0, K, R, rs + R, Q, mp.ldq);
       __syncthreads();
     }
     const float *wsc = MODE == 2 ? mp.swf[l] + (size_t) e * N : nullptr;
@@ -729,7 +792,9 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
       const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
       if (ntw == 0) continue;
       f32x4 acc[4][RT];
-      if (l == 0)
+      if constexpr (MODE == 2)
+        gemm_n<MODE, R, false, uint8_t>(W, t0, ntw, Q, mp.ldq, nullptr, gw, row0, K, rs + R, acc);
+      else if (l == 0)
         gemm_n<MODE, R, true, T>(W, t0, ntw, nullptr, 0, Gd, gw, row0, K, rs + R, acc);
       else
         gemm_n<MODE, R, false, T>(W, t0, ntw, Hin, ldi, nullptr, gw, row0, K, rs + R, acc);
@@ -757,6 +822,9 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
           }
           st4(H + row * ldo + c0, h);
           if (mp.storeD) st4(D + row * ldo + c0, dz);
+          if constexpr (MODE == 2) {
+            if (l < L - 2) note_max(rmax, row, h);
+          }
         }
       }
     }
@@ -805,7 +873,14 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
     T *gH = sm + mp.offGH;
     const int NT = K / 16;
     if constexpr (MODE == 2) {
-      row_scales<false>((const uint16_t *) Gl, ldg, nullptr, 0, 0, N, R, rs);
+      // the seeded gradient: a pass over its rows; later ones: the maxima
+      // the previous backward epilogue recorded
+      if (l == L - 2)
+        row_scales<false>((const uint16_t *) Gl, ldg, nullptr, 0, 0, N, R, rs);
+      else
+        scales_from_max(rmax, R, rs);
+      __syncthreads();
+      quantize_rows<false>((const uint16_t *) Gl, ldg, nullptr, 0, 0, N, R, rs + R, Q, mp.ldq);
       __syncthreads();
     }
     const float *wsc = MODE == 2 ? mp.swb[l] + (size_t) e * K : nullptr;
@@ -814,7 +889,10 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
       const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
       if (ntw == 0) continue;
       f32x4 acc[4][RT];
-      gemm_n<MODE, R, false, T>(W, t0, ntw, Gl, ldg, nullptr, gw, row0, N, rs + R, acc);
+      if constexpr (MODE == 2)
+        gemm_n<MODE, R, false, uint8_t>(W, t0, ntw, Q, mp.ldq, nullptr, gw, row0, N, rs + R, acc);
+      else
+        gemm_n<MODE, R, false, T>(W, t0, ntw, Gl, ldg, nullptr, gw, row0, N, rs + R, acc);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j >= ntw) continue;
@@ -845,6 +923,7 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
 #pragma unroll
             for (int i = 0; i < 4; i++) o[i] = v[i] * (mp.storeD ? d[i] : 1.0f - d[i] * d[i]);
             st4(D + row * ld + c0, o);
+            if constexpr (MODE == 2) note_max(rmax, row, o);
             if (mp.res[l - 1]) st4(gH + row * mp.ldGH + c0, v);
           }
         }
@@ -920,10 +999,12 @@ __global__ void k_quant_fp8(const float *W, const float *swf, const float *swb, 
   const size_t rem = i - (size_t) el * per;
   const int k = (int) (rem / N), n = (int) (rem - (size_t) k * N);
   const float w = W[i];
-  // forward operand: output n (tile n/16, lane row n%16), inner k (step k/32,
-  // lane group (k%32)/8, byte k%8); backward: output k, inner n
-  const size_t f = (((size_t) (n / 16) * (K / 32) + k / 32) * 64 + ((k % 32) / 8) * 16 + n % 16) * 8 + k % 8;
-  const size_t b = (((size_t) (k / 16) * (N / 32) + n / 32) * 64 + ((n % 32) / 8) * 16 + k % 16) * 8 + n % 8;
+  // forward operand: output n (tile n/16, lane row n%16), inner k (step
+  // k/128, lane group (k%128)/32, byte k%32); backward: output k, inner n.
+  // The B operands (quantize_rows, ld_x) place inner index k of a step at
+  // the same lane group and byte, so the MFMA pairs equal k.
+  const size_t f = (((size_t) (n / 16) * (K / 128) + k / 128) * 64 + ((k % 128) / 32) * 16 + n % 16) * 32 + k % 32;
+  const size_t b = (((size_t) (k / 16) * (N / 128) + n / 128) * 64 + ((n % 128) / 32) * 16 + k % 16) * 32 + n % 32;
   Af[(size_t) el * per + f] =
       (uint8_t) (__builtin_amdgcn_cvt_pk_fp8_f32(w / swf[(size_t) el * N + n], 0.0f, 0, false) & 0xff);
   Ab[(size_t) el * per + b] =
@@ -931,10 +1012,11 @@ __global__ void k_quant_fp8(const float *W, const float *swf, const float *swb, 
 }
 }  // namespace
 
-// every hidden width a multiple of 16 and every GEMM inner size a multiple of 2 KS
+// every hidden width a multiple of 16 and every GEMM inner size a multiple of
+// two fast-path steps (FP8: of one 128-wide step)
 bool mlp_fast_ok(const MlpParams &mp, int mode)
 {
-  const int ks2 = mode == 0 ? 32 : 64;  // two fast-path steps
+  const int ks2 = mode == 0 ? 32 : (mode == 2 ? 128 : 64);
   for (int l = 0; l < mp.nl - 1; l++) {
     if (mp.sizes[l + 1] % 16 || mp.sizes[l] % 16) return false;
     if (mp.sizes[l] % ks2 || mp.sizes[l + 1] % ks2) return false;
@@ -987,7 +1069,17 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
   if (anyres) off += (size_t) R * m.ldGH;
   size_t bytes = (off * esz + 15) & ~(size_t) 15;
   m.offRS = (int) bytes;
-  if (mode == 2) bytes += 2 * (size_t) R * sizeof(float);
+  m.offQ = 0;
+  m.ldq = 0;
+  if (mode == 2) {
+    bytes += 3 * (size_t) R * sizeof(float);  // scales, inverse scales, recorded maxima
+    // quantized B rows: the widest GEMM inner size, rows 16-byte aligned
+    int maxk = 0;
+    for (int l = 0; l < mp.nl; l++) maxk = std::max(maxk, mp.sizes[l]);
+    m.ldq = maxk + 16;
+    m.offQ = (int) ((bytes + 15) & ~(size_t) 15);
+    bytes = (size_t) m.offQ + (size_t) R * m.ldq;
+  }
   if (out) *out = m;
   return bytes;
 }

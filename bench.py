@@ -216,9 +216,11 @@ def _force_errors(got, ref, n):
                 max_abs_F=float(abs(ref["f"]).max()))
 
 
-def fp8_experiment(steps, warmup):
+def fp8_experiment(steps, warmup, keep=None):
     """1,024,000-atom 4-element BCC alloy, MLP 256-256-128-1: FP32, BF16 and
-    FP8 e4m3 MLPs side by side, with the device memory each system holds."""
+    FP8 e4m3 MLPs side by side, with the device memory each system holds.
+    keep (a dict) receives the model, the start and end positions and each
+    path's forces there, for a check against the FP64 oracle."""
     from tensormd_amd import TensorMD, device_memory, workloads as W
     symbols = ["Mo", "Nb", "Ta", "W"]
     masses = [95.95, 92.906, 180.95, 183.84]
@@ -230,10 +232,11 @@ def fp8_experiment(steps, warmup):
     v = W.maxwell_boltzmann(np.array(masses)[t - 1], 600.0, seed=10)
     n = len(x)
     out = dict(natoms=n, steps=steps, mlp="256-256-128-1 tanh",
-               errors_note="BF16 and FP8 forces and energies against the FP32 GPU path at the "
-                           "same positions: the starting state, and the FP32 run's state after "
-                           "warmup + steps (the FP32 path itself is within 1e-5 eV/A and 1e-7 "
-                           "eV/atom of the FP64 oracle on these models)")
+               errors_note="errors_vs_fp32_*: BF16 and FP8 forces and energies against the FP32 "
+                           "GPU path at the same positions: the starting state, and the FP32 run's "
+                           "state after warmup + steps; errors_vs_fp64_oracle_* (added by the CPU "
+                           "leg when it runs): every path against the FP64 oracle at those "
+                           "positions")
     ref = {}
     for prec in ("fp32", "bf16", "fp8"):
         m = TensorMD(model, symbols, prec, 0.001, device=0)
@@ -256,11 +259,15 @@ def fp8_experiment(steps, warmup):
             x_end = s.get()["x"]
             s.set_positions(x_end)
             ref = dict(start=r0, end=s.compute(), x_end=x_end)
+            r1 = ref["end"]
         else:
             s.set_positions(ref["x_end"])
             r1 = s.compute()
             out[prec]["errors_vs_fp32_start"] = _force_errors(r0, ref["start"], n)
             out[prec]["errors_vs_fp32_end"] = _force_errors(r1, ref["end"], n)
+        if keep is not None:
+            keep.update(model=model, x=x, t=t, cell=cell, x_end=ref["x_end"])
+            keep[prec] = dict(start=r0, end=r1)
         print(f"bench: c4 {prec}: {out[prec]}", flush=True)
         s.close()
         m.close()
@@ -442,8 +449,9 @@ def main():
         print(f"bench: list mode {json.dumps(listmode)}", flush=True)
 
     fp8 = None
+    c4_keep = {}
     if args.c4_steps > 0:
-        fp8 = fp8_experiment(args.c4_steps, 20)
+        fp8 = fp8_experiment(args.c4_steps, 20, c4_keep)
         print("bench: c4 " + json.dumps({k: dict(ms_per_step=v["ms_per_step"], mlp_ms=v["mlp_ms"])
                                          for k, v in fp8.items() if isinstance(v, dict) and "mlp_ms" in v}),
               flush=True)
@@ -496,6 +504,22 @@ def main():
         oracle.set_threads(threads)
         cpu["c1_2048_atoms_atom_steps_per_s"] = c1_rates
         print(f"bench: cpu baseline {cpu}", flush=True)
+        # the C4 experiment's paths against the FP64 oracle (the checker) at
+        # the start and end positions: one single-point evaluation each
+        if fp8 is not None and "model" in c4_keep:
+            o4 = oracle.OracleModel(c4_keep["model"], 0.001)
+            t0 = time.perf_counter()
+            refs = dict(start=o4.compute_periodic(c4_keep["x"], c4_keep["t"] - 1, c4_keep["cell"]),
+                        end=o4.compute_periodic(c4_keep["x_end"], c4_keep["t"] - 1, c4_keep["cell"]))
+            n4 = len(c4_keep["x"])
+            for prec in ("fp32", "bf16", "fp8"):
+                if prec in c4_keep and prec in fp8:
+                    for at in ("start", "end"):
+                        fp8[prec][f"errors_vs_fp64_oracle_{at}"] = _force_errors(c4_keep[prec][at], refs[at], n4)
+            fp8["oracle_check_s"] = time.perf_counter() - t0
+            print("bench: c4 vs FP64 oracle " + json.dumps(
+                {p: {k: v for k, v in fp8[p].items() if k.startswith("errors_vs_fp64")}
+                 for p in ("fp32", "bf16", "fp8") if p in fp8}), flush=True)
 
     if "bf16" not in res or "fp32" not in res:
         print(json.dumps({"partial": True, "rates": {k: v["rate"] for k, v in res.items()},

@@ -475,26 +475,110 @@ __device__ inline typename Frag<MODE>::A ld_x(const T *X, int ld, int r0, int k,
   }
 }
 
-// FP8: the row scales of the next GEMM's B operand from the row maxima that
-// the producing epilogue recorded (rmax, float bits; |x| >= 0 orders as
-// unsigned), which are then cleared for the next producer.  Threads 0..R-1.
-__device__ inline void scales_from_max(unsigned *rmax, int R, float *rs)
+// 8 consecutive values of row r at column c: the FP32 descriptor rows in
+// HBM (XG) or a BF16 LDS tile
+template <bool XG>
+__device__ inline void ld8_row(const uint16_t *X, int ld, const float *G, int gw, int grow0, int r, int c,
+                               float v[8])
 {
-  const int r = threadIdx.x;
-  if (r < R) {
-    const float m = __uint_as_float(rmax[r]);
-    const float sc = m > 0.0f ? m * (1.0f / 440.0f) : 1.0f;
-    rs[r] = sc;
-    rs[R + r] = 1.0f / sc;
-    rmax[r] = 0u;
+  if constexpr (XG) {
+    const float4 *p = reinterpret_cast<const float4 *>(G + (size_t) (grow0 + r) * gw + c);
+    const float4 a = p[0], b = p[1];
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+  } else {
+    const uint4 u = *reinterpret_cast<const uint4 *>(X + r * ld + c);
+    v[0] = __uint_as_float(u.x << 16), v[1] = __uint_as_float(u.x & 0xffff0000u);
+    v[2] = __uint_as_float(u.y << 16), v[3] = __uint_as_float(u.y & 0xffff0000u);
+    v[4] = __uint_as_float(u.z << 16), v[5] = __uint_as_float(u.z & 0xffff0000u);
+    v[6] = __uint_as_float(u.w << 16), v[7] = __uint_as_float(u.w & 0xffff0000u);
   }
 }
 
-// the largest |v| of 4 values into the row's recorded maximum
-__device__ inline void note_max(unsigned *rmax, int row, const float v[4])
+// FP8, layer 0: the row scales of the descriptor rows and their quantized
+// copy in one read of HBM: each thread holds its up to 8 x 8 values of the
+// row in registers (all loads in flight) while the threads of the row find
+// the maximum.  Returns false (nothing done) when a row does not fit.
+__device__ inline bool quantize_g_rows(const float *G, int gw, int grow0, int width, int R, float *rs, uint8_t *Q,
+                                       int ldq)
 {
-  const float m = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
-  atomicMax(rmax + row, __float_as_uint(m));
+  constexpr int kMaxChunks = 8;
+  const int tpr = 256 / R;
+  if (width > kMaxChunks * tpr * 8) return false;
+  const int r = threadIdx.x / tpr, part = threadIdx.x - r * tpr;
+  float v[kMaxChunks][8];
+  float m = 0.0f;
+#pragma unroll
+  for (int i = 0; i < kMaxChunks; i++) {
+    const int c = (part + i * tpr) * 8;
+    if (c < width) {
+      ld8_row<true>(nullptr, 0, G, gw, grow0, r, c, v[i]);
+#pragma unroll
+      for (int j = 0; j < 8; j++) m = fmaxf(m, fabsf(v[i][j]));
+    }
+  }
+  for (int o = 1; o < tpr; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+  const float sc = m > 0.0f ? m * (1.0f / 440.0f) : 1.0f, ri = 1.0f / sc;
+  if (part == 0) {
+    rs[r] = sc;
+    rs[R + r] = ri;
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxChunks; i++) {
+    const int c = (part + i * tpr) * 8;
+    if (c < width) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[i][j] *= ri;
+      *reinterpret_cast<long *>(Q + r * ldq + c) = pack_fp8(v[i]);
+    }
+  }
+  return true;
+}
+
+// FP8: the BF16 LDS rows [0, R) of width w quantized into Q with the row
+// scales taken from the maxima that the producing epilogue recorded (rin,
+// float bits; |x| >= 0 orders as unsigned); rs receives the scales and their
+// inverses for the GEMM's epilogue, and rnext (the other buffer, written by
+// this GEMM's epilogue) is cleared.
+__device__ inline void quantize_rows_max(const uint16_t *X, int ld, int w, int R, const unsigned *rin,
+                                         unsigned *rnext, float *rs, uint8_t *Q, int ldq)
+{
+  const int tpr = 256 / R;
+  const int r = threadIdx.x / tpr, part = threadIdx.x - r * tpr;
+  const float m = __uint_as_float(rin[r]);
+  const float sc = m > 0.0f ? m * (1.0f / 440.0f) : 1.0f, ri = 1.0f / sc;
+  if (part == 0) {
+    rs[r] = sc;
+    rs[R + r] = ri;
+    rnext[r] = 0u;
+  }
+#pragma unroll 4
+  for (int c = part * 8; c < w; c += tpr * 8) {
+    float v[8];
+    ld8_row<false>(X, ld, nullptr, 0, 0, r, c, v);
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] *= ri;
+    *reinterpret_cast<long *>(Q + r * ldq + c) = pack_fp8(v);
+  }
+}
+
+__device__ inline float max_abs4(const float v[4])
+{
+  return fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
+}
+
+// A wave's per-lane row maxima (row r * 16 + lane % 16 of each row tile r)
+// into the recorded maxima: the four lanes of a row combine first, then one
+// LDS atomic per row and wave.
+template <int RT>
+__device__ inline void flush_max(unsigned *rmax, const float (&mx)[RT])
+{
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < RT; r++) {
+    float m = fmaxf(mx[r], __shfl_xor(mx[r], 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    if (lane < 16) atomicMax(rmax + r * 16 + lane, __float_as_uint(m));
+  }
 }
 
 // FP8: rows [0, R) of width w (a multiple of 8) divided by their row scale
@@ -505,22 +589,13 @@ template <bool XG>
 __device__ inline void quantize_rows(const uint16_t *X, int ld, const float *G, int gw, int grow0, int w, int R,
                                      const float *rinv, uint8_t *Q, int ldq)
 {
-  const int per = w / 8;
-  for (int q = threadIdx.x; q < R * per; q += 256) {
-    const int r = q / per, c = (q - r * per) * 8;
+  const int tpr = 256 / R;
+  const int r = threadIdx.x / tpr, part = threadIdx.x - r * tpr;
+  const float ri = rinv[r];
+#pragma unroll 4
+  for (int c = part * 8; c < w; c += tpr * 8) {
     float v[8];
-    if constexpr (XG) {
-      const float4 *p = reinterpret_cast<const float4 *>(G + (size_t) (grow0 + r) * gw + c);
-      const float4 a = p[0], b = p[1];
-      v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
-    } else {
-      const uint4 u = *reinterpret_cast<const uint4 *>(X + r * ld + c);
-      v[0] = __uint_as_float(u.x << 16), v[1] = __uint_as_float(u.x & 0xffff0000u);
-      v[2] = __uint_as_float(u.y << 16), v[3] = __uint_as_float(u.y & 0xffff0000u);
-      v[4] = __uint_as_float(u.z << 16), v[5] = __uint_as_float(u.z & 0xffff0000u);
-      v[6] = __uint_as_float(u.w << 16), v[7] = __uint_as_float(u.w & 0xffff0000u);
-    }
-    const float ri = rinv[r];
+    ld8_row<XG>(X, ld, G, gw, grow0, r, c, v);
 #pragma unroll
     for (int i = 0; i < 8; i++) v[i] *= ri;
     *reinterpret_cast<long *>(Q + r * ldq + c) = pack_fp8(v);
@@ -597,6 +672,9 @@ __device__ inline f32x4 mma(typename Frag<MODE>::A a, typename Frag<MODE>::A b, 
 // FP8: per-row scale of the next GEMM's B operand (the rows of X, or of the
 // descriptor rows G): rs[r] = max_c |x[r][c]| / 440 (1 for a zero row),
 // rs[R + r] = 1 / rs[r]; every value then fits e4m3 (|x| <= 448).
+// Threads per row (256 / R consecutive threads) each take 8 consecutive
+// values at a time; the loops are unrolled so that several loads are in
+// flight (the descriptor rows come from HBM).
 template <bool XG>
 __device__ inline void row_scales(const uint16_t *X, int ld, const float *G, int gw, int grow0,
                                   int width, int R, float *rs)
@@ -604,8 +682,13 @@ __device__ inline void row_scales(const uint16_t *X, int ld, const float *G, int
   const int tpr = 256 / R;  // threads per row: 16, 8 or 4 consecutive lanes
   const int r = threadIdx.x / tpr, part = threadIdx.x - r * tpr;
   float m = 0.0f;
-  for (int c = part; c < width; c += tpr)
-    m = fmaxf(m, fabsf(XG ? G[(size_t) (grow0 + r) * gw + c] : bf2f(X[r * ld + c])));
+#pragma unroll 4
+  for (int c = part * 8; c < width; c += tpr * 8) {
+    float v[8];
+    ld8_row<XG>(X, ld, G, gw, grow0, r, c, v);
+#pragma unroll
+    for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(v[i]));
+  }
   for (int o = 1; o < tpr; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
   if (part == 0) {
     const float sc = m > 0.0f ? m * (1.0f / 440.0f) : 1.0f;
@@ -755,9 +838,11 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
   const int rq = lane & 15;        // the lane's row in a row tile
   float *rs = reinterpret_cast<float *>(smem_raw + mp.offRS);  // FP8 row scales [2][R]
   uint8_t *Q = smem_raw + mp.offQ;                              // FP8 B operand rows [R][ldq]
-  unsigned *rmax = reinterpret_cast<unsigned *>(smem_raw + mp.offRS) + 2 * R;  // FP8 row maxima [R]
+  // FP8 row maxima [2][R]: GEMM g (forward layer l: g = l; backward layer
+  // l: g = 2 L - 3 - l) reads buffer g % 2, its epilogue fills the other
+  unsigned *rmax = reinterpret_cast<unsigned *>(smem_raw + mp.offRS) + 2 * R;
   if constexpr (MODE == 2) {
-    if (threadIdx.x < R) rmax[threadIdx.x] = 0u;
+    if (threadIdx.x < 2 * R) rmax[threadIdx.x] = 0u;
   }
 
   // ---------------- forward ----------------
@@ -772,18 +857,17 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
     const T *Hin = l > 0 ? sm + mp.offH[l - 1] : nullptr;
     const int ldi = mp.ld[l];
     const int NT = N / 16;
+    unsigned *rnext = rmax + ((l + 1) & 1) * R;  // FP8: maxima of this layer's output
     if constexpr (MODE == 2) {
       // layer 0: a pass over the descriptor rows; later layers: the maxima
       // the previous epilogue recorded
-      if (l == 0)
+      if (l > 0) {
+        quantize_rows_max((const uint16_t *) Hin, ldi, K, R, rmax + (l & 1) * R, rnext, rs, Q, mp.ldq);
+      } else if (!quantize_g_rows(Gd, gw, row0, K, R, rs, Q, mp.ldq)) {
         row_scales<true>(nullptr, 0, Gd, gw, row0, K, R, rs);
-      else
-        scales_from_max(rmax, R, rs);
-      __syncthreads();
-      if (l == 0)
+        __syncthreads();
         quantize_rows<true>(nullptr, 0, Gd, gw, row0, K, R, rs + R, Q, mp.ldq);
-      else
-        quantize_rows<false>((const uint16_t *) Hin, ldi, nullptr, 0, 0, K, R, rs + R, Q, mp.ldq);
+      }
       __syncthreads();
     }
     const float *wsc = MODE == 2 ? mp.swf[l] + (size_t) e * N : nullptr;
@@ -792,6 +876,9 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
       const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
       if (ntw == 0) continue;
       f32x4 acc[4][RT];
+      float mx[RT];  // FP8: the lane's largest |h| per row tile
+#pragma unroll
+      for (int r = 0; r < RT; r++) mx[r] = 0.0f;
       if constexpr (MODE == 2)
         gemm_n<MODE, R, false, uint8_t>(W, t0, ntw, Q, mp.ldq, nullptr, gw, row0, K, rs + R, acc);
       else if (l == 0)
@@ -822,10 +909,11 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
           }
           st4(H + row * ldo + c0, h);
           if (mp.storeD) st4(D + row * ldo + c0, dz);
-          if constexpr (MODE == 2) {
-            if (l < L - 2) note_max(rmax, row, h);
-          }
+          if constexpr (MODE == 2) mx[r] = fmaxf(mx[r], max_abs4(h));
         }
+      }
+      if constexpr (MODE == 2) {
+        if (l < L - 2) flush_max<RT>(rnext, mx);
       }
     }
     __syncthreads();
@@ -872,15 +960,19 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
     const int ldg = mp.ld[l + 1];
     T *gH = sm + mp.offGH;
     const int NT = K / 16;
+    const int g = 2 * L - 3 - l;                  // FP8: this GEMM's index
+    unsigned *rnext = rmax + ((g + 1) & 1) * R;  // FP8: maxima of G_{l-1}
     if constexpr (MODE == 2) {
       // the seeded gradient: a pass over its rows; later ones: the maxima
       // the previous backward epilogue recorded
-      if (l == L - 2)
+      if (l == L - 2) {
         row_scales<false>((const uint16_t *) Gl, ldg, nullptr, 0, 0, N, R, rs);
-      else
-        scales_from_max(rmax, R, rs);
-      __syncthreads();
-      quantize_rows<false>((const uint16_t *) Gl, ldg, nullptr, 0, 0, N, R, rs + R, Q, mp.ldq);
+        if (threadIdx.x < R) rnext[threadIdx.x] = 0u;
+        __syncthreads();
+        quantize_rows<false>((const uint16_t *) Gl, ldg, nullptr, 0, 0, N, R, rs + R, Q, mp.ldq);
+      } else {
+        quantize_rows_max((const uint16_t *) Gl, ldg, N, R, rmax + (g & 1) * R, rnext, rs, Q, mp.ldq);
+      }
       __syncthreads();
     }
     const float *wsc = MODE == 2 ? mp.swb[l] + (size_t) e * K : nullptr;
@@ -889,6 +981,9 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
       const int ntw = t0 < NT ? min(4, (NT - t0 + 3) / 4) : 0;
       if (ntw == 0) continue;
       f32x4 acc[4][RT];
+      float mx[RT];  // FP8: the lane's largest |G| per row tile
+#pragma unroll
+      for (int r = 0; r < RT; r++) mx[r] = 0.0f;
       if constexpr (MODE == 2)
         gemm_n<MODE, R, false, uint8_t>(W, t0, ntw, Q, mp.ldq, nullptr, gw, row0, N, rs + R, acc);
       else
@@ -923,10 +1018,13 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
 #pragma unroll
             for (int i = 0; i < 4; i++) o[i] = v[i] * (mp.storeD ? d[i] : 1.0f - d[i] * d[i]);
             st4(D + row * ld + c0, o);
-            if constexpr (MODE == 2) note_max(rmax, row, o);
+            if constexpr (MODE == 2) mx[r] = fmaxf(mx[r], max_abs4(o));
             if (mp.res[l - 1]) st4(gH + row * mp.ldGH + c0, v);
           }
         }
+      }
+      if constexpr (MODE == 2) {
+        if (l > 0) flush_max<RT>(rnext, mx);
       }
     }
     __syncthreads();
@@ -1072,7 +1170,7 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
   m.offQ = 0;
   m.ldq = 0;
   if (mode == 2) {
-    bytes += 3 * (size_t) R * sizeof(float);  // scales, inverse scales, recorded maxima
+    bytes += 4 * (size_t) R * sizeof(float);  // scales, inverse scales, two buffers of recorded maxima
     // quantized B rows: the widest GEMM inner size, rows 16-byte aligned
     int maxk = 0;
     for (int l = 0; l < mp.nl; l++) maxk = std::max(maxk, mp.sizes[l]);
@@ -1094,8 +1192,10 @@ hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const 
   const size_t limit = 160 * 1024;
   // 32-row tiles by default: several workgroups per CU hide the LDS/L2
   // latency better than one 64-row tile, at twice the (L2-resident) weight
-  // traffic.  rtile (16/32/64) overrides the first choice.
-  int Rs[3] = {32, 16, 16};
+  // traffic.  FP8: 16-row tiles (its 32-byte K=128 operands need 152 VGPRs
+  // at 32 rows, 3 waves per SIMD; 128 and 4 waves at 16 rows, 2.5% faster
+  // on the 1M-atom C4 system).  rtile (16/32/64) overrides the first choice.
+  int Rs[3] = {mode == 2 ? 16 : 32, 16, 16};
   if (rtile == 16 || rtile == 32 || rtile == 64) Rs[0] = rtile;
   for (int q = 0; q < 3; q++) {
     int R = Rs[q];

@@ -181,6 +181,7 @@ void PairTensorMDAMD::coeff(int narg, char **arg)
       if (v == "fp32") opt.mlp_precision = TMD_MLP_FP32;
       else if (v == "bf16") opt.mlp_precision = TMD_MLP_BF16;
       else if (v == "fp8") opt.mlp_precision = TMD_MLP_FP8;
+      else if (v == "bf16x3") opt.mlp_precision = TMD_MLP_BF16X3;
       else error->all(FLERR, "Unknown option for precision");
       i += 2;
       read_elt = false;

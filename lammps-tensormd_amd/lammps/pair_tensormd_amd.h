@@ -6,8 +6,8 @@
      pair_style tensormd/amd
      pair_coeff * * model.npz El1 El2 ... [interp dx] [etemp T] [cmax exact|default]
    With `suffix amd` (or -sf amd) an input that says `pair_style tensormd`
-   selects this class.  Extra keyword: `precision fp32|bf16` (MLP GEMMs;
-   default fp32).
+   selects this class.  Extra keyword: `precision fp32|bf16|bf16x3|fp8`
+   (MLP GEMMs; default fp32).
 ------------------------------------------------------------------------- */
 
 #ifdef PAIR_CLASS

@@ -661,6 +661,28 @@ def test_bf16x3_mlp_error(synth_models):
     assert errs["bf16x3"]["dF"] <= 0.1 * errs["bf16"]["dF"]
 
 
+def _guard_atoms(ref, G_gpu, x, cell, k, nelt, rc):
+    """Atoms whose forces hang on the reference's |G_0| < 1e-8 guard
+    (dE/dG_0 is dropped below it: gpu/cuda/kernel.cu:754-759): every atom
+    with a G_0 that the FP32 device path and the FP64 oracle put on
+    different sides of the threshold (G_0 = P_0 sums radial values that can
+    cancel to within FP32 rounding of 1e-8), and its neighbors within rc,
+    which receive that atom's pair forces.  There the two force fields
+    differ by the whole dropped term; elsewhere the guard acts alike."""
+    g64 = np.abs(ref["G"].reshape(len(x), nelt, k, 4)[..., 0]) < 1e-8
+    g32 = np.abs(G_gpu.reshape(len(x), nelt, k, 4)[..., 0]) < 1e-8
+    near = np.where((g64 != g32).any(axis=(1, 2)))[0]
+    mask = np.zeros(len(x), bool)
+    inv = np.linalg.inv(cell)
+    for a in near:
+        d = x - x[a]
+        fr = d @ inv
+        fr -= np.round(fr)
+        d = fr @ cell
+        mask |= (d * d).sum(1) < (rc + 1e-6) ** 2
+    return near, mask
+
+
 def test_c2_trajectory_errors_and_drift(tmp_path):
     """The 128,000-atom BCC benchmark system (same model, lattice and velocity
     seeds as bench.py): forces, energies and virials of the BF16 path along
@@ -699,17 +721,27 @@ def test_c2_trajectory_errors_and_drift(tmp_path):
             ep_b.append(ep)
             ek_b.append(ek)
         got = sb.get()
-        ref = om.compute_periodic(got["x"], t - 1, cell)
+        ref = om.compute_periodic(got["x"], t - 1, cell, want_desc=True)
         probe.set_positions(got["x"])
         gf = probe.compute()
         probe3.set_positions(got["x"])
         g3 = probe3.compute()
+        # forces compared away from the reference's G_0 guard (its atoms are
+        # reported; they are a handful per 10^5)
+        Gp, _ = probe.descriptors(mf.info()["width"])
+        near, gmask = _guard_atoms(ref, Gp, got["x"], cell, 24, 4, om.rc)
+        keep = ~gmask
         eb, ef, e3 = _errs(got, ref, n), _errs(gf, ref, n), _errs(g3, ref, n)
+        for e, g in ((eb, got), (ef, gf), (e3, g3)):
+            e["dF_all"] = e["dF"]
+            e["dF"] = float(abs(g["f"][keep] - ref["f"][keep]).max())
         maxf = float(abs(ref["f"]).max())
         rms = float(np.sqrt(((got["f"] - ref["f"]) ** 2).mean()))
-        say(f"step {250 * seg}: max|F| {maxf:.3f} bf16 {eb} rms {rms:.2e} bf16x3 {e3} fp32 {ef}")
+        say(f"step {250 * seg}: max|F| {maxf:.3f} guard atoms {list(near)} ({int(gmask.sum())} excluded) "
+            f"bf16 {eb} rms {rms:.2e} bf16x3 {e3} fp32 {ef}")
+        assert gmask.sum() <= 0.005 * n
         assert ef["dF"] <= F_TOL and ef["dE"] <= E_TOL and ef["dV"] <= 1e-6
-        # the north-star bar, and a regression bar 10x above the measured
+        # the 5e-3 eV/A target, and a regression bar 10x above the measured
         # split-BF16 error (max 4.7e-5 eV/A at the hottest checkpoint)
         assert e3["dF"] <= 5e-3 and e3["dF"] <= 5e-4 and e3["dE"] <= 1e-5
         if seg == 0:

@@ -656,8 +656,16 @@ def test_bf16x3_mlp_error(synth_models):
         m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], prec, 0.001)
         errs[prec] = _errs(m.system(x, t, cell, skin=1.0).compute(), ref, len(x))
         m.close()
+    # the same structure through list mode (the LAMMPS path)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, om.rc + 1.0)
+    tall = t[owner]
+    ref_l = om.compute_list(len(x), xall, tall - 1, ilist, numneigh, flat)
+    m = TensorMD(synth_models["c2"], ["Mo", "Nb", "Ta", "W"], "bf16x3", 0.001)
+    errs["bf16x3_list"] = _errs(m.compute_list(len(x), xall, tall, ilist, numneigh, flat), ref_l, len(x))
+    m.close()
     say("errors", errs)
-    assert errs["bf16x3"]["dF"] <= 5e-4 and errs["bf16x3"]["dE"] <= 1e-5
+    for key in ("bf16x3", "bf16x3_list"):
+        assert errs[key]["dF"] <= 5e-4 and errs[key]["dE"] <= 1e-5
     assert errs["bf16x3"]["dF"] <= 0.1 * errs["bf16"]["dF"]
 
 

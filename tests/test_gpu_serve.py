@@ -70,7 +70,7 @@ def test_serve_fp32_vs_oracle(synth_models):
         assert dEa <= 2e-6
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp32"])
 def test_serve_alone_equals_mixed(synth_models, precision):
     say("case: serving alone vs in the mix,", precision)
     probe = _structures(6, seed=51, nmax=2000)

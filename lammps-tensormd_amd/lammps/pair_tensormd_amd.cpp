@@ -11,6 +11,7 @@
 #include "atom.h"
 #include "comm.h"
 #include "error.h"
+#include "fix_nve_tensormd_amd.h"
 #include "force.h"
 #include "memory.h"
 #include "neigh_list.h"
@@ -42,6 +43,7 @@ PairTensorMDAMD::PairTensorMDAMD(LAMMPS *lmp) : Pair(lmp)
   satom = aatom = nullptr;
   maxsatom = 0;
   scale = nullptr;
+  resident = nullptr;
 }
 
 PairTensorMDAMD::~PairTensorMDAMD()
@@ -82,14 +84,29 @@ void PairTensorMDAMD::fail(int code, const char *where)
    the last neighbor build (neighbor->lastcall): between builds only the
    positions go to the device (the reference repacks and uploads the whole
    list every step, gpu/cuda/kernel.cu:1500-1521). */
-void PairTensorMDAMD::compute(int eflag, int vflag)
+void PairTensorMDAMD::send_etemp()
 {
-  ev_init(eflag, vflag);
   if (etemp != etemp_sent) {  // fix tensormd/etemp/amd or extract("etemp") changed it
     int trc = tmd_set_electron_temperature(ctx, etemp);
     if (trc != TMD_OK) fail(trc, "pair tensormd/amd electron temperature");
     etemp_sent = etemp;
   }
+}
+
+void PairTensorMDAMD::compute(int eflag, int vflag)
+{
+  ev_init(eflag, vflag);
+  if (resident) {
+    // fix nve/tensormd/amd evaluated this step's forces on the device; f
+    // reaches the host at output steps, energy and virial here
+    if (vflag_atom || cvflag_atom)
+      error->all(FLERR, "pair tensormd/amd: per-atom virials are not available while fix nve/tensormd/amd runs");
+    if (scale[1][1] != 1.0)
+      error->all(FLERR, "pair tensormd/amd: fix adapt scaling is not available while fix nve/tensormd/amd runs");
+    resident->pair_results(eflag_global, vflag_global, eflag_atom ? eatom : nullptr, eng_vdwl, virial);
+    return;
+  }
+  send_etemp();
   const int nlocal = atom->nlocal;
   const int nall = nlocal + atom->nghost;
   double energy = 0.0;

@@ -28,6 +28,8 @@ struct tmd_ctx;
 
 namespace LAMMPS_NS {
 
+class FixNVETensorMDAMD;
+
 class PairTensorMDAMD : public Pair {
  public:
   explicit PairTensorMDAMD(class LAMMPS *);
@@ -54,6 +56,11 @@ class PairTensorMDAMD : public Pair {
   // the pair_coeff arguments after "* *" (model file, symbols, keywords),
   // for compute tensormd/density/amd, which loads the model itself
   const std::vector<std::string> &coeff_args() const { return kwargs; }
+  // pass a changed etemp to the device
+  void send_etemp();
+  // set by fix nve/tensormd/amd for the length of a run: positions and
+  // forces are on the device, compute() reports the device results
+  FixNVETensorMDAMD *resident;
 
  protected:
   tmd_ctx *ctx;

@@ -1,5 +1,6 @@
 /* Registers pair_style tensormd/amd, compute tensormd/density/amd,
-   compute tensormd/tdpe/amd and fix tensormd/etemp/amd with a
+   compute tensormd/tdpe/amd, fix tensormd/etemp/amd and
+   fix nve/tensormd/amd with a
    LAMMPS binary built with PKG_PLUGIN (src/lammpsplugin.h:17-39,
    src/PLUGIN/plugin.cpp:98-122 of the reference tree):
    `plugin load tensormd_amd_plugin.so`. */
@@ -9,6 +10,7 @@
 
 #include "compute_tensormd_density_amd.h"
 #include "compute_tensormd_tdpe_amd.h"
+#include "fix_nve_tensormd_amd.h"
 #include "fix_tensormd_etemp_amd.h"
 #include "pair_tensormd_amd.h"
 
@@ -32,6 +34,11 @@ static Compute *tensormd_tdpe_amd_creator(LAMMPS *lmp, int narg, char **arg)
 static Fix *tensormd_etemp_amd_creator(LAMMPS *lmp, int narg, char **arg)
 {
   return new FixTensorMDElectronTemperatureAMD(lmp, narg, arg);
+}
+
+static Fix *nve_tensormd_amd_creator(LAMMPS *lmp, int narg, char **arg)
+{
+  return new FixNVETensorMDAMD(lmp, narg, arg);
 }
 
 extern "C" void lammpsplugin_init(void *lmp, void *handle, void *regfunc)
@@ -62,5 +69,10 @@ extern "C" void lammpsplugin_init(void *lmp, void *handle, void *regfunc)
   plugin.name = "tensormd/etemp/amd";
   plugin.info = "electron temperature of pair_style tensormd/amd from the thermo temperature";
   plugin.creator.v2 = (lammpsplugin_factory2 *) &tensormd_etemp_amd_creator;
+  (*reg)(&plugin, lmp);
+
+  plugin.name = "nve/tensormd/amd";
+  plugin.info = "velocity-Verlet NVE with x, v and f kept on the MI355X (pair_style tensormd/amd)";
+  plugin.creator.v2 = (lammpsplugin_factory2 *) &nve_tensormd_amd_creator;
   (*reg)(&plugin, lmp);
 }

@@ -847,6 +847,51 @@ def test_md_run_grows_list_and_rolls_back(synth_models):
     assert abs(out["energy"] - ref["energy"]) / len(x) <= 1e-6 * max(1.0, abs(ref["energy"]) / len(x))
 
 
+def test_lammps_fix_call_sequence(synth_models):
+    """fix nve/tensormd/amd drives the device system as LAMMPS calls it: at
+    each run's setup a system is made from the host state (positions wrapped
+    into the box, the wraps counted in image flags, as LAMMPS' setup leaves
+    them), every step is one tmd_md_run step, thermo steps read the energy,
+    and the end of a run copies x and v back.  Two runs of 25 steps agree
+    with one continuous 50-step run and with the oracle's NVE."""
+    say("case: LAMMPS fix call sequence, C1 2048 atoms, 2 runs x 25 steps")
+    om = oracle.OracleModel(synth_models["c1"], 0.001)
+    x, t, cell = W.crystal("fcc", 3.615, (8, 8, 8), sigma=0.05, seed=25)
+    v = W.maxwell_boltzmann(np.full(len(x), om.masses[0]), 300.0, seed=26)
+    nsteps, every = 50, 5
+    ref = om.nve(x, v, t - 1, cell, 0.001, nsteps)
+    m = TensorMD(synth_models["c1"], ["Cu"], "fp32", 0.001)
+    L = np.diag(cell)
+    hx, hv, img = x.copy(), v.copy(), np.zeros_like(x)
+    epot = []
+    for run in range(2):
+        wraps = np.floor(hx / L)
+        hx, img = hx - wraps * L, img + wraps
+        s = m.system(hx, t, cell, skin=1.0)
+        s.set_velocities(hv)
+        for step in range(nsteps // 2):
+            s.md_run(1, 0.001, history=False)
+            if (step + 1) % every == 0:
+                epot.append(s.get()["energy"])
+        out = s.get()
+        hx, hv = out["x"], out["v"]
+        s.close()
+    cont = m.system(x, t, cell, skin=1.0)
+    cont.set_velocities(v)
+    ep_c, _ = cont.md_run(nsteps, 0.001)
+    end = cont.get()
+    n = len(x)
+    epot = np.array(epot)
+    dx = float(np.abs(hx + img * L - end["x"]).max())
+    de_c = float(np.abs(epot - ep_c[every - 1::every]).max()) / n
+    de_o = float(np.abs(epot - ref["epot"][every::every]).max()) / n
+    say("max |dx| vs one run", dx, "max |dE|/atom vs one run", de_c, "vs oracle", de_o)
+    assert dx <= 1e-6
+    assert float(np.abs(hv - end["v"]).max()) <= 1e-5
+    assert de_c <= 1e-7
+    assert de_o <= 1e-6
+
+
 def test_fp8_mlp_error(tmp_path):
     """FP8 e4m3 MLP (weights with per-output-channel scales, activations and
     gradients with per-row scales, FP32 accumulation) on the 256-256-128-1

@@ -17,7 +17,7 @@ LMP_SRC = "/root/reference/src"
                     shutil.which("g++") is None, reason="LAMMPS headers not available")
 @pytest.mark.parametrize("src", ["pair_tensormd_amd.cpp", "compute_tensormd_density_amd.cpp",
                                  "compute_tensormd_tdpe_amd.cpp", "fix_tensormd_etemp_amd.cpp",
-                                 "tensormd_amd_plugin.cpp"])
+                                 "fix_nve_tensormd_amd.cpp", "tensormd_amd_plugin.cpp"])
 def test_adaptor_compiles(src):
     here = os.path.join(ROOT, "lammps-tensormd_amd", "lammps")
     cmd = ["g++", "-std=c++17", "-fsyntax-only", "-I" + LMP_SRC, "-I" + os.path.join(LMP_SRC, "STUBS"),

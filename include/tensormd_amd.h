@@ -78,6 +78,10 @@ typedef struct tmd_options {
                         compatibility; padding does not exist here, so it
                         has no effect on results. */
   int device;        /* HIP device ordinal; -1 = current device */
+  int table_device;  /* 1: build the radial table on the device at load, as
+                        forward_fnn_GPU (gpu/cuda/nnp.cu:59-116) does; 0
+                        (default): on the host.  Both in FP64; they differ
+                        only in the rounding of exp/log1p/tanh/cos/pow. */
 } tmd_options;
 
 /* Fill `opt` with defaults: FP32, interp_dx 0.001, cmax exact, device -1. */
@@ -103,6 +107,11 @@ int tmd_type_map(const tmd_ctx *ctx, int *map);
  * model file (max_moment 0..5; a model with fewer than 4 moments is evaluated
  * with zero first-layer weights on the moment channels it lacks) */
 int tmd_model_info(const tmd_ctx *ctx, int *info5);
+/* The FP64 radial table the context uploaded (FeaturePotential::setup_ration,
+ * fnn.cpp:77-105): n grid points x num_filters x {y, dy/dr slope, c1, c2}
+ * (the Kerley form of ration.hpp:166-225), step dx.  Writes *n and *dx and,
+ * when tab is non-NULL and cap >= n * num_filters * 4, the table. */
+int tmd_radial_table(const tmd_ctx *ctx, double *tab, long cap, int *n, double *dx);
 
 /* ---- Pair::compute drop-in ------------------------------------------------
  * x[nall][3] (local + ghost), type[nall] (1-based), full neighbor list

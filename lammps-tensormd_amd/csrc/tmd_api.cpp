@@ -527,6 +527,7 @@ void tmd_options_default(tmd_options *opt)
   opt->interp_dx = 0.001;
   opt->cmax_exact = 1;
   opt->device = -1;
+  opt->table_device = 0;
 }
 
 const char *tmd_last_error(const tmd_ctx *ctx)
@@ -558,7 +559,7 @@ int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes
   }
 
   std::string err;
-  if (!load_model(npz_path, ctx->opt.interp_dx, ctx->model, err)) {
+  if (!load_model(npz_path, ctx->opt.interp_dx, ctx->model, err, ctx->opt.table_device == 0)) {
     set_err(nullptr, err);
     return TMD_ERR_MODEL;
   }
@@ -589,6 +590,10 @@ int tmd_create(const char *npz_path, const char *const *elem_symbols, int ntypes
     if ((e = hipGetDevice(&dev)) != hipSuccess) return hip_fail(nullptr, e, "hipGetDevice");
   }
   ctx->device = dev;
+  if (ctx->opt.table_device && !build_table_device(ctx->model, ctx->opt.interp_dx, err)) {
+    set_err(nullptr, err);
+    return TMD_ERR_DEVICE;
+  }
   if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess)
     return hip_fail(nullptr, e, "hipStreamCreate");
   {
@@ -656,6 +661,18 @@ int tmd_model_info(const tmd_ctx *ctx, int *info)
   info[2] = ctx->model.max_moment_file;
   info[3] = ctx->model.gwidth_file();
   info[4] = ctx->model.tdnp ? ctx->model.tdH[0].nlayers : ctx->model.mlp[0].nlayers;
+  return TMD_OK;
+}
+
+int tmd_radial_table(const tmd_ctx *ctx, double *tab, long cap, int *n, double *dx)
+{
+  if (!ctx || !n || !dx) return TMD_ERR_ARG;
+  const tmd::Model &m = ctx->model;
+  *n = m.tab_n;
+  *dx = m.dx;
+  if (!tab) return TMD_OK;
+  if (cap < (long) m.tab.size()) return TMD_ERR_CAPACITY;
+  std::copy(m.tab.begin(), m.tab.end(), tab);
   return TMD_OK;
 }
 

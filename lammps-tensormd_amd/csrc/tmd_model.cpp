@@ -14,13 +14,38 @@
 
 namespace tmd {
 
+double softplus_threshold()
+{
+  return std::log(2.22045e-16) + 2.0;  // nnp.cu:30
+}
+
+void table_grid(Model &m, double delta, std::vector<double> &xs)
+{
+  xs.clear();
+  int n;
+  if (m.fp32_model) {
+    // the reference computes the grid in float for precision-32 models
+    float d = (float) delta, rcf = (float) m.rc;
+    n = (int) (rcf / d) + 1;
+    for (int i = 0; i < n; i++) xs.push_back((double) ((float) i * d));
+    m.dx = (double) d;
+    m.xdx = (double) (1.0f / d);
+  } else {
+    n = (int) (m.rc / delta) + 1;
+    for (int i = 0; i < n; i++) xs.push_back((double) i * delta);
+    m.dx = delta;
+    m.xdx = 1.0 / delta;
+  }
+  m.tab_n = n;
+}
+
 namespace {
 
 double act_value(int act, double z)
 {
   switch (act) {
     case ACT_SOFTPLUS: {
-      const double thr = std::log(2.22045e-16) + 2.0;  // nnp.cu:30
+      const double thr = softplus_threshold();
       return (z < -thr) ? std::log1p(std::exp(z)) : z;
     }
     case ACT_SQUAREPLUS: return 0.5 * (z + std::sqrt(z * z + 4.0));
@@ -147,22 +172,8 @@ void pad_moments(Model &m)
 void build_table(Model &m, double delta)
 {
   std::vector<double> xs;
-  int n;
-  if (m.fp32_model) {
-    // the reference computes the grid in float for precision-32 models
-    float d = (float) delta, rcf = (float) m.rc;
-    n = (int) (rcf / d) + 1;
-    for (int i = 0; i < n; i++) xs.push_back((double) ((float) i * d));
-    m.dx = (double) d;
-    m.xdx = (double) (1.0f / d);
-  } else {
-    n = (int) (m.rc / delta) + 1;
-    for (int i = 0; i < n; i++) xs.push_back((double) i * delta);
-    m.dx = delta;
-    m.xdx = 1.0 / delta;
-  }
-  const int k = m.k;
-  m.tab_n = n;
+  table_grid(m, delta, xs);
+  const int n = m.tab_n, k = m.k;
   m.tab.assign((size_t) n * k * 4, 0.0);
   auto at = [&](int i, int j, int q) -> double & { return m.tab[((size_t) i * k + j) * 4 + q]; };
   std::vector<double> y(k);
@@ -199,7 +210,7 @@ void build_table(Model &m, double delta)
 
 }  // namespace
 
-bool load_model(const std::string &path, double dx, Model &m, std::string &err)
+bool load_model(const std::string &path, double dx, Model &m, std::string &err, bool host_table)
 {
   Npz z;
   if (!read_npz(path, z, err)) return false;
@@ -348,7 +359,12 @@ bool load_model(const std::string &path, double dx, Model &m, std::string &err)
     if (m.max_moment < 3) pad_moments(m);
     if (!(dx > 0)) throw std::string("interp dx must be > 0");
     if (m.rc / dx > 4.0e6) throw std::string("interp dx too small");
-    build_table(m, dx);
+    if (host_table) {
+      build_table(m, dx);
+    } else {
+      std::vector<double> xs;
+      table_grid(m, dx, xs);
+    }
   } catch (const std::string &e) {
     err = e;
     return false;

@@ -68,8 +68,24 @@ struct Model {
   int gwidth_file() const { return nelt * k * (max_moment_file + 1); }
 };
 
-// Reads `path`, validates the schema and builds T, slots and the table for
-// interpolation step `dx`.  Returns false with a message on failure.
-bool load_model(const std::string &path, double dx, Model &m, std::string &err);
+// Reads `path`, validates the schema and builds T, slots and (host_table)
+// the table for interpolation step `dx`; without host_table only the grid
+// (tab_n, dx, xdx) is set and the table is left to build_table_device.
+// Returns false with a message on failure.
+bool load_model(const std::string &path, double dx, Model &m, std::string &err, bool host_table = true);
+
+// The table's grid points r_i (FeaturePotential::setup_ration, fnn.cpp:81-86):
+// sets m.tab_n, m.dx and m.xdx.
+void table_grid(Model &m, double delta, std::vector<double> &xs);
+
+// Below -threshold softplus(z) = log1p(exp(z)), else z (nnp.cu:30).
+double softplus_threshold();
+
+// The table [n][k][4] = {y, s, c1, c2} for step `delta`, built on the
+// current device in FP64 (the radial network or analytic functions, the
+// cutoff and the Kerley coefficients: the work forward_fnn_GPU,
+// gpu/cuda/nnp.cu:59-116, and BatchRation1d::setup, ration.hpp:166-225, do
+// at model load) and copied to m.tab.
+bool build_table_device(Model &m, double delta, std::string &err);
 
 }  // namespace tmd

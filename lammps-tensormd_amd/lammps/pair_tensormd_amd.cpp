@@ -193,6 +193,13 @@ void PairTensorMDAMD::coeff(int narg, char **arg)
       else error->all(FLERR, "Unknown option for cmax");
       i += 2;
       read_elt = false;
+    } else if (key == "table") {  // where the radial table is built at load
+      const std::string v = value();
+      if (v == "host") opt.table_device = 0;
+      else if (v == "device") opt.table_device = 1;
+      else error->all(FLERR, "Unknown option for table");
+      i += 2;
+      read_elt = false;
     } else if (key == "precision") {
       const std::string v = value();
       if (v == "fp32") opt.mlp_precision = TMD_MLP_FP32;

@@ -6,8 +6,9 @@
      pair_style tensormd/amd
      pair_coeff * * model.npz El1 El2 ... [interp dx] [etemp T] [cmax exact|default]
    With `suffix amd` (or -sf amd) an input that says `pair_style tensormd`
-   selects this class.  Extra keyword: `precision fp32|bf16|bf16x3|fp8`
-   (MLP GEMMs; default fp32).
+   selects this class.  Extra keywords: `precision fp32|bf16|bf16x3|fp8`
+   (MLP GEMMs; default fp32) and `table host|device` (where the radial
+   table is built at load; default host).
 ------------------------------------------------------------------------- */
 
 #ifdef PAIR_CLASS

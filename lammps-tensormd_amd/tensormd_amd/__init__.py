@@ -34,7 +34,8 @@ class TmdError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("mlp_precision", ctypes.c_int), ("interp_dx", ctypes.c_double),
-                ("cmax_exact", ctypes.c_int), ("device", ctypes.c_int)]
+                ("cmax_exact", ctypes.c_int), ("device", ctypes.c_int),
+                ("table_device", ctypes.c_int)]
 
 
 class ServeOptions(ctypes.Structure):
@@ -85,6 +86,7 @@ def lib():
     L.tmd_masses.argtypes = [_vp, _dp]
     L.tmd_type_map.argtypes = [_vp, _ip]
     L.tmd_model_info.argtypes = [_vp, _ip]
+    L.tmd_radial_table.argtypes = [_vp, _dp, ctypes.c_long, _ip, _dp]
     L.tmd_compute_list.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int, _ip,
                                    _ip, ctypes.POINTER(_ip), ctypes.c_double, _dp, _dp, _dp, _dp,
                                    _dp]
@@ -179,7 +181,7 @@ class TensorMD:
     """One model on one device (PairTensorMD::coeff + read_potential_file)."""
 
     def __init__(self, npz_path, symbols, precision="fp32", interp_dx=0.001, device=-1,
-                 cmax_exact=True):
+                 cmax_exact=True, table_device=False):
         L = lib()
         opt = Options()
         L.tmd_options_default(ctypes.byref(opt))
@@ -187,6 +189,7 @@ class TensorMD:
         opt.interp_dx = float(interp_dx)
         opt.device = int(device)
         opt.cmax_exact = 1 if cmax_exact else 0
+        opt.table_device = 1 if table_device else 0
         arr = (ctypes.c_char_p * len(symbols))(*[s.encode() if s else None for s in symbols])
         h = _vp()
         rc = L.tmd_create(str(npz_path).encode(), arr, len(symbols), ctypes.byref(opt),
@@ -212,6 +215,15 @@ class TensorMD:
         rc = ctypes.c_double()
         _check(lib().tmd_cutoff(self.h, ctypes.byref(rc)), self.h)
         return rc.value
+
+    def radial_table(self):
+        """(table[n][num_filters][4] = y, slope, c1, c2 in FP64, dx)."""
+        n, dx = ctypes.c_int(), ctypes.c_double()
+        _check(lib().tmd_radial_table(self.h, None, 0, ctypes.byref(n), ctypes.byref(dx)), self.h)
+        tab = np.zeros((n.value, self.info()["num_filters"], 4))
+        _check(lib().tmd_radial_table(self.h, _p(tab, _dp), tab.size, ctypes.byref(n),
+                                      ctypes.byref(dx)), self.h)
+        return tab, dx.value
 
     def masses(self):
         m = np.zeros(self.ntypes + 1)

@@ -516,6 +516,48 @@ def test_analytic_radial_models(tmp_path, method):
     assert e["dF"] <= F_TOL and e["dE"] <= E_TOL and e["dV"] <= 1e-6
 
 
+@pytest.mark.parametrize("case", ["W_k32", "MoNi_squareplus_500k", "W_k128", "tanh_poly", "pexp",
+                                  "morse", "density", "sf"])
+def test_device_table_build(goldens, tmp_path, case):
+    """The radial table built on the device at load (table_device = 1, the
+    work of forward_fnn_GPU, gpu/cuda/nnp.cu:59-116, and the Kerley set-up,
+    ration.hpp:166-225) against the host FP64 build: values, slopes and
+    Kerley coefficients, then forces and energy of the same structure."""
+    say("case: device table build,", case)
+    if case in goldens["cases"]:
+        c = goldens["cases"][case]
+        path, symbols, dx = model_path(c["model"]), c.get("symbols") or ["W"], c["interp"]
+    elif case == "W_k128":
+        path, symbols, dx = model_path("W_k128_3H_ver_2c.npz"), ["W"], 0.001
+    else:
+        kw = dict(fctype=1, seed=47) if case == "tanh_poly" else dict(radial=case, seed=70)
+        path = W.write_model(str(tmp_path / f"{case}.npz"), symbols=["Mo", "W"], masses=[95.95, 183.84],
+                             rmax=5.5, num_filters=16, layer_sizes=[64, 32, 1], actfn="tanh",
+                             radial_scale=0.2, **kw)
+        symbols, dx = ["Mo", "W"], 0.001
+    host = TensorMD(path, symbols, "fp32", dx)
+    dev = TensorMD(path, symbols, "fp32", dx, table_device=True)
+    th, dxh = host.radial_table()
+    td, dxd = dev.radial_table()
+    assert th.shape == td.shape and dxh == dxd
+    d = [float(abs(td[..., q] - th[..., q]).max()) for q in range(4)]
+    say("max |difference| of y, slope, c1, c2:", d, "max |y|", float(abs(th[..., 0]).max()))
+    # y differs by the rounding of the elementary functions (allowed: 16 ulps
+    # of max |y|); the slope is a difference of two values over dx, and c1,
+    # c2 are differences of at most three slopes over dx, so their bounds
+    # grow by 2/dx and a further 4/dx
+    u = 16 * np.finfo(np.float64).eps * float(abs(th[..., 0]).max())
+    assert d[0] <= u and d[1] <= 2 * u / dxh and max(d[2], d[3]) <= 8 * u / dxh ** 2
+    nelt = len(symbols)
+    x, t, cell = W.crystal("bcc", 3.2, (4, 4, 4), nelt=nelt, sigma=0.08, seed=91)
+    a = host.system(x, t, cell, skin=1.0).compute()
+    b = dev.system(x, t, cell, skin=1.0).compute()
+    fmax = max(1.0, float(abs(a["f"]).max()))
+    say("max |dF|", float(abs(a["f"] - b["f"]).max()), "|dE|", abs(a["energy"] - b["energy"]))
+    assert float(abs(a["f"] - b["f"]).max()) <= 1e-6 * fmax
+    assert abs(a["energy"] - b["energy"]) <= 1e-9 * max(1.0, abs(a["energy"]))
+
+
 @pytest.mark.parametrize("mm,k,nelt", [(4, 24, 4), (5, 16, 2), (5, 32, 3), (4, 128, 1)])
 def test_high_moment_models(tmp_path, mm, k, nelt):
     """max_moment 4 and 5 models (35 and 56 monomials, tensormd.cpp:234-273;

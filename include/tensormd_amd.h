@@ -201,6 +201,14 @@ int tmd_system_get(tmd_system *sys, double *x, double *v, double *f, double *eat
  * rebuilds; epot/ekin[nsteps] receive per-step energies if non-NULL
  * (read back once at the end).  */
 int tmd_md_run(tmd_system *sys, int nsteps, double dt, double *epot, double *ekin);
+/* One step of tmd_md_run enqueued without waiting for the device: the
+ * per-step call of a host loop (fix nve/tensormd/amd, in place of the
+ * initial/final_integrate pair of FixNVE, fix_nve.cpp:64-130, around
+ * Verlet's force call), so that the host's own work overlaps the step.  The
+ * step's neighbor-list capacity check, and any device error, are taken up
+ * by the next call on sys; a step whose list overflowed is then run again
+ * from the state before it with a larger list, as tmd_md_run does. */
+int tmd_md_step_async(tmd_system *sys, double dt);
 /* Run nsteps MD steps with HIP events recorded on the library stream
  * between phases; ms[0..6] = mean ms per step spent in: neighbor
  * maintenance, descriptor kernel, MLP kernel, force kernel, FP64
@@ -223,7 +231,8 @@ int tmd_system_kernel_timing(const tmd_system *sys, double *ms, double *kept, lo
 /* Replay each MD step as one captured HIP graph (default 1) or launch its
  * kernels one by one (0); results are the same either way. */
 int tmd_system_set_graph(tmd_system *sys, int enable);
-/* Number of neighbor-list builds done so far. */
+/* Number of neighbor-list builds done so far (up to the last step whose
+ * check was taken up; see tmd_md_step_async). */
 int tmd_system_nbuilds(const tmd_system *sys);
 /* Export the current full neighbor list (cutoff rc + skin): off[natoms+1],
  * and up to cap entries of (j, image shift s0 s1 s2).  Returns the total

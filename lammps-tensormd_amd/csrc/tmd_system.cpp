@@ -43,6 +43,13 @@ struct tmd_system {
   double tsum[3] = {0, 0, 0};    // descriptor, MLP, force kernel ms summed over the samples
   double tkept_sum = 0;          // kept neighbors summed over the samples (all atoms)
   long long tsamples = 0;
+  // a step of tmd_md_step_async whose list-capacity check is still open:
+  // the control words are copied into pinned memory behind it, and the
+  // event marks the copy; the next call on the system resolves it
+  bool pending = false;
+  double pending_dt = 0;
+  hipEvent_t pend_ev = nullptr;
+  PinnedBuffer pend_ctl;
 };
 
 namespace tmd {
@@ -166,13 +173,10 @@ int sys_forces(tmd_system *s, PhaseEvents *pe = nullptr)
   return e == hipSuccess ? TMD_OK : hip_fail(s->ctx, e, "kept count");
 }
 
-// overflow check after the queued work; grows the list and asks for a rebuild
-int check_overflow(tmd_system *s, bool *overflow)
+// the control words read after the queued work: an overflow grows the list
+// and asks for a rebuild
+int apply_overflow(tmd_system *s, const unsigned long long *ctl, bool *overflow)
 {
-  unsigned long long ctl[4];
-  hipError_t e = hipMemcpyAsync(ctl, s->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, s->ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
-  if (e != hipSuccess) return hip_fail(s->ctx, e, "device step");
   s->nbuilds_seen = (int) ctl[3];
   *overflow = ctl[2] != 0;
   if (*overflow) {
@@ -190,6 +194,41 @@ int check_overflow(tmd_system *s, bool *overflow)
       return hip_fail(s->ctx, ce, "list control reset");
   }
   return TMD_OK;
+}
+
+// overflow check after the queued work (waits for it)
+int check_overflow(tmd_system *s, bool *overflow)
+{
+  unsigned long long ctl[4];
+  hipError_t e = hipMemcpyAsync(ctl, s->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, s->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "device step");
+  return apply_overflow(s, ctl, overflow);
+}
+
+int md_run_impl(tmd_system *s, int nsteps, double dt, double *epot, double *ekin, bool defer);
+
+// Closes the open step of tmd_md_step_async: waits for it and, if its list
+// overflowed, restores the snapshot taken before it and runs it again with
+// the grown list.
+int resolve_pending(tmd_system *s)
+{
+  if (!s->pending) return TMD_OK;
+  s->pending = false;
+  if (hipError_t e = hipEventSynchronize(s->pend_ev); e != hipSuccess) return hip_fail(s->ctx, e, "device step");
+  unsigned long long ctl[4];
+  std::memcpy(ctl, s->pend_ctl.p, sizeof(ctl));
+  bool over = false;
+  int rc = apply_overflow(s, ctl, &over);
+  if (rc != TMD_OK || !over) return rc;
+  const size_t xb = sizeof(double4) * (size_t) s->n, vb = 24 * (size_t) s->n;
+  unsigned char *snap = (unsigned char *) s->snap.p;
+  hipStream_t st = s->ctx->stream;
+  hipError_t e = hipMemcpyAsync(s->w.pos.p, snap, xb, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->vel.p, snap + xb, vb, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->w.f.p, snap + xb + vb, vb, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return hip_fail(s->ctx, e, "snapshot restore");
+  return md_run_impl(s, 1, s->pending_dt, nullptr, nullptr, false);
 }
 
 }  // namespace
@@ -309,6 +348,7 @@ void tmd_system_destroy(tmd_system *s)
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   if (s->step_graph) hipGraphExecDestroy(s->step_graph);
+  if (s->pend_ev) hipEventDestroy(s->pend_ev);
   for (auto &pe : s->tev)
     for (auto &ev : pe.ev) hipEventDestroy(ev);
   delete s;
@@ -319,6 +359,7 @@ int tmd_system_set_positions(tmd_system *s, const double *x)
   if (!s || !x) return TMD_ERR_ARG;
   if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
     return hip_fail(s->ctx, de, "hipSetDevice");
+  if (int rc = resolve_pending(s); rc != TMD_OK) return rc;
   std::vector<double4> pos(s->n);
   hipStream_t st = s->ctx->stream;
   hipError_t e = hipMemcpyAsync(pos.data(), s->w.pos.p, sizeof(double4) * s->n, hipMemcpyDeviceToHost, st);
@@ -343,6 +384,7 @@ int tmd_system_set_velocities(tmd_system *s, const double *v)
   if (!s || !v) return TMD_ERR_ARG;
   if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
     return hip_fail(s->ctx, de, "hipSetDevice");
+  if (int rc = resolve_pending(s); rc != TMD_OK) return rc;
   hipError_t e = hipMemcpy(s->vel.p, v, sizeof(double) * 3 * (size_t) s->n, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "set velocities");
   return TMD_OK;
@@ -353,6 +395,7 @@ int tmd_system_compute(tmd_system *s)
   if (!s) return TMD_ERR_ARG;
   if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
     return hip_fail(s->ctx, de, "hipSetDevice");
+  if (int rc = resolve_pending(s); rc != TMD_OK) return rc;
   for (int attempt = 0; attempt < 4; attempt++) {
     int rc = sys_forces(s);
     if (rc != TMD_OK) return rc;
@@ -373,6 +416,7 @@ int tmd_system_get(tmd_system *s, double *x, double *v, double *f, double *eatom
   if (!s) return TMD_ERR_ARG;
   if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
     return hip_fail(s->ctx, de, "hipSetDevice");
+  if (int rc = resolve_pending(s); rc != TMD_OK) return rc;
   hipStream_t st = s->ctx->stream;
   std::vector<double4> pos;
   double red[7];
@@ -405,6 +449,29 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
   if (!s || nsteps < 0 || !(dt > 0)) return TMD_ERR_ARG;
   if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
     return hip_fail(s->ctx, de, "hipSetDevice");
+  if (int rc = resolve_pending(s); rc != TMD_OK) return rc;
+  return md_run_impl(s, nsteps, dt, epot, ekin, false);
+}
+
+int tmd_md_step_async(tmd_system *s, double dt)
+{
+  if (!s || !(dt > 0)) return TMD_ERR_ARG;
+  if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
+    return hip_fail(s->ctx, de, "hipSetDevice");
+  if (int rc = resolve_pending(s); rc != TMD_OK) return rc;
+  // sampled kernel timing needs the synchronous loop
+  return md_run_impl(s, 1, dt, nullptr, nullptr, s->time_every == 0);
+}
+
+}  // extern "C"
+
+namespace {
+
+// nsteps of velocity Verlet in run segments; defer (one step, no history):
+// the step's capacity check is left open (resolve_pending) instead of
+// waiting for it
+int md_run_impl(tmd_system *s, int nsteps, double dt, double *epot, double *ekin, bool defer)
+{
   hipStream_t st = s->ctx->stream;
   const Model &m = s->ctx->model;
   hipError_t e;
@@ -504,6 +571,25 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
     int nrun = std::min(kSeg, nsteps - done);
     if ((e = copy3(true)) != hipSuccess) return hip_fail(s->ctx, e, "snapshot");
     bool over = true;
+    if (defer) {
+      // one step behind its snapshot; the control words follow it into
+      // pinned memory, read by resolve_pending
+      if (!s->pend_ev && (e = hipEventCreateWithFlags(&s->pend_ev, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(s->ctx, e, "step event");
+      if ((e = s->pend_ctl.ensure(4 * sizeof(unsigned long long))) != hipSuccess)
+        return hip_fail(s->ctx, e, "step control staging");
+      if (s->use_graph)
+        e = hipGraphLaunch(s->step_graph, st);
+      else if (int rc = enqueue_step(nullptr); rc != TMD_OK)
+        return rc;
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(s->pend_ctl.p, s->ctl.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipEventRecord(s->pend_ev, st);
+      if (e != hipSuccess) return hip_fail(s->ctx, e, "step launch");
+      s->pending = true;
+      s->pending_dt = dt;
+      return TMD_OK;
+    }
     for (int attempt = 0; attempt < 4 && over; attempt++) {
       if (want_hist) {
         // stream-ordered clear (this loop is never captured into a graph)
@@ -582,6 +668,10 @@ int tmd_md_run(tmd_system *s, int nsteps, double dt, double *epot, double *ekin)
   return TMD_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 int tmd_system_set_kernel_timing(tmd_system *s, int every)
 {
   if (!s || every < 0) return TMD_ERR_ARG;
@@ -608,6 +698,7 @@ int tmd_system_set_graph(tmd_system *s, int enable)
   s->use_graph = enable != 0;
   return TMD_OK;
 }
+
 
 int tmd_md_profile(tmd_system *s, int nsteps, double dt, double *ms)
 {
@@ -685,6 +776,7 @@ int tmd_system_neighbors(tmd_system *s, long cap, long *off, int *j, int *shift,
   if (!s || !off || !total) return TMD_ERR_ARG;
   if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
     return hip_fail(s->ctx, de, "hipSetDevice");
+  if (int rc = resolve_pending(s); rc != TMD_OK) return rc;
   std::vector<int> nn(s->n);
   std::vector<uint32_t> list((size_t) s->n * s->nb.cap);
   hipError_t e = hipMemcpy(nn.data(), s->w.nn.p, 4 * (size_t) s->n, hipMemcpyDeviceToHost);
@@ -714,6 +806,7 @@ int tmd_system_descriptors(tmd_system *s, float *G, float *dEdG)
   if (!s) return TMD_ERR_ARG;
   if (hipError_t de = hipSetDevice(s->ctx->device); de != hipSuccess)
     return hip_fail(s->ctx, de, "hipSetDevice");
+  if (int rc = resolve_pending(s); rc != TMD_OK) return rc;
   int gw = s->ctx->model.gwidth();
   int nr = s->w.rl.nrows;
   std::vector<float> g((size_t) nr * gw), dg((size_t) nr * gw);

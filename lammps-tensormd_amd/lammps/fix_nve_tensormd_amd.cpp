@@ -6,9 +6,10 @@
    setup():             a tmd_system (the pair style's context) from the
                         host x, v and types; the pair style switches to
                         reporting device results
-   initial_integrate(): one tmd_md_run step: v += dtf f/m, x += dt v, the
-                        device neighbor check (and rebuild), forces, energy
-                        and virial at the new positions, v += dtf f/m
+   initial_integrate(): one device step enqueued (tmd_md_step_async):
+                        v += dtf f/m, x += dt v, the device neighbor check
+                        (and rebuild), forces, energy and virial at the new
+                        positions, v += dtf f/m
    pair compute:        PairTensorMDAMD::compute asks pair_results() for the
                         energy, virial and per-atom energies of the step
    end_of_step():       on the steps LAMMPS' Output writes (output->next),
@@ -149,7 +150,10 @@ void FixNVETensorMDAMD::release()
 void FixNVETensorMDAMD::initial_integrate(int /*vflag*/)
 {
   pair->send_etemp();  // a temperature changed through extract("etemp")
-  const int rc = tmd_md_run(sys, 1, dtv, nullptr, nullptr);
+  // enqueued without waiting: LAMMPS' host work of the step overlaps it,
+  // and the next call on the system (pair_results, a write-back or the
+  // next step) takes up its list check
+  const int rc = tmd_md_step_async(sys, dtv);
   if (rc != TMD_OK) device_fail(rc, "step");
 }
 

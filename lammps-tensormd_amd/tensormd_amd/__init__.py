@@ -99,6 +99,7 @@ def lib():
     L.tmd_system_create.argtypes = [_vp, ctypes.c_int, _dp, _ip, _dp, ctypes.c_double,
                                     ctypes.POINTER(_vp)]
     L.tmd_system_destroy.argtypes = [_vp]
+    L.tmd_md_step_async.argtypes = [_vp, ctypes.c_double]
     L.tmd_system_set_positions.argtypes = [_vp, _dp]
     L.tmd_system_set_velocities.argtypes = [_vp, _dp]
     L.tmd_system_compute.argtypes = [_vp]
@@ -370,6 +371,10 @@ class System:
         _check(lib().tmd_system_get(self.h, _p(x, _dp), _p(v, _dp), _p(f, _dp), _p(ea, _dp),
                                     ctypes.byref(e), _p(vir, _dp)), self.model.h)
         return dict(x=x, v=v, f=f, eatom=ea, energy=e.value, virial=vir)
+
+    def step_async(self, dt):
+        """One MD step enqueued without waiting (tmd_md_step_async)."""
+        _check(lib().tmd_md_step_async(self.h, float(dt)), self.model.h)
 
     def md_run(self, nsteps, dt, history=True):
         ep = np.zeros(nsteps) if history else None

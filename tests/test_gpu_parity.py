@@ -859,11 +859,14 @@ def test_graph_replay_matches_eager(synth_models):
     assert np.abs(out[0][2] - out[1][2]).max() < 1e-9
 
 
-def test_md_run_grows_list_and_rolls_back(synth_models):
+@pytest.mark.parametrize("stepping", ["md_run", "async"])
+def test_md_run_grows_list_and_rolls_back(synth_models, stepping):
     """A converging flow raises the local density past the neighbor-list
     capacity during the run: the run must grow the list, rerun the segment
-    from its snapshot and end with the forces of a fresh evaluation."""
-    say("case: list overflow during md_run, C1 2048 atoms, converging flow")
+    from its snapshot and end with the forces of a fresh evaluation.  With
+    one tmd_md_step_async call per step the overflowed step is run again
+    when the next call takes up its check."""
+    say(f"case: list overflow during {stepping}, C1 2048 atoms, converging flow")
     x, t, cell = W.crystal("fcc", 3.615, (8, 8, 8), sigma=0.05, seed=27)
     L = np.diag(cell)
     v = -1.5 * (x - 0.5 * L)  # A/ps toward the box center
@@ -873,7 +876,11 @@ def test_md_run_grows_list_and_rolls_back(synth_models):
     s.compute()
     off0, _, _ = s.neighbors()
     cap0 = int(np.diff(off0).max())
-    s.md_run(150, 0.001, history=False)
+    if stepping == "md_run":
+        s.md_run(150, 0.001, history=False)
+    else:
+        for q in range(150):
+            s.step_async(0.001)
     off, _, _ = s.neighbors()
     out = s.get()
     say("max list length before", cap0, "after", int(np.diff(off).max()), "builds", s.nbuilds)
@@ -893,7 +900,7 @@ def test_lammps_fix_call_sequence(synth_models):
     """fix nve/tensormd/amd drives the device system as LAMMPS calls it: at
     each run's setup a system is made from the host state (positions wrapped
     into the box, the wraps counted in image flags, as LAMMPS' setup leaves
-    them), every step is one tmd_md_run step, thermo steps read the energy,
+    them), every step is one tmd_md_step_async call, thermo steps read the energy,
     and the end of a run copies x and v back.  Two runs of 25 steps agree
     with one continuous 50-step run and with the oracle's NVE."""
     say("case: LAMMPS fix call sequence, C1 2048 atoms, 2 runs x 25 steps")
@@ -912,7 +919,7 @@ def test_lammps_fix_call_sequence(synth_models):
         s = m.system(hx, t, cell, skin=1.0)
         s.set_velocities(hv)
         for step in range(nsteps // 2):
-            s.md_run(1, 0.001, history=False)
+            s.step_async(0.001)
             if (step + 1) % every == 0:
                 epot.append(s.get()["energy"])
         out = s.get()

@@ -209,6 +209,55 @@ def list_mode_bench(model, symbols, x, t, cell, prec, calls):
                                          ms_per_call=1e3 * dt_ser / calls))
 
 
+def fix_path_bench(model, symbols, x, t, cell, v, steps, host_ms=0.5):
+    """The step of fix nve/tensormd/amd (one tmd_md_step_async per LAMMPS
+    step) against one tmd_md_run of the same steps, BF16, each from the same
+    state; with host_ms of host work per step (LAMMPS' own bookkeeping,
+    modelled by a sleep) overlapped by the asynchronous step, or waited for
+    after each synchronous one.  Median of three."""
+    from tensormd_amd import TensorMD
+    m = TensorMD(model, symbols, "bf16", 0.001, device=0)
+    s = m.system(x, t, cell, skin=1.0)
+    s.set_velocities(v)
+    s.compute()
+    s.md_run(20, 0.001, history=False)
+    st = s.get()
+
+    def one_run():
+        s.md_run(steps, 0.001, history=False)
+
+    def async_steps(work):
+        for _ in range(steps):
+            s.step_async(0.001)
+            if work:
+                time.sleep(1e-3 * host_ms)
+
+    def sync_steps(work):
+        for _ in range(steps):
+            s.md_run(1, 0.001, history=False)
+            if work:
+                time.sleep(1e-3 * host_ms)
+
+    modes = {"one_md_run": one_run, "async_steps": lambda: async_steps(False),
+             "async_steps_with_host_work": lambda: async_steps(True),
+             "sync_steps_with_host_work": lambda: sync_steps(True)}
+    out = {k: [] for k in modes}
+    for _ in range(3):
+        for k, fn in modes.items():
+            s.set_positions(st["x"])
+            s.set_velocities(st["v"])
+            s.compute()
+            t0 = time.perf_counter()
+            fn()
+            s.get()
+            out[k].append(1e3 * (time.perf_counter() - t0) / steps)
+    s.close()
+    m.close()
+    res = {f"{k}_ms_per_step": float(np.median(a)) for k, a in out.items()}
+    res.update(steps=steps, host_work_ms=host_ms, precision="bf16")
+    return res
+
+
 def _force_errors(got, ref, n):
     df = got["f"] - ref["f"]
     return dict(max_abs_dF=float(abs(df).max()), rms_dF=float(np.sqrt(np.mean(df * df))),
@@ -289,6 +338,8 @@ def main():
     # calls of the Pair::compute drop-in (host buffers, LAMMPS-style ghosts)
     # on the same system, per precision and list mode
     ap.add_argument("--list-calls", type=int, default=20)
+    # steps of the fix nve/tensormd/amd timing (asynchronous per-step calls)
+    ap.add_argument("--fix-steps", type=int, default=100)
     # every N-th timed step of the headline precision is launched kernel by
     # kernel with HIP events around the kernels (0: off; the roofline then
     # uses the kernel times of the profile steps after the run)
@@ -447,6 +498,10 @@ def main():
         listmode = {prec: list_mode_bench(model, symbols, x, t, cell, prec, args.list_calls)
                     for prec in ("fp32", "bf16")}
         print(f"bench: list mode {json.dumps(listmode)}", flush=True)
+    fixpath = None
+    if args.fix_steps > 0:
+        fixpath = fix_path_bench(model, symbols, x, t, cell, v, args.fix_steps)
+        print(f"bench: lammps fix path {json.dumps(fixpath)}", flush=True)
 
     fp8 = None
     c4_keep = {}
@@ -524,7 +579,8 @@ def main():
     if "bf16" not in res or "fp32" not in res:
         print(json.dumps({"partial": True, "rates": {k: v["rate"] for k, v in res.items()},
                           "roofline": roof, "cpu_baseline": cpu, "serving": serving,
-                          "fp8_experiment": fp8, "list_mode": listmode, "nve_drift": drift or None}),
+                          "fp8_experiment": fp8, "list_mode": listmode, "lammps_fix_path": fixpath,
+                          "nve_drift": drift or None}),
               flush=True)
         return
     bf, fp = res["bf16"], res["fp32"]
@@ -555,6 +611,7 @@ def main():
         "serving": serving,
         "fp8_experiment": fp8,
         "list_mode": listmode,
+        "lammps_fix_path": fixpath,
         "nve_drift": drift or None,
     }
     print(json.dumps(line), flush=True)

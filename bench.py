@@ -347,6 +347,9 @@ def main():
     # NVE total-energy drift of each precision over this many steps, from the
     # same initial state, in a separate untimed run with per-step energies
     ap.add_argument("--drift-steps", type=int, default=1000)
+    # launch each MD step's kernels one by one instead of replaying the
+    # captured step graph (same kernels and results; for kernel tracers)
+    ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -379,6 +382,8 @@ def main():
     for prec in args.precisions.split(","):
         m = TensorMD(model, symbols, prec, 0.001, device=0)
         s = m.system(x, t, cell, skin=1.0)
+        if args.no_graph:
+            s.set_graph(False)
         s.set_velocities(v)
         s.compute()
         if nbr_within_rc is None:

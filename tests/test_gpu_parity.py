@@ -965,3 +965,13 @@ def test_fp8_mlp_error(tmp_path):
     assert out["bf16"]["dF"] <= 5e-3
     assert out["fp8"]["dF"] <= 0.1
     assert out["fp8"]["dE"] <= 0.01
+    # the Pair::compute drop-in (LAMMPS-style ghosts) in FP8, against the
+    # oracle's list evaluation, to the same bars
+    m = TensorMD(model, ["Mo", "Nb", "Ta", "W"], "fp8", 0.001)
+    xall, owner, ilist, numneigh, flat = W.lammps_ghost_list(x, cell, om.rc + 1.0)
+    tall = t[owner]
+    ref_l = om.compute_list(len(x), xall, tall - 1, ilist, numneigh, flat)
+    got_l = m.compute_list(len(x), xall, tall, ilist, numneigh, flat)
+    e_l = _errs(got_l, ref_l, len(x))
+    say("fp8 list-mode errors", e_l)
+    assert e_l["dF"] <= 0.1 and e_l["dE"] <= 0.01

@@ -167,3 +167,24 @@ def test_tdnp_entry_points_reject_null_context():
     d = ctypes.c_double()
     assert L.tmd_set_electron_temperature(None, 0.1) == -1
     assert L.tmd_tdnp_results(None, ctypes.byref(d), ctypes.byref(d), None, None) == -1
+
+
+def test_system_step_and_table_entry_points_reject_null():
+    """The per-step entry point and the table export check their arguments
+    before any device work; the device-built table is off by default."""
+    L = T.lib()
+    assert L.tmd_md_step_async(None, 0.001) == -1
+    assert L.tmd_md_run(None, 1, 0.001, None, None) == -1
+    n, dx = ctypes.c_int(), ctypes.c_double()
+    assert L.tmd_radial_table(None, None, 0, ctypes.byref(n), ctypes.byref(dx)) == -1
+    opt = T.Options()
+    L.tmd_options_default(ctypes.byref(opt))
+    assert opt.table_device == 0
+
+
+def test_device_table_build_needs_a_device(synth_models):
+    """table_device = 1 loads the model on the host, then fails at device
+    initialisation here (no GPU) instead of falling back to the host table."""
+    with pytest.raises(T.TmdError) as ei:
+        T.TensorMD(synth_models["c1"], ["Cu"], "fp32", 0.001, table_device=True)
+    assert ei.value.code == -3

@@ -34,26 +34,31 @@ struct ListPlan {
   int cnt[kMaxElts] = {};
 };
 
-// Host staging: zero cell | pos[nall] | list[nlocal][cap] | nn[nlocal] | row2atom[nrows]
+// Host staging: zero cell | pos[nall] | list[nlocal][cap] | nn[nlocal] |
+// row2atom[nrows] | f[nall][3] (with stage_f: the forces come back into
+// pinned memory at the full DMA rate and are copied out by host threads,
+// instead of a copy into the caller's pageable array)
 struct HostStage {
   CellInfo *cell;
   double4 *pos;
   uint32_t *list;
   int *nn;
   int *row2atom;
+  double *f;  // null without stage_f
 };
 
-size_t host_stage_bytes(const ListArgs &a, const ListPlan &p, size_t *off)
+size_t host_stage_bytes(const ListArgs &a, const ListPlan &p, size_t *off, bool stage_f)
 {
   off[0] = 0;
   off[1] = al256(sizeof(CellInfo));
   off[2] = off[1] + al256(sizeof(double4) * (size_t) std::max(a.nall, 1));
   off[3] = off[2] + al256(4 * (size_t) std::max(a.nlocal, 1) * p.cap);
   off[4] = off[3] + al256(4 * (size_t) std::max(a.nlocal, 1));
-  return off[4] + al256(4 * (size_t) std::max(p.nrows, 1));
+  off[5] = off[4] + al256(4 * (size_t) std::max(p.nrows, 1));
+  return off[5] + (stage_f ? al256(3 * sizeof(double) * (size_t) std::max(a.nall, 1)) : 0);
 }
 
-HostStage carve_host(unsigned char *base, const size_t *off)
+HostStage carve_host(unsigned char *base, const size_t *off, bool stage_f)
 {
   HostStage h;
   h.cell = (CellInfo *) (base + off[0]);
@@ -61,6 +66,7 @@ HostStage carve_host(unsigned char *base, const size_t *off)
   h.list = (uint32_t *) (base + off[2]);
   h.nn = (int *) (base + off[3]);
   h.row2atom = (int *) (base + off[4]);
+  h.f = stage_f ? (double *) (base + off[5]) : nullptr;
   return h;
 }
 
@@ -187,8 +193,9 @@ int list_compute(tmd_ctx *ctx, const ListArgs &a, const ListResources &r, long l
   }
 
   // ---- host staging ----
-  size_t hoff[5];
-  const size_t hbytes = host_stage_bytes(a, plan, hoff);
+  size_t hoff[6];
+  const bool stage_f = !r.fixed && !a.Gout;
+  const size_t hbytes = host_stage_bytes(a, plan, hoff, stage_f);
   unsigned char *hbase;
   if (r.fixed) {
     if (hbytes > r.host_bytes) {
@@ -202,7 +209,7 @@ int list_compute(tmd_ctx *ctx, const ListArgs &a, const ListResources &r, long l
     if (e != hipSuccess) return hip_fail(ctx, e, "pinned staging alloc");
     hbase = (unsigned char *) ctx->list_host.p;
   }
-  HostStage hs = carve_host(hbase, hoff);
+  HostStage hs = carve_host(hbase, hoff, stage_f);
 
   // positions with the element in w; every position's type must be mapped
   std::atomic<int> bad_type{0};
@@ -366,8 +373,8 @@ int list_compute(tmd_ctx *ctx, const ListArgs &a, const ListResources &r, long l
   // the sums land in the host staging's cell slot (pinned or fixed host
   // memory), read after the stream is drained
   double *red = reinterpret_cast<double *>(hs.cell);
-  if ((e = hipMemcpyAsync(a.f, w.f.p, sizeof(double) * 3 * (size_t) a.nall, hipMemcpyDeviceToHost, s)) !=
-          hipSuccess ||
+  if ((e = hipMemcpyAsync(hs.f ? hs.f : a.f, w.f.p, sizeof(double) * 3 * (size_t) a.nall, hipMemcpyDeviceToHost,
+                          s)) != hipSuccess ||
       (a.vatom && (e = hipMemcpyAsync(a.vatom, w.vatom.p, sizeof(double) * 6 * (size_t) a.nall,
                                       hipMemcpyDeviceToHost, s)) != hipSuccess) ||
       (e = hipMemcpyAsync(red, w.red.p, 7 * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess ||
@@ -385,6 +392,13 @@ int list_compute(tmd_ctx *ctx, const ListArgs &a, const ListResources &r, long l
   }
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(ctx, e, who);
   drain.done = true;
+  if (hs.f) {
+    const size_t nf = 3 * (size_t) a.nall;
+    parallel_range((int) ((nf + 4095) / 4096), 16, threads, [&](int lo, int hi) {
+      const size_t b = (size_t) lo * 4096, e2 = std::min(nf, (size_t) hi * 4096);
+      std::memcpy(a.f + b, hs.f + b, sizeof(double) * (e2 - b));
+    });
+  }
   if (td) {
     ctx->td_satom.assign(a.nlocal, 0.0);
     ctx->td_aatom.assign(a.nlocal, 0.0);

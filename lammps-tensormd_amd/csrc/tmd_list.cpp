@@ -14,7 +14,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -107,25 +110,108 @@ void carve_device(Workspace &w, unsigned char *base, const size_t *off)
   for (int q = 0; q < 13; q++) b[q]->set_view(base + off[q], off[q + 1] - off[q]);
 }
 
-// fn(lo, hi) over [0, n) on up to 16 threads when there is enough work
+// Host worker threads kept for the process (up to 15, started at the first
+// call that has work for them), so that a list-mode call does not pay for
+// creating threads.  run(n, job) calls job(0 .. n-1) on the workers and the
+// calling thread and returns when all are done; concurrent callers take
+// turns.
+class HostPool {
+ public:
+  static HostPool &get()
+  {
+    static HostPool pool;
+    return pool;
+  }
+  int size() const { return nworkers + 1; }
+  void run(int n, const std::function<void(int)> &job)
+  {
+    std::lock_guard<std::mutex> turn(run_mu);
+    {
+      std::lock_guard<std::mutex> g(mu);
+      cur = &job;
+      njobs = n;
+      next = 0;
+      finished = 0;
+      ++gen;
+    }
+    cv.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(mu);
+    done_cv.wait(lk, [&] { return finished == njobs; });
+    cur = nullptr;
+  }
+  ~HostPool()
+  {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto &t : workers) t.join();
+  }
+
+ private:
+  HostPool()
+  {
+    nworkers = (int) std::min<unsigned>(15u, std::max(1u, std::thread::hardware_concurrency()) - 1);
+    for (int i = 0; i < nworkers; i++) workers.emplace_back([this] { loop(); });
+  }
+  // takes jobs of the current run until none is left
+  void work()
+  {
+    for (;;) {
+      int k;
+      const std::function<void(int)> *f;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        if (!cur || next >= njobs) return;
+        k = next++;
+        f = cur;
+      }
+      (*f)(k);
+      std::lock_guard<std::mutex> g(mu);
+      if (++finished == njobs) done_cv.notify_all();
+    }
+  }
+  void loop()
+  {
+    unsigned long seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || gen != seen; });
+        if (stop) return;
+        seen = gen;
+      }
+      work();
+    }
+  }
+  std::mutex run_mu, mu;
+  std::condition_variable cv, done_cv;
+  std::vector<std::thread> workers;
+  const std::function<void(int)> *cur = nullptr;
+  int nworkers = 0, njobs = 0, next = 0, finished = 0;
+  unsigned long gen = 0;
+  bool stop = false;
+};
+
+// fn(lo, hi) over [0, n) in up to 16 pieces on the host pool when there is
+// enough work (at least min_per_thread items a piece)
 template <class F>
 void parallel_range(int n, int min_per_thread, bool threads, F fn)
 {
-  int nt = threads ? (int) std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())) : 1;
+  int nt = threads ? std::min(16, HostPool::get().size()) : 1;
   nt = std::min(nt, n / std::max(min_per_thread, 1));
   if (nt <= 1) {
     fn(0, n);
     return;
   }
-  std::vector<std::thread> th;
-  th.reserve(nt - 1);
   const int per = (n + nt - 1) / nt;
-  for (int t = 1; t < nt; t++) {
+  const std::function<void(int)> job = [&](int t) {
     const int lo = t * per, hi = std::min(n, lo + per);
-    if (lo < hi) th.emplace_back([=] { fn(lo, hi); });
-  }
-  fn(0, std::min(n, per));
-  for (auto &q : th) q.join();
+    if (lo < hi) fn(lo, hi);
+  };
+  HostPool::get().run(nt, job);
 }
 
 // Waits for the stream on every return path, so that no copy from the host

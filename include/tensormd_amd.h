@@ -29,10 +29,14 @@
  *   tmd_descriptors_list ComputeTensorMDDensity::compute_peratom
  *                       (src/ML-TENSORMD/compute_tensormd_density.cpp:129-205)
  *   tmd_system_*,       the LAMMPS step loop around the pair style:
- *   tmd_md_run          Verlet::run (src/verlet.cpp:246-350), NPairFullBin::build
- *                       (src/npair_full_bin.cpp:68-126), Neighbor::decide
+ *   tmd_md_run,         Verlet::run (src/verlet.cpp:246-350), NPairFullBin::build
+ *   tmd_md_step_async   (src/npair_full_bin.cpp:68-126), Neighbor::decide
  *                       (src/neighbor.cpp:2293-2360), FixNVE
  *                       (src/fix_nve.cpp:64-130), kept on the device
+ *   tmd_radial_table    the table of FeaturePotential::setup_ration
+ *                       (src/ML-TENSORMD/fnn.cpp:77-105), built on the host
+ *                       or, with tmd_options.table_device, on the device as
+ *                       forward_fnn_GPU does (gpu/cuda/nnp.cu:59-116)
  *   tmd_destroy         PairTensorMD::~PairTensorMD (pair_tensormd.cpp:68-83)
  *   tmd_serve_*         no reference counterpart: batched single-point
  *                       requests (each one a PairTensorMD::compute of a

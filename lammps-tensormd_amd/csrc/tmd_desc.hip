@@ -317,7 +317,7 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
                                                    const float4 *tab, float *G, uint32_t *kl)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int row = rbase + (int) blockIdx.x;
+  const int row = geo.order ? geo.order[rbase + (int) blockIdx.x] : rbase + (int) blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int gw = g.nelt * K * kNM;
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(64, (kForceWaves<K, NS>)) void k_force(ModelGeo g, 
 {
   using T = Tiling<K>;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int row = rbase + (int) blockIdx.x;
+  const int row = geo.order ? geo.order[rbase + (int) blockIdx.x] : rbase + (int) blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int i = row2atom[row];
@@ -709,6 +709,7 @@ hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *
                              hipStream_t stream, int row_begin)
 {
   if (nrows <= row_begin) return hipSuccess;
+  if (geo.order && row_begin != 0) return hipErrorInvalidValue;  // an order covers whole launches
   if (g.mm > 3)
     return launch_descriptor_hm(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, stream, row_begin);
 #define TMD_K_CASE(K)                                                                                    \
@@ -732,6 +733,7 @@ hipError_t launch_force(const ModelGeo &g, const Geom &geo, const double4 *pos, 
                         double *watom, double *vatom, hipStream_t stream, int row_begin)
 {
   if (nrows <= row_begin) return hipSuccess;
+  if (geo.order && row_begin != 0) return hipErrorInvalidValue;  // an order covers whole launches
   if (g.mm > 3)
     return launch_force_hm(g, geo, pos, cap, row2atom, nrows, tab, kl, dEdG, E, scale, f, eatom, watom, vatom,
                            stream, row_begin);

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(64) void k_descriptor_hm(ModelGeo g, Geom geo, cons
   using Lo = HmLayout<K, MM>;
   constexpr int ND = Mo::ND, NM = Mo::NM;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int row = rbase + (int) blockIdx.x;
+  const int row = geo.order ? geo.order[rbase + (int) blockIdx.x] : rbase + (int) blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int gw = g.nelt * K * NM;
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(64) void k_force_hm(ModelGeo g, Geom geo, const dou
   constexpr int ND = Mo::ND, NM = Mo::NM;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double facc[9];
-  const int row = rbase + (int) blockIdx.x;
+  const int row = geo.order ? geo.order[rbase + (int) blockIdx.x] : rbase + (int) blockIdx.x;
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int i = row2atom[row];

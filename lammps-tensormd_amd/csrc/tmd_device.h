@@ -139,6 +139,12 @@ struct Geom {
   const CellInfo *cells;
   const int *atom_struct;
   const BatchDims *dyn;  // rows beyond dyn->rl.nrows are skipped when non-null
+  // the order in which the descriptor and force kernels take the rows: a
+  // permutation of [0, nrows) (launch position p runs row order[p]); null:
+  // row order.  Rows are element blocks, so atoms of one element run
+  // together; the standalone system orders them by atom index instead,
+  // which for spatially ordered atoms keeps neighboring atoms' work together
+  const int *order = nullptr;
 };
 // The descriptor kernel's compaction of each row's list, reused by the force
 // kernel of the same evaluation: kl[row][kKeptHdr + cap], kl[0] = nk,

@@ -27,6 +27,7 @@ struct tmd_system {
   std::vector<int> row2atom;
   DeviceBuffer vel, dtfm, mass, bins, atominfo, xbuild, ctl, hist;
   DeviceBuffer snap;  // x, v, f at the start of the current run segment
+  DeviceBuffer order;  // rows in atom order (the descriptor and force kernels' launch order)
   bool have_forces = false;
   double dt_cached = -1;
   int nbuilds_seen = 0;
@@ -150,6 +151,7 @@ Chain sys_chain(tmd_system *s)
 {
   Chain c;
   c.geo.cells = (const CellInfo *) s->w.cells.p;
+  c.geo.order = (const int *) s->order.p;
   c.pos = (const double4 *) s->w.pos.p;
   c.nbr = (const uint32_t *) s->w.nbr.p;
   c.nn = (const int *) s->w.nn.p;
@@ -293,6 +295,22 @@ int tmd_system_create(tmd_ctx *ctx, int natoms, const double *x, const int *type
   make_rows(m.nelt, atoms, elem, s->w.rl, s->row2atom);
   int rc = ensure_rows(ctx, s->w, s->w.rl.nrows);
   if (rc != TMD_OK) return rc;
+  {
+    // the descriptor and force kernels take the rows in atom order (the
+    // element blocks interleaved), padding rows last: for atoms given in
+    // spatial order, consecutive workgroups gather the same neighbors
+    // (1.2% on the 128k-atom step)
+    std::vector<int> ord;
+    ord.reserve(s->w.rl.nrows);
+    for (int r = 0; r < s->w.rl.nrows; r++)
+      if (s->row2atom[r] >= 0) ord.push_back(r);
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return s->row2atom[a] < s->row2atom[b]; });
+    for (int r = 0; r < s->w.rl.nrows; r++)
+      if (s->row2atom[r] < 0) ord.push_back(r);
+    hipError_t oe = s->order.ensure(4 * std::max<size_t>(ord.size(), 1));
+    if (oe == hipSuccess) oe = hipMemcpy(s->order.p, ord.data(), 4 * ord.size(), hipMemcpyHostToDevice);
+    if (oe != hipSuccess) return hip_fail(ctx, oe, "row order upload");
+  }
   hipError_t e;
   auto chk = [&](hipError_t er, const char *what) -> int {
     if (er != hipSuccess) return hip_fail(ctx, er, what);

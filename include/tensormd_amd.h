@@ -244,7 +244,9 @@ int tmd_system_nbuilds(const tmd_system *sys);
 int tmd_system_neighbors(tmd_system *sys, long cap, long *off, int *j, int *shift, long *total);
 /* Descriptor rows G[natoms][width] and dE/dG of the last compute (FP32,
  * atom order, width and [b][k][m] layout of the model file), for
- * kernel-level parity tests. */
+ * kernel-level parity tests.  Where the BF16 MLP takes its descriptor rows
+ * as BF16 (tanh networks of the benchmark layer shapes), G is those rows,
+ * rounded to BF16 (nearest even) and widened exactly. */
 int tmd_system_descriptors(tmd_system *sys, float *G, float *dEdG);
 
 /* ---- serving: batched single-point requests --------------------------------

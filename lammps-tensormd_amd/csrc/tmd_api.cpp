@@ -454,7 +454,10 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
     return (set_err(ctx, "internal: kept-list cache not allocated for this list"), TMD_ERR_STATE);
   uint32_t *kl = (uint32_t *) w.kl.p;
   if (pe && (e = hipEventRecord(pe->ev[0], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
-  e = launch_descriptor(ctx->geo, c.geo, c.pos, c.nbr, c.nn, c.cap, row2atom, w.rl.nrows,
+  Geom geo = c.geo;
+  geo.g_bf16 = c.g_bf16 && !ctx->model.tdnp && ctx->geo.mm <= 3 && mlp_bf16_rows(ctx->mlp, mode, c.rtile);
+  w.g_bf16 = geo.g_bf16;
+  e = launch_descriptor(ctx->geo, geo, c.pos, c.nbr, c.nn, c.cap, row2atom, w.rl.nrows,
                         (const float4 *) ctx->tab.p, (float *) w.G.p, kl, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "descriptor launch");
   if (pe && (e = hipEventRecord(pe->ev[1], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");
@@ -466,7 +469,7 @@ int run_chain(tmd_ctx *ctx, Workspace &w, const Chain &c, hipStream_t s, PhaseEv
     if (e != hipSuccess) return hip_fail(ctx, e, "TDNP launch");
   } else {
     e = launch_mlp(mode, w.rl, ctx->mlp, (const float *) w.G.p, (double *) w.E.p, (float *) w.dEdG.p, s,
-                   c.rtile, c.dyn);
+                   c.rtile, c.dyn, 0, -1, geo.g_bf16);
     if (e != hipSuccess) return hip_fail(ctx, e, "mlp launch");
   }
   if (pe && (e = hipEventRecord(pe->ev[2], s)) != hipSuccess) return hip_fail(ctx, e, "timing event");

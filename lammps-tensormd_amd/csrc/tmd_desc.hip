@@ -264,9 +264,12 @@ __device__ inline void accumulate_P(const ModelGeo &g, const Box &cell, const do
 // G[b0 + b][k][m] = sum_d T[m][d] P^2 (m >= 1) for the nsl slots of a pass:
 // partial sums over the lane's d block, then a sum over the NDG lanes that
 // share k; G[b][k][0] = P0 (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+
 template <int K, int NS>
 __device__ inline void write_G(const ModelGeo &g, float (*acc)[Tiling<K>::KT][Tiling<K>::DT],
-                               float *Grow, int b0, int nsl)
+                               float *Grow, int b0, int nsl, bool gbf16)
 {
   using T = Tiling<K>;
   const int lane = threadIdx.x & 63;
@@ -300,8 +303,13 @@ __device__ inline void write_G(const ModelGeo &g, float (*acc)[Tiling<K>::KT][Ti
           if (group_has_degree(t * T::DT, T::DT, m)) tot[m] += __shfl(part[m], gbase + t);
       if (lane < T::LANES && dq == 0) {
         const int k = kq * T::KT + a;
-        *reinterpret_cast<float4 *>(Grow + ((size_t) (b0 + b) * K + k) * kNM) =
-            make_float4(acc[b][a][0], tot[1], tot[2], tot[3]);
+        const size_t o = ((size_t) (b0 + b) * K + k) * kNM;
+        if (gbf16) {
+          const f32x4v xv = {acc[b][a][0], tot[1], tot[2], tot[3]};
+          *reinterpret_cast<bf16x4v *>(reinterpret_cast<uint16_t *>(Grow) + o) = __builtin_convertvector(xv, bf16x4v);
+        } else {
+          *reinterpret_cast<float4 *>(Grow + o) = make_float4(acc[b][a][0], tot[1], tot[2], tot[3]);
+        }
       }
     }
   }
@@ -321,10 +329,13 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
   if (row >= nrows || (geo.dyn && row >= geo.dyn->rl.nrows)) return;
   const int lane = threadIdx.x & 63;
   const int gw = g.nelt * K * kNM;
-  float *Grow = G + (size_t) row * gw;
+  // BF16 rows: gw 2-byte values per row (gw is a multiple of 4)
+  float *Grow = geo.g_bf16 ? reinterpret_cast<float *>(reinterpret_cast<uint16_t *>(G) + (size_t) row * gw)
+                           : G + (size_t) row * gw;
   const int i = row2atom[row];
   if (i < 0) {  // padding row
-    for (int e = lane; e < gw; e += 64) Grow[e] = 0.0f;
+    const int nw = geo.g_bf16 ? gw / 2 : gw;
+    for (int e = lane; e < nw; e += 64) Grow[e] = 0.0f;
     return;
   }
   using T = Tiling<K>;
@@ -348,7 +359,7 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
     float acc[NS][T::KT][T::DT];
     accumulate_P<K, NS>(g, cell, xi, pos, tab, L, start, g.nelt, acc);
     write_kept();
-    write_G<K, NS>(g, acc, Grow, 0, g.nelt);
+    write_G<K, NS>(g, acc, Grow, 0, g.nelt, geo.g_bf16);
   } else {
     __shared__ int s_start[kMaxElts + 1];
     if (lane == 0)
@@ -359,7 +370,7 @@ __global__ __launch_bounds__(64) void k_descriptor(ModelGeo g, Geom geo, const d
       const int nsl = min(NS, g.nelt - b0);
       float acc[NS][T::KT][T::DT];
       accumulate_P<K, NS>(g, cell, xi, pos, tab, L, s_start + b0, nsl, acc);
-      write_G<K, NS>(g, acc, Grow, b0, nsl);
+      write_G<K, NS>(g, acc, Grow, b0, nsl, geo.g_bf16);
     }
   }
 }
@@ -709,6 +720,7 @@ hipError_t launch_descriptor(const ModelGeo &g, const Geom &geo, const double4 *
                              hipStream_t stream, int row_begin)
 {
   if (nrows <= row_begin) return hipSuccess;
+  if (geo.g_bf16 && g.mm > 3) return hipErrorInvalidValue;  // FP32 rows only there
   if (geo.order && row_begin != 0) return hipErrorInvalidValue;  // an order covers whole launches
   if (g.mm > 3)
     return launch_descriptor_hm(g, geo, pos, nbr, nn, cap, row2atom, nrows, tab, G, kl, stream, row_begin);

@@ -145,6 +145,9 @@ struct Geom {
   // together; the standalone system orders them by atom index instead,
   // which for spatially ordered atoms keeps neighboring atoms' work together
   const int *order = nullptr;
+  // the descriptor kernel stores G as BF16 (round to nearest even, the BF16
+  // MLP's own operand rounding) instead of FP32; max_moment <= 3 kernels only
+  bool g_bf16 = false;
 };
 // The descriptor kernel's compaction of each row's list, reused by the force
 // kernel of the same evaluation: kl[row][kKeptHdr + cap], kl[0] = nk,
@@ -189,9 +192,15 @@ hipError_t launch_force_hm(const ModelGeo &g, const Geom &geo, const double4 *po
 // With dyn, rl gives the capacity (grid) and dyn->rl the live layout.
 // Rows [row_begin, row_end) only (row_end < 0: to rl.nrows); both bounds
 // are multiples of 64 (element blocks start on such rows).
+// g_bf16: G holds BF16 rows (the descriptor kernel's Geom::g_bf16 output);
+// only where mlp_bf16_rows() holds.
 hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, double *E,
                       float *dEdG, hipStream_t stream, int rtile = 0,
-                      const BatchDims *dyn = nullptr, int row_begin = 0, int row_end = -1);
+                      const BatchDims *dyn = nullptr, int row_begin = 0, int row_end = -1,
+                      bool g_bf16 = false);
+// the MLP launch takes BF16 descriptor rows: BF16 mode, tanh, a compile-time
+// layer shape, 32-row tiles
+bool mlp_bf16_rows(const MlpParams &mp, int mode, int rtile);
 size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out);
 // the fast (aligned) MLP kernel applies: required by the FP8 mode
 bool mlp_fast_ok(const MlpParams &mp, int mode);

@@ -454,6 +454,7 @@ int list_compute(tmd_ctx *ctx, const ListArgs &a, const ListResources &r, long l
   c.vbytes = sizeof(double) * 6 * (size_t) a.nall;
   c.natoms = a.nlocal;
   c.zero_ew = a.inum < a.nlocal;
+  c.g_bf16 = true;
   int rc = run_chain(ctx, w, c, s);
   if (rc != TMD_OK) return rc;
   // the sums land in the host staging's cell slot (pinned or fixed host

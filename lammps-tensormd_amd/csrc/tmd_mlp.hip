@@ -615,12 +615,20 @@ struct GRaw<0> {
   f32x4 v;
 };
 
-template <int MODE>
+// GB: the descriptor rows are stored as BF16 (the BF16 mode's operand
+// values, rounded once by the descriptor kernel), read 16 bytes per lane
+template <int MODE, bool GB = false>
 __device__ inline GRaw<MODE> ld_graw(const float *G, int gw, int grow, int k)
 {
   const int lane = threadIdx.x & 63;
-  const float *p = G + (size_t) (grow + (lane & 15)) * gw + k;
   GRaw<MODE> g;
+  if constexpr (GB) {
+    static_assert(MODE == 1, "BF16 descriptor rows feed the BF16 mode only");
+    const uint16_t *pb = reinterpret_cast<const uint16_t *>(G) + (size_t) (grow + (lane & 15)) * gw + k;
+    g.lo = *reinterpret_cast<const float4 *>(pb + (lane >> 4) * 8);
+    return g;
+  }
+  const float *p = G + (size_t) (grow + (lane & 15)) * gw + k;
   if constexpr (MODE == 0) {
     g.v = *reinterpret_cast<const f32x4 *>(p + (lane >> 4) * 4);
   } else {
@@ -631,10 +639,12 @@ __device__ inline GRaw<MODE> ld_graw(const float *G, int gw, int grow, int k)
   return g;
 }
 
-template <int MODE>
+template <int MODE, bool GB = false>
 __device__ inline typename Frag<MODE>::A cvt_graw(const GRaw<MODE> &g, float ri)
 {
-  if constexpr (MODE == 0) {
+  if constexpr (GB) {
+    return __builtin_bit_cast(bf16x8, g.lo);
+  } else if constexpr (MODE == 0) {
     return g.v;
   } else if constexpr (MODE == 1) {
     f32x8 xv = {g.lo.x, g.lo.y, g.lo.z, g.lo.w, g.hi.x, g.hi.y, g.hi.z, g.hi.w};
@@ -736,7 +746,7 @@ __device__ inline void st4(uint16_t *p, const float v[4])
 // same number of loads (past the last step they repeat it), which lets the
 // compiler wait for exactly the step being consumed.  Activations in LDS
 // are read at the step that uses them.
-template <int MODE, int R, bool XG, int NTW, typename T>
+template <int MODE, int R, bool XG, int NTW, typename T, bool GB = false>
 __device__ inline void gemm_t(const void *W, int t0, const T *X, int ldx, const float *G, int gw,
                               int grow0, int K, const float *rinv, f32x4 (&acc)[4][R / 16])
 {
@@ -760,7 +770,7 @@ __device__ inline void gemm_t(const void *W, int t0, const T *X, int ldx, const 
     for (int j = 0; j < NTW; j++) as[j] = ld_a<MODE>(W, t0 + 4 * j, sc, nst);
     if constexpr (XG) {
 #pragma unroll
-      for (int r = 0; r < RT; r++) gs[r] = ld_graw<MODE>(G, gw, grow0 + r * 16, k);
+      for (int r = 0; r < RT; r++) gs[r] = ld_graw<MODE, GB>(G, gw, grow0 + r * 16, k);
     }
   };
 #pragma unroll
@@ -776,7 +786,7 @@ __device__ inline void gemm_t(const void *W, int t0, const T *X, int ldx, const 
 #pragma unroll
         for (int r = 0; r < RT; r++) {
           if constexpr (XG)
-            b[r] = cvt_graw<MODE>(g[i][r], ri[r]);
+            b[r] = cvt_graw<MODE, GB>(g[i][r], ri[r]);
           else
             b[r] = ld_x<MODE>(X, ldx, r * 16, s * KS, ri[r]);
         }
@@ -790,18 +800,18 @@ __device__ inline void gemm_t(const void *W, int t0, const T *X, int ldx, const 
 }
 
 // gemm_t for a runtime tile count ntw in 1..4 (wave-uniform)
-template <int MODE, int R, bool XG, typename T>
+template <int MODE, int R, bool XG, typename T, bool GB = false>
 __device__ inline void gemm_n(const void *W, int t0, int ntw, const T *X, int ldx, const float *G,
                               int gw, int grow0, int K, const float *rinv, f32x4 (&acc)[4][R / 16])
 {
   if (ntw >= 4)
-    gemm_t<MODE, R, XG, 4, T>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
+    gemm_t<MODE, R, XG, 4, T, GB>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
   else if (ntw == 3)
-    gemm_t<MODE, R, XG, 3, T>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
+    gemm_t<MODE, R, XG, 3, T, GB>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
   else if (ntw == 2)
-    gemm_t<MODE, R, XG, 2, T>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
+    gemm_t<MODE, R, XG, 2, T, GB>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
   else
-    gemm_t<MODE, R, XG, 1, T>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
+    gemm_t<MODE, R, XG, 1, T, GB>(W, t0, X, ldx, G, gw, grow0, K, rinv, acc);
 }
 
 }  // namespace
@@ -835,7 +845,8 @@ struct MlpShape<2> {  // 384-256-256-128-1
   static constexpr int w(int l) { return l == 0 ? 384 : (l == 1 ? 256 : (l == 2 ? 256 : (l == 3 ? 128 : 1))); }
 };
 
-template <int MODE, int ACT, int R, int SH = 0>
+// GB: Gd holds BF16 descriptor rows (MODE 1 only)
+template <int MODE, int ACT, int R, int SH = 0, bool GB = false>
 __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayout rl_cap, MlpParams mp, const float *Gd,
                                                   double *E, float *dEdG, const BatchDims *dyn, int rbase)
 {
@@ -903,7 +914,7 @@ __global__ __launch_bounds__(256, (kMlpWaves<MODE, R>)) void k_mlp_fast(RowLayou
       if constexpr (MODE == 2)
         gemm_n<MODE, R, false, uint8_t>(W, t0, ntw, Q, mp.ldq, nullptr, gw, row0, K, rs + R, acc);
       else if (l == 0)
-        gemm_n<MODE, R, true, T>(W, t0, ntw, nullptr, 0, Gd, gw, row0, K, rs + R, acc);
+        gemm_n<MODE, R, true, T, GB>(W, t0, ntw, nullptr, 0, Gd, gw, row0, K, rs + R, acc);
       else
         gemm_n<MODE, R, false, T>(W, t0, ntw, Hin, ldi, nullptr, gw, row0, K, rs + R, acc);
 #pragma unroll
@@ -1074,7 +1085,7 @@ int mlp_shape(const MlpParams &mp)
 // generic kernel.
 template <int MODE, int A, int R>
 hipError_t launch_fast(dim3 grid, size_t lds, hipStream_t s, const RowLayout &rl, const MlpParams &mp,
-                       const float *G, double *E, float *dEdG, const BatchDims *dyn, int rb)
+                       const float *G, double *E, float *dEdG, const BatchDims *dyn, int rb, bool gb = false)
 {
   auto go = [&](auto kern) -> hipError_t {
     if (hipError_t ae = hipFuncSetAttribute((const void *) kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);
@@ -1083,6 +1094,15 @@ hipError_t launch_fast(dim3 grid, size_t lds, hipStream_t s, const RowLayout &rl
     kern<<<grid, 256, lds, s>>>(rl, mp, G, E, dEdG, dyn, rb);
     return hipGetLastError();
   };
+  if constexpr (MODE == 1 && A == 2 && R == 32) {
+    if (gb) {
+      const int sh = mlp_shape(mp);
+      if (sh == 1) return go(k_mlp_fast<MODE, A, R, 1, true>);
+      if (sh == 2) return go(k_mlp_fast<MODE, A, R, 2, true>);
+      return hipErrorInvalidValue;
+    }
+  }
+  if (gb) return hipErrorInvalidValue;
   if constexpr (A == 2 && R == (MODE == 2 ? 16 : 32)) {
     const int sh = mlp_shape(mp);
     if (sh == 1) return go(k_mlp_fast<MODE, A, R, 1>);
@@ -1093,7 +1113,7 @@ hipError_t launch_fast(dim3 grid, size_t lds, hipStream_t s, const RowLayout &rl
 
 template <int MODE, int R>
 hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, double *E, float *dEdG,
-                    size_t lds, hipStream_t s, const BatchDims *dyn, int rb, int re)
+                    size_t lds, hipStream_t s, const BatchDims *dyn, int rb, int re, bool gb = false)
 {
   if (rb % R || re % R) return hipErrorInvalidValue;
   dim3 grid((re - rb) / R);
@@ -1102,7 +1122,7 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
     if (!fast) return hipErrorInvalidConfiguration;
 #define TMD_MLP8_CASE(A)                                                                         \
   case A:                                                                                        \
-    return launch_fast<MODE, A, R>(grid, lds, s, rl, mp, G, E, dEdG, dyn, rb);
+    return launch_fast<MODE, A, R>(grid, lds, s, rl, mp, G, E, dEdG, dyn, rb, gb);
     switch (mp.act) {
       TMD_MLP8_CASE(0)
       TMD_MLP8_CASE(1)
@@ -1116,8 +1136,9 @@ hipError_t launch_r(const RowLayout &rl, const MlpParams &mp, const float *G, do
 #define TMD_MLP_CASE(A)                                                                          \
   case A:                                                                                        \
     if (fast) {                                                                                  \
-      return launch_fast<MODE, A, R>(grid, lds, s, rl, mp, G, E, dEdG, dyn, rb);                 \
+      return launch_fast<MODE, A, R>(grid, lds, s, rl, mp, G, E, dEdG, dyn, rb, gb);             \
     } else {                                                                                     \
+      if (gb) return hipErrorInvalidValue;                                                       \
       if (hipError_t ae = hipFuncSetAttribute((const void *) k_mlp<MODE, A, R>,                  \
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds);            \
           ae != hipSuccess)                                                                      \
@@ -1232,10 +1253,18 @@ size_t mlp_lds_bytes(const MlpParams &mp, int mode, int R, MlpParams *out)
   return bytes;
 }
 
+bool mlp_bf16_rows(const MlpParams &mp, int mode, int rtile)
+{
+  if (mode != 1 || !mlp_fast_ok(mp, 1) || mp.act != 2 || mlp_shape(mp) == 0) return false;
+  if (rtile != 0 && rtile != 32) return false;
+  return mlp_lds_bytes(mp, 1, 32, nullptr) <= 160 * 1024;  // the 32-row tile is the one taken
+}
+
 hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const float *G, double *E,
                       float *dEdG, hipStream_t s, int rtile, const BatchDims *dyn, int row_begin,
-                      int row_end)
+                      int row_end, bool g_bf16)
 {
+  if (g_bf16 && !mlp_bf16_rows(mp, mode, rtile)) return hipErrorInvalidValue;
   if (row_end < 0 || row_end > rl.nrows) row_end = rl.nrows;
   if (row_begin < 0 || row_begin > row_end) return hipErrorInvalidValue;
   if (row_end == row_begin) return hipSuccess;
@@ -1267,9 +1296,9 @@ hipError_t launch_mlp(int mode, const RowLayout &rl, const MlpParams &mp, const 
       if (R == 32) return launch_r<2, 32>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
       return launch_r<2, 16>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
     }
-    if (R == 64) return launch_r<1, 64>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
-    if (R == 32) return launch_r<1, 32>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
-    return launch_r<1, 16>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end);
+    if (R == 64) return launch_r<1, 64>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end, g_bf16);
+    if (R == 32) return launch_r<1, 32>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end, g_bf16);
+    return launch_r<1, 16>(rl, m, G, E, dEdG, lds, s, dyn, row_begin, row_end, g_bf16);
   }
   return hipErrorInvalidConfiguration;
 }

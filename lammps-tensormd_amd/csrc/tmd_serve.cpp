@@ -290,6 +290,7 @@ int enqueue_batch(Server *S, Slot &sl, const Bucket &b, int rtile)
   c.struct_out = (double *) sl.sout.p;
   c.rtile = rtile;
   c.dyn = a.dyn;
+  c.g_bf16 = true;
   int rc = run_chain(ctx, sl.w, c, st);
   if (rc != TMD_OK) return rc;
   e = launch_batch_pack(sl.out_dev, b.caps, a, st);

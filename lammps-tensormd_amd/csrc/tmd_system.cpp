@@ -159,6 +159,7 @@ Chain sys_chain(tmd_system *s)
   c.f = (double *) s->w.f.p;
   c.fbytes = sizeof(double) * 3 * (size_t) s->n;
   c.natoms = s->n;
+  c.g_bf16 = true;
   return c;
 }
 
@@ -828,7 +829,17 @@ int tmd_system_descriptors(tmd_system *s, float *G, float *dEdG)
   int gw = s->ctx->model.gwidth();
   int nr = s->w.rl.nrows;
   std::vector<float> g((size_t) nr * gw), dg((size_t) nr * gw);
-  hipError_t e = hipMemcpy(g.data(), s->w.G.p, 4 * g.size(), hipMemcpyDeviceToHost);
+  hipError_t e;
+  if (s->w.g_bf16) {  // BF16 rows: widen (exact)
+    std::vector<uint16_t> gb(g.size());
+    e = hipMemcpy(gb.data(), s->w.G.p, 2 * gb.size(), hipMemcpyDeviceToHost);
+    for (size_t q = 0; q < gb.size(); q++) {
+      const uint32_t bits = (uint32_t) gb[q] << 16;
+      std::memcpy(&g[q], &bits, 4);
+    }
+  } else {
+    e = hipMemcpy(g.data(), s->w.G.p, 4 * g.size(), hipMemcpyDeviceToHost);
+  }
   if (e == hipSuccess) e = hipMemcpy(dg.data(), s->w.dEdG.p, 4 * dg.size(), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(s->ctx, e, "descriptor export");
   for (int r = 0; r < nr; r++) {

@@ -685,6 +685,40 @@ def test_bf16_mlp_error(synth_models):
     assert e["dF"] <= 5e-3
 
 
+def _bf16_rne(a):
+    """FP32 -> BF16 (round to nearest even) -> FP32, for finite values."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def test_bf16_descriptor_rows(synth_models):
+    """The BF16 path of a benchmark-shaped network (384-256-128-64-1, tanh)
+    stores its descriptor rows as BF16: the exported G equals the FP32
+    path's G rounded to nearest even, element for element (and differs from
+    the unrounded rows, so the BF16 rows were taken); the forces stay
+    within the BF16 bar of the oracle."""
+    say("case: BF16 descriptor rows, C2 model, bcc 6x6x6")
+    x, t, cell = W.crystal("bcc", 3.2, (6, 6, 6), nelt=4, sigma=0.08, seed=31)
+    syms = ["Mo", "Nb", "Ta", "W"]
+    m32 = TensorMD(synth_models["c2"], syms, "fp32", 0.001)
+    m16 = TensorMD(synth_models["c2"], syms, "bf16", 0.001)
+    width = m32.info()["width"]
+    s32 = m32.system(x, t, cell, skin=1.0)
+    s32.compute()
+    G32, _ = s32.descriptors(width)
+    s16 = m16.system(x, t, cell, skin=1.0)
+    got = s16.compute()
+    G16, dG16 = s16.descriptors(width)
+    rounded = _bf16_rne(G32)
+    say("G rows differing from their BF16 rounding:", int((rounded != G32).sum()), "of", G32.size)
+    assert not np.array_equal(rounded, G32)
+    assert np.array_equal(G16, rounded)
+    assert np.isfinite(dG16).all()
+    ref = oracle.OracleModel(synth_models["c2"], 0.001).compute_periodic(x, t - 1, cell)
+    assert _errs(got, ref, len(x))["dF"] <= 5e-3
+
+
 def test_bf16x3_mlp_error(synth_models):
     """Split-BF16 MLP (operands as hi + lo BF16 pairs, three BF16 MFMA
     products per product): the C2 model on a jittered bcc 8x8x8 lattice

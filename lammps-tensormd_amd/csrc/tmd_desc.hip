@@ -261,12 +261,13 @@ __device__ inline void accumulate_P(const ModelGeo &g, const Box &cell, const do
   }
 }
 
-// G[b0 + b][k][m] = sum_d T[m][d] P^2 (m >= 1) for the nsl slots of a pass:
-// partial sums over the lane's d block, then a sum over the NDG lanes that
-// share k; G[b][k][0] = P0 (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 
+// G[b0 + b][k][m] = sum_d T[m][d] P^2 (m >= 1) for the nsl slots of a pass:
+// partial sums over the lane's d block, then a sum over the NDG lanes that
+// share k; G[b][k][0] = P0 (= sign(P0) sqrt(P0^2)), held by the dq == 0 lane.
+// gbf16: the row is stored as BF16 (round to nearest even).
 template <int K, int NS>
 __device__ inline void write_G(const ModelGeo &g, float (*acc)[Tiling<K>::KT][Tiling<K>::DT],
                                float *Grow, int b0, int nsl, bool gbf16)

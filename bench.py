@@ -43,7 +43,7 @@ KB_EV = 8.617343e-5          # Boltzmann constant, eV/K
 MVV2E = 1.0364269e-4         # (g/mol) (A/ps)^2 -> eV
 
 
-def per_atom_counts(c, *, d, k, b, m, gw, sizes):
+def per_atom_counts(c, *, d, k, b, m, gw, sizes, g_bytes=4):
     """Algorithmic FLOPs and compulsory HBM bytes per atom of each kernel.
 
     c = mean neighbors within rc, d = 20 monomials, k radial functions,
@@ -52,16 +52,17 @@ def per_atom_counts(c, *, d, k, b, m, gw, sizes):
     (19 c k); G from P is small.  force: P again and U, V (2 x 2 c d k),
     per-neighbor chain rule c (2 k + 2 d + 6), S from dE/dG 2 b k d (m + 1).
     MLP: forward + input gradient 4 sum n_l n_{l+1}.
-    bytes: list entries within rc (4 c), x + type (28), G write or dE/dG
-    read (4 gw), force write (24)."""
+    bytes: list entries within rc (4 c), x + type (28), G write or read
+    (g_bytes gw: 4 for FP32 rows, 2 where the BF16 path stores them as BF16),
+    dE/dG write or read (4 gw), force write (24)."""
     mlp = 4 * sum(a * b_ for a, b_ in zip(sizes[:-1], sizes[1:]))
     return dict(
         descriptor_flop=2 * c * d * k + 19 * c * k,
-        descriptor_bytes=4 * c + 28 + 4 * gw,
+        descriptor_bytes=4 * c + 28 + g_bytes * gw,
         force_flop=2 * (2 * c * d * k) + c * (2 * k + 2 * d + 6) + 2 * b * k * d * (m + 1),
         force_bytes=4 * c + 28 + 4 * gw + 24,
         mlp_flop=mlp,
-        mlp_bytes=8 * gw + 8,
+        mlp_bytes=(g_bytes + 4) * gw + 8,
     )
 
 
@@ -446,13 +447,16 @@ def main():
         timing = "HIP events around the kernels of the profile steps after the timed run"
     # work per launch at the sampled steps: their mean kept-neighbor count
     c = kt["kept_per_atom"]
-    per = per_atom_counts(c, d=20, k=24, b=4, m=3, gw=384, sizes=sizes)
+    per = per_atom_counts(c, d=20, k=24, b=4, m=3, gw=384, sizes=sizes,
+                          g_bytes=2 if head == "bf16" else 4)
     dom = max(["descriptor", "mlp", "force"], key=lambda k: kt[k])
 
     def kernel_row(kname, prec, ms, c_kept):
         """Roofline row of one kernel: algorithmic FLOP/s against its
         arithmetic roof and algorithmic bytes/s against HBM."""
-        pk = per_atom_counts(c_kept, d=20, k=24, b=4, m=3, gw=384, sizes=sizes)
+        # the BF16 path of this tanh benchmark network stores G as BF16
+        pk = per_atom_counts(c_kept, d=20, k=24, b=4, m=3, gw=384, sizes=sizes,
+                             g_bytes=2 if prec == "bf16" else 4)
         t_s = ms * 1e-3
         flops = pk[kname + "_flop"] * natoms / t_s / 1e12
         gbs = pk[kname + "_bytes"] * natoms / t_s / 1e9

@@ -2,7 +2,7 @@
 atom_chain_emul): the same chain with chosen stages in FP32.
 
 It prices the rounding floor of an FP32 evaluation, against which the GPU
-FP32 path's errors on the reference's trained FP64 models are judged
+FP32 path's errors on the reference's trained FP64 models are measured
 (tests/test_gpu_parity.py).  Pinned here by:
   * with every stage in FP64 it equals the FP64 chain (energies, forces,
     virial, descriptor rows and dE/dG) to rounding, so it runs the same
